@@ -1,0 +1,4 @@
+"""pyjjasim_amd: MI355X-native time evolution of Josephson-junction arrays."""
+from .graph import *  # noqa: F401,F403
+from .circuit import *  # noqa: F401,F403
+from .time_evolution import *  # noqa: F401,F403

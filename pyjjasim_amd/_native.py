@@ -1,0 +1,110 @@
+"""ctypes binding of libjja (include/jja.h).
+
+The library is built in-tree (pyjjasim_amd/libjja.so) by ``_build.build_library``.
+There is no fallback: if the library or a HIP device is missing, the GPU entry
+points raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjja.so")
+
+JJA_OK = 0
+JJA_EINVAL = -1
+JJA_ENOMEM = -2
+JJA_EFACTOR = -3
+JJA_EHIP = -4
+JJA_EUNSUPPORTED = -5
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("n_junctions", C.c_int32), ("n_faces", C.c_int32),
+        ("stencil_width", C.c_int32), ("n_stages", C.c_int32),
+        ("face_ptr", i32p), ("face_code", i32p), ("junc_code", i32p),
+        ("Ic", f64p), ("Rv", f64p), ("coef", f64p),
+        ("face_pos", i32p), ("stage_ptr", i32p),
+        ("lo_ptr", i64p), ("lo_col", i32p), ("lo_val", f64p),
+        ("up_ptr", i64p), ("up_col", i32p), ("up_val", f64p),
+        ("row_block", i32p), ("row_bsize", i32p), ("row_boff", i64p),
+        ("n_inv", C.c_int64), ("inv_cm", f64p), ("inv_rm", f64p),
+    ]
+
+
+class Source(C.Structure):
+    _fields_ = [
+        ("base", C.c_void_p), ("s_i", C.c_int64), ("s_w", C.c_int64), ("s_t", C.c_int64),
+        ("amp", C.c_void_p), ("a_w", C.c_int64), ("a_t", C.c_int64), ("t0", C.c_int64),
+        ("zero", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class ProblemRun(C.Structure):
+    _fields_ = [
+        ("W", C.c_int32), ("column_offset", C.c_int32), ("dt", C.c_double),
+        ("theta", C.c_void_p), ("theta_s", C.c_void_p),
+        ("Is", Source), ("Vs", Source), ("f", Source), ("T", Source),
+        ("noise", C.c_void_p), ("noise_t0", C.c_int64), ("seed", C.c_uint64),
+        ("th_slot", C.c_void_p), ("I_slot", C.c_void_p), ("slot_t0", C.c_int64),
+        ("th_out", C.c_void_p), ("I_out", C.c_void_p),
+    ]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+EXPORTS = ["jja_cholesky", "jja_factor_get", "jja_factor_free", "jja_factor_last_error",
+           "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes", "jja_evolve",
+           "jja_vortex", "jja_last_error"]
+
+
+def lib():
+    """Load libjja.so once and declare the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError(f"{LIB_PATH} is not built; run pyjjasim_amd._build.build_library()")
+    L = C.CDLL(LIB_PATH)
+    L.jja_cholesky.argtypes = [C.c_int32, i64p, i32p, f64p, C.POINTER(C.c_void_p), i64p]
+    L.jja_factor_get.argtypes = [C.c_void_p, i64p, i32p, f64p]
+    L.jja_factor_free.argtypes = [C.c_void_p]
+    L.jja_factor_free.restype = None
+    L.jja_factor_last_error.restype = C.c_char_p
+    L.jja_plan_create.argtypes = [C.c_int32, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
+    L.jja_plan_destroy.argtypes = [C.c_void_p]
+    L.jja_plan_lds_bytes.argtypes = [C.c_void_p, C.c_int32]
+    L.jja_plan_lds_bytes.restype = C.c_int64
+    L.jja_evolve.argtypes = [C.c_void_p, C.POINTER(ProblemRun), C.c_int32, C.c_int64, C.c_int32,
+                             C.c_void_p]
+    L.jja_vortex.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    L.jja_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, factor=False):
+    """Map a libjja status code to the exception the reference raises for the
+    same class of problem (ValueError for bad input, RuntimeError otherwise)."""
+    if rc == JJA_OK:
+        return
+    msg = (lib().jja_factor_last_error() if factor else lib().jja_last_error()).decode()
+    if rc == JJA_EINVAL:
+        raise ValueError(msg)
+    if rc == JJA_EFACTOR:
+        raise ValueError(msg)
+    raise NativeError(f"libjja error {rc}: {msg}")
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

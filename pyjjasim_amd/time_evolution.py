@@ -1,0 +1,665 @@
+"""Drop-in TimeEvolutionProblem / TimeEvolutionResult running on the GPU.
+
+Same constructor, input semantics, storage rules, results and exceptions as the
+reference (time_evolution.py:20-1067). ``compute()`` keeps the reference's
+pre/post-processing in Python (``time_evolution``, time_evolution.py:422-458) and
+replaces its core loop (``time_evolution_core``, :461-582) by libjja's persistent
+HIP kernel: one workgroup per problem column integrates a chunk of steps, the face
+system is solved with the staged Cholesky schedule, and vortex configurations of
+stored steps are counted on the device.
+
+Extensions (keyword-only, defaults keep reference behaviour):
+  noise="philox" | "reference"
+      "philox": seeded counter-based Gaussian stream computed in the kernel
+      (reproducible for a given ``seed``; statistically equal to the reference's
+      draws, eta = sqrt(2 T Rv) Z);
+      "reference": the host replays the reference's legacy numpy RNG call sequence
+      (np.random.randn / permutation, time_evolution.py:533-537) and streams the
+      draws to the device, giving the reference's exact trajectories at T > 0.
+  seed=None        Philox key; None draws one from numpy's global RNG.
+  steps_per_launch Steps integrated per kernel launch (default: whole run in
+                   launches of at most 2000 steps).
+  block_size       Threads per workgroup (64..1024; default chosen by size).
+  leaf_size        Largest leaf block of the nested-dissection factor schedule.
+
+There is no CPU fallback: without a HIP device ``compute()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .circuit import Circuit, DefaultCPR
+from . import _native
+from .solver_plan import CircuitPlan, STENCILS
+
+__all__ = ["TimeEvolutionProblem", "TimeEvolutionResult", "ThetaNotStored", "CurrentNotStored",
+           "VoltageNotStored", "DataAtTimepointNotStored"]
+
+
+class ThetaNotStored(Exception):
+    pass
+
+
+class CurrentNotStored(Exception):
+    pass
+
+
+class VoltageNotStored(Exception):
+    pass
+
+
+class DataAtTimepointNotStored(Exception):
+    pass
+
+
+def _prob_cnt(x):
+    s = np.array(x(0) if callable(x) else x).shape
+    return s[1] if len(s) > 1 else 1
+
+
+class TimeEvolutionProblem:
+    """Time evolution of W problems on one circuit (time_evolution.py:20-149).
+
+    Parameters are those of the reference: time_step (dt), time_step_count (Nt),
+    current_phase_relation, external_flux (f, (Nf,W,Nt)-broadcastable or
+    callable i -> (Nf,W)), current_sources (Is), voltage_sources (Vs), temperature
+    (T) (each (Nj,W,Nt)-broadcastable or callable), store_time_steps (None, index
+    array or (Nt,) mask), store_theta/voltage/current, config_at_minus_1..4,
+    stencil_width (3, 4 or 5).
+    """
+
+    def __init__(self, circuit: Circuit, time_step=0.05, time_step_count=1000,
+                 current_phase_relation=None, external_flux=0.0, current_sources=0.0,
+                 voltage_sources=0.0, temperature=0.0, store_time_steps=None, store_theta=True,
+                 store_voltage=True, store_current=True, config_at_minus_1=None,
+                 config_at_minus_2=None, config_at_minus_3=None, config_at_minus_4=None,
+                 stencil_width=3, *, noise="philox", seed=None, steps_per_launch=None,
+                 block_size=None, leaf_size=16):
+        self.circuit = circuit
+        self.time_step = time_step
+        self.time_step_count = time_step_count
+        self.current_phase_relation = DefaultCPR() if current_phase_relation is None \
+            else current_phase_relation
+        self.problem_count = max(_prob_cnt(external_flux), _prob_cnt(current_sources),
+                                 _prob_cnt(voltage_sources), _prob_cnt(temperature))
+        Nj, Nf, W, Nt = circuit._Nj(), circuit._Nf(), self.problem_count, time_step_count
+        self._f_is_timedep = self._is_timedep(external_flux)
+        self._Is_is_timedep = self._is_timedep(current_sources)
+        self._Vs_is_timedep = self._is_timedep(voltage_sources)
+        self._T_is_timedep = self._is_timedep(temperature)
+        bc = lambda x, n: x if callable(x) else np.broadcast_to(np.array(x), (n, W, Nt))
+        # un-broadcast inputs: what is uploaded to the device (with zero strides)
+        self._raw = {k: (v if callable(v) else np.array(v)) for k, v in
+                     (("f", external_flux), ("Is", current_sources), ("Vs", voltage_sources),
+                      ("T", temperature))}
+        self.external_flux = bc(external_flux, Nf)
+        self.current_sources = bc(current_sources, Nj)
+        self.voltage_sources = bc(voltage_sources, Nj)
+        self.temperature = bc(temperature, Nj)
+        self.store_time_steps = np.ones(Nt, dtype=bool)
+        self.store_time_steps = self._to_time_point_mask(store_time_steps)
+        self.store_theta = store_theta
+        self.store_voltage = store_voltage
+        self.store_current = store_current
+        if not (store_theta or store_voltage or store_current):
+            raise ValueError("No output is stored")
+        if np.sum(self.store_time_steps) == 0:
+            raise ValueError("No output is stored")
+        self.stencil_width = stencil_width
+        self.stencil = self._get_stencil(stencil_width)
+        self.config_at_minus_1 = self._get_config(config_at_minus_1, np.zeros((Nj, W)), (Nj, W))
+        self.config_at_minus_2 = self._get_config(config_at_minus_2, self.config_at_minus_1, (Nj, W))
+        if stencil_width >= 4:
+            self.config_at_minus_3 = self._get_config(config_at_minus_3, self.config_at_minus_2, (Nj, W))
+        if stencil_width >= 5:
+            self.config_at_minus_4 = self._get_config(config_at_minus_4, self.config_at_minus_3, (Nj, W))
+        if noise not in ("philox", "reference"):
+            raise ValueError("noise must be 'philox' or 'reference'")
+        self.noise = noise
+        self.seed = seed
+        self.steps_per_launch = steps_per_launch
+        self.block_size = block_size
+        self.leaf_size = leaf_size
+
+    # ------------------------------------------------------------- getters
+    def get_problem_count(self):
+        return self.problem_count
+
+    def get_circuit(self) -> Circuit:
+        return self.circuit
+
+    def get_time_step(self):
+        return self.time_step
+
+    def get_time_step_count(self):
+        return self.time_step_count
+
+    def get_current_phase_relation(self):
+        return self.current_phase_relation
+
+    def get_phase_zone(self):
+        return 0
+
+    def get_external_flux(self):
+        return self.external_flux
+
+    def get_current_sources(self):
+        return self.current_sources
+
+    def get_voltage_sources(self):
+        return self.voltage_sources
+
+    def get_temperature(self):
+        return self.temperature
+
+    def get_store_time_steps(self):
+        return self.store_time_steps
+
+    def get_store_theta(self):
+        return self.store_theta
+
+    def get_store_voltage(self):
+        return self.store_voltage
+
+    def get_store_current(self):
+        return self.store_current
+
+    def get_time(self):
+        return np.arange(self._Nt(), dtype=np.double) * self._dt()
+
+    def get_net_sourced_current(self, time_step):
+        M = self.get_circuit().get_cut_matrix()
+        return 0.5 * np.sum(np.abs(M @ self._Is(time_step)), axis=0)
+
+    def get_node_current_sources(self, time_step):
+        return self.get_circuit().get_cut_matrix() @ self._Is(time_step)
+
+    def compute(self) -> "TimeEvolutionResult":
+        """Run the time evolution on the GPU (time_evolution.py:303-307)."""
+        return time_evolution(self)
+
+    def __str__(self):
+        return (f"time evolution problem: {self.time_step_count} steps of {self.time_step}, "
+                f"W={self.problem_count}, circuit {self.circuit}")
+
+    # ------------------------------------------------------------- helpers
+    def _Nt(self):
+        return self.time_step_count
+
+    def _Nt_s(self):
+        return int(np.sum(self.store_time_steps))
+
+    def _dt(self):
+        return self.time_step
+
+    @staticmethod
+    def _get_config(cur, prev, shape):
+        cur = prev.copy() if cur is None else cur
+        if hasattr(cur, "get_theta"):
+            cur = cur.get_theta()
+        return np.asarray(cur, dtype=np.double).reshape(shape)
+
+    @staticmethod
+    def _is_timedep(x):
+        if callable(x):
+            return True
+        s = np.array(x).shape
+        return len(s) > 0 and s[-1] > 1
+
+    def _slice(self, x, n, t):
+        if callable(x):
+            return np.broadcast_to(x(t), (n, self.problem_count))
+        return x[:, :, t]
+
+    def _f(self, t):
+        return self._slice(self.external_flux, self.circuit._Nf(), t)
+
+    def _Is(self, t):
+        return self._slice(self.current_sources, self.circuit._Nj(), t)
+
+    def _Vs(self, t):
+        return self._slice(self.voltage_sources, self.circuit._Nj(), t)
+
+    def _T(self, t):
+        return self._slice(self.temperature, self.circuit._Nj(), t)
+
+    def _cp(self, theta):
+        return self.current_phase_relation.eval(self.circuit._Ic()[:, None], theta)
+
+    def _dcp(self, theta):
+        return self.current_phase_relation.d_eval(self.circuit._Ic()[:, None], theta)
+
+    def _icp(self, theta):
+        return self.current_phase_relation.i_eval(self.circuit._Ic()[:, None], theta)
+
+    def _to_time_point_mask(self, time_points):
+        if time_points is None:
+            time_points = self.store_time_steps
+        time_points = np.array(time_points)
+        if time_points.dtype != bool:
+            try:
+                x = np.zeros(self._Nt(), dtype=bool)
+                x[time_points] = True
+                time_points = x
+            except (IndexError, TypeError, ValueError):
+                raise ValueError("Invalid store_time_steps; must be None, mask, slice or index array")
+        return time_points
+
+    def _get_stencil(self, width):
+        if width not in STENCILS:
+            raise ValueError(f"stencil width must be 3, 4 or 5 (equals {width})")
+        return STENCILS[width]
+
+
+def _apply_derivative(x, index, stencil, dt):
+    acc = stencil[0] * x[:, :, index] + stencil[1] * x[:, :, index - 1]
+    if len(stencil) >= 4:
+        acc = acc + stencil[2] * x[:, :, index - 2]
+    if len(stencil) >= 5:
+        acc = acc + stencil[3] * x[:, :, index - 3]
+    return acc / dt
+
+
+def time_evolution(problem: TimeEvolutionProblem):
+    """Storage masks, GPU core, voltage post-processing (time_evolution.py:422-458)."""
+    Nt = problem._Nt()
+    th_mask = problem.store_time_steps.copy() if (problem.store_theta or problem.store_voltage) \
+        else np.zeros(Nt, dtype=bool)
+    I_mask = problem.store_time_steps.copy() if (problem.store_current or problem.store_voltage) \
+        else np.zeros(Nt, dtype=bool)
+    Vth, VI = th_mask.copy(), I_mask.copy()
+    t_ids = np.flatnonzero(problem.store_time_steps)
+    Nj = problem.circuit._Nj()
+    off = problem.stencil_width - 1
+    if problem.store_voltage and len(t_ids):
+        ids = (t_ids[:, None] - np.arange(off)).ravel()
+        ids = ids[(ids >= 0) & (ids < Nt)]
+        Vth[ids] = True
+        if problem.circuit._has_inductance():
+            VI[ids] = True
+    th_out, I_out, n_out = _gpu_core(problem, Vth, VI)
+    V_out = None
+    if problem.store_voltage:
+        ts = np.flatnonzero(problem.store_time_steps[Vth])
+        V_out = _apply_derivative(th_out, ts + off, problem.stencil[0], problem._dt())
+        if problem.circuit._has_inductance():
+            V_ind = _apply_derivative(I_out, ts + off, problem.stencil[0], problem._dt())
+            V_out += (problem.circuit.get_inductance() @ V_ind.reshape((Nj, -1))).reshape(V_out.shape)
+        drop_th = np.flatnonzero((Vth & ~th_mask)[Vth])
+        th_out = np.delete(th_out, drop_th + off, axis=2)
+        I_out = np.delete(I_out, np.flatnonzero((VI & ~I_mask)[VI]) + off, axis=2)
+        if n_out is not None:
+            n_out = np.delete(n_out, drop_th, axis=2)
+    th_out = th_out[:, :, off:]
+    I_out = I_out[:, :, off:]
+    return TimeEvolutionResult(problem, th_out if problem.store_theta else None,
+                               I_out if problem.store_current else None,
+                               V_out if problem.store_voltage else None,
+                               vortex=n_out if problem.store_theta else None)
+
+
+# ---------------------------------------------------------------- GPU core
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyjjasim_amd: TimeEvolutionProblem.compute() needs a HIP (ROCm) "
+                           "device; none is visible")
+    return torch
+
+
+class _DeviceSource:
+    """Device copy of one input plus its jja_source descriptor."""
+
+    def __init__(self, torch, value, n, W, Nt, zero, chunk=False):
+        self.callable = callable(value)
+        self.value = value
+        self.n, self.W = n, W
+        self.src = _native.Source()
+        self.src.zero = int(bool(zero))
+        self.tensor = None
+        if not self.callable:
+            arr = np.ascontiguousarray(np.array(value, dtype=np.float64))
+            view = np.broadcast_to(arr, (n, W, Nt))
+            strides = [s // 8 for s in view.strides]
+            self.tensor = torch.from_numpy(arr.reshape(-1) if arr.size else np.zeros(1)).cuda()
+            self.src.base = self.tensor.data_ptr()
+            self.src.s_i, self.src.s_w, self.src.s_t = strides
+            self.src.t0 = 0
+
+    def fill_chunk(self, torch, t0, n_steps):
+        """Evaluate a callable input for steps t0..t0+n_steps-1 into a device buffer
+        laid out [t][w][i]."""
+        if not self.callable:
+            return
+        buf = np.empty((n_steps, self.W, self.n))
+        for k in range(n_steps):
+            buf[k] = np.broadcast_to(self.value(t0 + k), (self.n, self.W)).T
+        self.tensor = torch.from_numpy(buf.reshape(-1) if buf.size else np.zeros(1)).cuda()
+        self.src.base = self.tensor.data_ptr()
+        self.src.s_i, self.src.s_w, self.src.s_t = 1, self.n, self.n * self.W
+        self.src.t0 = t0
+
+
+def _zero_flag(problem, name, sample, timedep):
+    return (not timedep) and bool(np.allclose(sample, 0))
+
+
+def _reference_noise(Nj, W):
+    """Replays the reference's legacy-RNG draw order (time_evolution.py:533-537)."""
+    state = {}
+
+    def draw(i):
+        if Nj > 500:
+            if i % 3 == 0:
+                state["r"] = np.random.randn(Nj, W)
+            else:
+                state["r"] = state["r"][np.random.permutation(Nj), :]
+        else:
+            state["r"] = np.random.randn(Nj, W)
+        return state["r"]
+    return draw
+
+
+def _default_block_size(Nj, Nf):
+    work = max(Nj, Nf)
+    for bs in (64, 128, 256, 512):
+        if work <= 2 * bs:
+            return bs
+    return 1024
+
+
+def _gpu_core(problem: TimeEvolutionProblem, th_mask, I_mask):
+    """Replacement of time_evolution_core: returns (th_out, I_out, n_out) with the
+    reference layout (Nj, W, prefix + stored) and vortex counts of stored theta."""
+    if not isinstance(problem.current_phase_relation, DefaultCPR):
+        raise ValueError("the GPU time evolution evaluates the default current-phase relation "
+                         "Ic*sin(theta) only; got a custom CurrentPhaseRelation")
+    torch = _torch()
+    circuit = problem.circuit
+    Nj, Nf, W, Nt = circuit._Nj(), circuit._Nf(), problem.problem_count, problem._Nt()
+    dt = problem._dt()
+    sw = problem.stencil_width
+    S = sw - 1
+    plan_host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size)
+    lib = _native.lib()
+    desc = plan_host.desc()
+    handle = C.c_void_p()
+    _native.check(lib.jja_plan_create(1, C.byref(desc), C.byref(handle)))
+    try:
+        if lib.jja_plan_lds_bytes(handle, 0) == 0:
+            raise _native.NativeError(
+                f"circuit with {Nf} faces needs {plan_host.lds_bytes()} bytes of LDS per column; "
+                "larger than one workgroup's LDS")
+        dev = torch.device("cuda")
+        f64 = torch.float64
+        # history slots: theta at step m in slot m mod S; initial configs at m = -1..-S
+        cfgs = [problem.config_at_minus_1, problem.config_at_minus_2]
+        if sw >= 4:
+            cfgs.append(problem.config_at_minus_3)
+        if sw >= 5:
+            cfgs.append(problem.config_at_minus_4)
+        hist = np.zeros((S, W, Nj))
+        for k, cfg in enumerate(cfgs):
+            hist[(-(k + 1)) % S] = np.asarray(cfg, dtype=np.float64).reshape(Nj, W).T
+        theta = torch.from_numpy(hist).to(dev)
+        # inputs and the reference's zero flags (time_evolution.py:509-519)
+        srcs = {}
+        for name, val, n, td in (("Is", problem.current_sources, Nj, problem._Is_is_timedep),
+                                 ("Vs", problem.voltage_sources, Nj, problem._Vs_is_timedep),
+                                 ("f", problem.external_flux, Nf, problem._f_is_timedep),
+                                 ("T", problem.temperature, Nj, problem._T_is_timedep)):
+            sample = np.broadcast_to(val(0), (n, W)) if callable(val) else val[:, :, 0]
+            zero = _zero_flag(problem, name, sample, td)
+            srcs[name] = _DeviceSource(torch, problem._raw[name], n, W, Nt, zero)
+        theta_s = torch.zeros((W, Nj), dtype=f64, device=dev) if not srcs["Vs"].src.zero else None
+        # storage slots
+        th_slot = np.where(th_mask, np.cumsum(th_mask) - 1, -1).astype(np.int32)
+        I_slot = np.where(I_mask, np.cumsum(I_mask) - 1, -1).astype(np.int32)
+        n_th, n_I = int(th_mask.sum()), int(I_mask.sum())
+        th_slot_d = torch.from_numpy(th_slot).to(dev)
+        I_slot_d = torch.from_numpy(I_slot).to(dev)
+        th_out = torch.empty((max(n_th, 1), W, Nj), dtype=f64, device=dev)
+        I_out = torch.empty((max(n_I, 1), W, Nj), dtype=f64, device=dev)
+
+        run = _native.ProblemRun()
+        run.W = W
+        run.column_offset = 0
+        run.dt = dt
+        run.theta = theta.data_ptr()
+        run.theta_s = theta_s.data_ptr() if theta_s is not None else None
+        run.th_slot = th_slot_d.data_ptr()
+        run.I_slot = I_slot_d.data_ptr()
+        run.slot_t0 = 0
+        run.th_out = th_out.data_ptr()
+        run.I_out = I_out.data_ptr()
+        T_zero = srcs["T"].src.zero
+        ref_noise = (not T_zero) and problem.noise == "reference"
+        if not T_zero and problem.noise == "philox":
+            run.seed = int(problem.seed) if problem.seed is not None \
+                else int(np.random.randint(0, 2 ** 62, dtype=np.int64))
+        draw = _reference_noise(Nj, W) if ref_noise else None
+        any_callable = any(s.callable for s in srcs.values())
+        spl = problem.steps_per_launch
+        if spl is None:
+            spl = 2000 if not (any_callable or ref_noise) else 500
+        bs = problem.block_size or _default_block_size(Nj, Nf)
+        stream = torch.cuda.current_stream().cuda_stream
+        t0 = 0
+        noise_buf = None
+        while t0 < Nt:
+            n_steps = min(spl, Nt - t0)
+            for s in srcs.values():
+                s.fill_chunk(torch, t0, n_steps)
+            if ref_noise:
+                z = np.empty((n_steps, W, Nj))
+                for k in range(n_steps):
+                    z[k] = draw(t0 + k).T
+                noise_buf = torch.from_numpy(z.reshape(-1)).to(dev)
+                run.noise = noise_buf.data_ptr()
+                run.noise_t0 = t0
+            run.Is, run.Vs, run.f, run.T = (srcs[k].src for k in ("Is", "Vs", "f", "T"))
+            _native.check(lib.jja_evolve(handle, C.byref(run), n_steps, t0, bs, stream))
+            if any_callable or ref_noise:
+                torch.cuda.current_stream().synchronize()   # chunk buffers are reused
+            t0 += n_steps
+        n_dev = None
+        if n_th and Nf:
+            n_dev = torch.empty((n_th, W, Nf), dtype=torch.int32, device=dev)
+            _native.check(lib.jja_vortex(handle, 0, th_out.data_ptr(), n_th * W,
+                                         n_dev.data_ptr(), stream))
+        torch.cuda.synchronize()
+        th_host = th_out[:n_th].cpu().numpy().transpose(2, 1, 0) if n_th else np.zeros((Nj, W, 0))
+        I_host = I_out[:n_I].cpu().numpy().transpose(2, 1, 0) if n_I else np.zeros((Nj, W, 0))
+        n_host = None
+        if n_dev is not None:
+            n_host = n_dev.cpu().numpy().astype(np.int64).transpose(2, 1, 0)
+        elif n_th:
+            n_host = np.zeros((0, W, n_th), dtype=np.int64)
+    finally:
+        lib.jja_plan_destroy(handle)
+    # prefix of initial conditions (time_evolution.py:484-502)
+    pre = np.stack([np.asarray(cfgs[S - 1 - k]).reshape(Nj, W) for k in range(S)], axis=2)
+    Ic = circuit._Ic()[:, None]
+    pre_I = np.stack([problem.current_phase_relation.eval(Ic, pre[:, :, k]) for k in range(S)], axis=2)
+    th_full = np.concatenate((pre, th_host), axis=2)
+    I_full = np.concatenate((pre_I, I_host), axis=2)
+    return th_full, I_full, n_host
+
+
+class TimeEvolutionResult:
+    """Stored theta, current and voltage of a time evolution (time_evolution.py:597-1067)."""
+
+    def __init__(self, problem: TimeEvolutionProblem, theta, current, voltage, vortex=None):
+        self.problem = problem
+        Nj, W, Nt_s = problem.circuit._Nj(), problem.get_problem_count(), problem._Nt_s()
+        for name, val, flag in (("theta", theta, problem.store_theta),
+                                ("current", current, problem.store_current),
+                                ("voltage", voltage, problem.store_voltage)):
+            if flag and val.shape != (Nj, W, Nt_s):
+                raise ValueError(f"{name} must have shape {(Nj, W, Nt_s)}; has shape {val.shape}")
+        self.theta = theta if problem.store_theta else None
+        self.current = current if problem.store_current else None
+        self.voltage = voltage if problem.store_voltage else None
+        self._vortex = vortex
+        s = problem.store_time_steps.astype(int)
+        self.time_point_indices = np.cumsum(s) - s
+        self.animation = None
+
+    def _time_point_index(self, time_points):
+        if time_points is None:
+            time_points = self.problem.store_time_steps
+        if not np.all(self.problem.store_time_steps[time_points]):
+            raise DataAtTimepointNotStored("Queried a timepoint that is not stored during time "
+                                           "evolution simulation.")
+        return self.time_point_indices[time_points]
+
+    def _th(self, tp):
+        if self.theta is None:
+            raise ThetaNotStored("Cannot query theta; quantity is not stored during time "
+                                 "evolution simulation.")
+        return self.theta[:, :, self._time_point_index(tp)]
+
+    def _V(self, tp):
+        if self.voltage is None:
+            raise VoltageNotStored("Cannot query voltage; quantity is not stored during time "
+                                   "evolution simulation.")
+        return self.voltage[:, :, self._time_point_index(tp)]
+
+    def _I(self, tp):
+        if self.current is None:
+            raise CurrentNotStored("Cannot query current; quantity is not stored during time "
+                                   "evolution simulation.")
+        return self.current[:, :, self._time_point_index(tp)]
+
+    def _select(self, select_time_points, N, func, is_zero=False):
+        tps = np.flatnonzero(self.problem._to_time_point_mask(select_time_points))
+        out = np.zeros((N, self.get_problem_count(), len(tps)), dtype=np.double)
+        if is_zero:
+            return out
+        for i, tp in enumerate(tps):
+            out[:, :, i] = func(tp)
+        return out
+
+    def get_problem_count(self):
+        return self.problem.get_problem_count()
+
+    def get_circuit(self) -> Circuit:
+        return self.problem.get_circuit()
+
+    def get_theta(self, select_time_points=None):
+        return self._select(select_time_points, self.get_circuit()._Nj(), self._th)
+
+    def get_voltage(self, select_time_points=None):
+        return self._select(select_time_points, self.get_circuit()._Nj(), self._V)
+
+    def get_current(self, select_time_points=None):
+        return self._select(select_time_points, self.get_circuit()._Nj(), self._I)
+
+    def get_vortex_configuration(self, select_time_points=None):
+        """n = -A round(theta / 2pi) (time_evolution.py:734-755), counted on the GPU for
+        every stored step."""
+        if self.theta is None:
+            raise ThetaNotStored("Cannot compute n; requires theta to be stored in "
+                                 "TimeEvolutionConfig")
+        tps = np.flatnonzero(self.problem._to_time_point_mask(select_time_points))
+        idx = self._time_point_index(tps)
+        if self._vortex is not None and self._vortex.shape[0] == self.get_circuit()._Nf():
+            return self._vortex[:, :, idx].astype(int)
+        A = self.get_circuit().get_cycle_matrix()
+        return self._select(select_time_points, self.get_circuit()._Nf(),
+                            lambda tp: -A @ np.round(self._th(tp) / (2.0 * np.pi))).astype(int)
+
+    def get_phase(self, select_time_points=None):
+        c = self.get_circuit()
+        M, Nj = c.get_cut_matrix(), c._Nj()
+        try:
+            return self._select(select_time_points, c._Nn(),
+                                lambda tp: c.Msq_solve(M @ self._th(tp).reshape(Nj, -1)))
+        except ThetaNotStored:
+            raise ThetaNotStored("Cannot compute phi; requires theta to be stored in "
+                                 "TimeEvolutionConfig")
+
+    def get_potential(self, select_time_points=None):
+        c = self.get_circuit()
+        M, Nj = c.get_cut_matrix(), c._Nj()
+        try:
+            return self._select(select_time_points, c._Nn(),
+                                lambda tp: c.Msq_solve(M @ self._V(tp).reshape(Nj, -1)))
+        except VoltageNotStored:
+            raise VoltageNotStored("Cannot compute electric potential U; requires voltage to be "
+                                   "stored in TimeEvolutionConfig")
+
+    def get_josephson_energy(self, select_time_points=None):
+        try:
+            return self._select(select_time_points, self.get_circuit()._Nj(),
+                                lambda tp: self.problem._icp(self._th(tp)))
+        except ThetaNotStored:
+            raise ThetaNotStored("Cannot compute Josephson energy EJ; requires theta to be "
+                                 "stored in TimeEvolutionConfig")
+
+    def get_supercurrent(self, select_time_points=None):
+        try:
+            return self._select(select_time_points, self.get_circuit()._Nj(),
+                                lambda tp: self.problem._cp(self._th(tp)))
+        except ThetaNotStored:
+            raise ThetaNotStored("Cannot compute supercurrent Isup; requires theta to be stored "
+                                 "in TimeEvolutionConfig")
+
+    def get_cycle_current(self, select_time_points=None):
+        c = self.get_circuit()
+        A = c.get_cycle_matrix()
+        try:
+            return self._select(select_time_points, c._Nf(),
+                                lambda tp: c.Asq_solve(A @ (self._I(tp) - self.problem._Is(tp))))
+        except CurrentNotStored:
+            raise CurrentNotStored("Cannot compute cycle-current J; requires current to be "
+                                   "stored in TimeEvolutionConfig")
+
+    def get_flux(self, select_time_points=None):
+        c = self.get_circuit()
+        A = c.get_cycle_matrix()
+        try:
+            return self._select(select_time_points, c._Nf(),
+                                lambda tp: self.problem._f(tp) + A @ (c._L() @ self._I(tp)) / (2 * np.pi))
+        except CurrentNotStored:
+            raise CurrentNotStored("Cannot compute magnetic flux; requires current to be stored "
+                                   "in TimeEvolutionConfig")
+
+    def get_magnetic_energy(self, select_time_points=None):
+        c = self.get_circuit()
+        try:
+            return self._select(select_time_points, c._Nj(),
+                                lambda tp: 0.5 * c._L() @ (self._I(tp) ** 2),
+                                is_zero=not c._has_inductance())
+        except CurrentNotStored:
+            raise CurrentNotStored("Cannot compute magnetic energy EM; requires current to be "
+                                   "stored in TimeEvolutionConfig")
+
+    def get_capacitive_energy(self, select_time_points=None):
+        c = self.get_circuit()
+        Cj = c._C()
+        try:
+            return self._select(select_time_points, c._Nj(),
+                                lambda tp: 0.5 * Cj[:, None] * self._V(tp) ** 2,
+                                is_zero=not c._has_capacitance())
+        except VoltageNotStored:
+            raise VoltageNotStored("Cannot compute capacitive energy EC; requires voltage to be "
+                                   "stored in TimeEvolutionConfig")
+
+    def get_energy(self, select_time_points=None):
+        return (self.get_josephson_energy(select_time_points) +
+                self.get_magnetic_energy(select_time_points) +
+                self.get_capacitive_energy(select_time_points))
+
+    def plot(self, *args, **kwargs):
+        raise NotImplementedError("visualisation is not part of pyjjasim_amd")
+
+    def animate(self, *args, **kwargs):
+        raise NotImplementedError("visualisation is not part of pyjjasim_amd")
+
+    def __str__(self):
+        shapes = ", ".join(f"{n}{v.shape}" for n, v in (("th", self.theta), ("I", self.current),
+                                                       ("V", self.voltage)) if v is not None)
+        return f"time evolution configuration: ({shapes})"

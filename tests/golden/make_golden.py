@@ -1,0 +1,289 @@
+"""Write the golden fixtures under tests/golden/ by running the reference package.
+
+This script is the only place the reference code is executed. It runs in the
+build container (never on the GPU machine), imports the pure-Python reference from
+/root/reference under the package name ``pyjjasim`` (via a symlink in a temporary
+directory), and stores plain numpy arrays (inputs and outputs) in ``.npz`` files
+that the tests load with ``allow_pickle=False``.
+
+Bytecode shipped next to the reference sources is never loaded:
+``sys.pycache_prefix`` redirects Python's bytecode cache to a temporary directory,
+so every reference module is compiled from its source text.
+
+Two shims are needed for numpy 2.x:
+  * ``np.asscalar`` (removed in numpy 1.23) is used by
+    ``TimeEvolutionProblem._Nt_s`` (time_evolution.py:321-322);
+  * nothing else.
+
+What is pinned by these fixtures (and not by closed forms in the tests):
+  * face order and orientation of the cycle matrix A for square, honeycomb,
+    triangular lattices, the SQUID and a custom graph with user-reversed edges
+    (embedded_graph.py:518-554, 937-1033);
+  * current_base() for the lattices (josephson_circuit.py:913-998);
+  * end-to-end time-evolution trajectories (theta, current, voltage and vortex
+    configuration at selected steps) for small problems at T=0, and at T>0 with
+    the legacy numpy global RNG seeded before compute() (time_evolution.py:422-582).
+
+Usage:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REFERENCE = "/root/reference"
+
+
+def _import_reference():
+    tmp = tempfile.mkdtemp(prefix="pyjjasim_golden_")
+    os.symlink(REFERENCE, os.path.join(tmp, "pyjjasim"))
+    sys.pycache_prefix = os.path.join(tmp, "pycache")
+    sys.dont_write_bytecode = True
+    os.environ.setdefault("MPLBACKEND", "Agg")
+    if not hasattr(np, "asscalar"):
+        np.asscalar = lambda a: np.asarray(a).item()
+    sys.path.insert(0, tmp)
+    import pyjjasim  # noqa: E402
+    return pyjjasim
+
+
+def _sparse_to_dense(M):
+    return np.asarray(M.todense())
+
+
+def graph_record(pj, name, circuit, lattice=False):
+    x, y = circuit.get_node_coordinates()
+    n1, n2 = circuit.get_junction_nodes()
+    rec = {
+        "x": np.asarray(x, dtype=np.float64),
+        "y": np.asarray(y, dtype=np.float64),
+        "node1": np.asarray(n1, dtype=np.int64),
+        "node2": np.asarray(n2, dtype=np.int64),
+        "A": _sparse_to_dense(circuit.get_cycle_matrix()).astype(np.int8),
+        "M": _sparse_to_dense(circuit.get_cut_matrix()).astype(np.int8),
+        "areas": np.asarray(circuit.get_face_areas(), dtype=np.float64),
+    }
+    cx, cy = circuit.get_face_centroids()
+    rec["centroid_x"] = np.asarray(cx, dtype=np.float64)
+    rec["centroid_y"] = np.asarray(cy, dtype=np.float64)
+    if lattice:
+        rec["cb0"] = np.asarray(circuit.current_base(0.0), dtype=np.float64)
+        rec["cb90"] = np.asarray(circuit.current_base(np.pi / 2), dtype=np.float64)
+        rec["cb30"] = np.asarray(circuit.current_base(np.pi / 6), dtype=np.float64)
+    return rec
+
+
+def make_graphs(pj):
+    out = {}
+    out["sq_3x4"] = graph_record(pj, "sq_3x4", pj.SquareArray(3, 4), lattice=True)
+    out["sq_10x10"] = graph_record(pj, "sq_10x10", pj.SquareArray(10, 10), lattice=True)
+    out["sq_5x3_scaled"] = graph_record(pj, "sq", pj.SquareArray(5, 3, 1.5, 0.5), lattice=True)
+    out["hc_3x3"] = graph_record(pj, "hc", pj.HoneycombArray(3, 3), lattice=True)
+    out["hc_4x2"] = graph_record(pj, "hc", pj.HoneycombArray(4, 2), lattice=True)
+    out["hc_6x6"] = graph_record(pj, "hc", pj.HoneycombArray(6, 6), lattice=True)
+    out["tri_4x3"] = graph_record(pj, "tri", pj.TriangularArray(4, 3), lattice=True)
+    out["tri_3x5"] = graph_record(pj, "tri", pj.TriangularArray(3, 5), lattice=True)
+    out["tri_6x6"] = graph_record(pj, "tri", pj.TriangularArray(6, 6), lattice=True)
+    out["squid"] = graph_record(pj, "squid", pj.SQUID())
+    # custom graph: 3x3 nodes, unsorted edges, several given high->low
+    x = [0, 1, 2, 0, 1, 2, 0, 1, 2]
+    y = [0, 0, 0, 1, 1, 1, 2, 2, 2]
+    n1 = [4, 2, 7, 0, 5, 6, 4, 1, 8, 7, 7]
+    n2 = [3, 1, 6, 3, 2, 3, 5, 0, 5, 4, 8]
+    g = pj.EmbeddedGraph(x, y, n1, n2)
+    out["custom"] = graph_record(pj, "custom", pj.Circuit(g))
+    # the same graph plus a dangling junction inside the top-left face
+    g2 = pj.EmbeddedGraph(x + [0.4], y + [1.6], n1 + [9], n2 + [3])
+    out["custom_dangling"] = graph_record(pj, "custom_dangling", pj.Circuit(g2))
+    # square array with a hole (nodes removed), irregular faces
+    full = pj.SquareArray(8, 8)
+    fx, fy = full.get_node_coordinates()
+    hole = (fx > 2) & (fx < 5) & (fy > 2) & (fy < 5)
+    out["hole_8"] = graph_record(pj, "hole", full.remove_nodes(hole))
+    flat = {}
+    for gname, rec in out.items():
+        for k, v in rec.items():
+            flat[f"{gname}__{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "graphs.npz"), **flat)
+    print("graphs:", sorted(out))
+
+
+def run_case(pj, name, circuit, store, seed=None, **kw):
+    prob = pj.TimeEvolutionProblem(circuit, **kw)
+    if seed is not None:
+        np.random.seed(seed)
+    res = prob.compute()
+    rec = {}
+    if prob.store_theta:
+        rec["theta"] = res.get_theta()
+        rec["n"] = res.get_vortex_configuration().astype(np.int64)
+    if prob.store_current:
+        rec["current"] = res.get_current()
+    if prob.store_voltage:
+        rec["voltage"] = res.get_voltage()
+    rec["store_mask"] = np.asarray(prob.store_time_steps, dtype=bool)
+    for k, v in store.items():
+        rec["in_" + k] = np.asarray(v)
+    return rec
+
+
+def make_trajectories(pj):
+    cases = {}
+    rng = np.random.default_rng(20261202)
+
+    # C1 shape: SquareArray(10,10), f=0.1, Is=0.5*current_base(0), dt=0.05, 10000 steps
+    sq = pj.SquareArray(10, 10)
+    Is = 0.5 * sq.current_base(0.0)[:, None, None]
+    ts = np.array([0, 1, 2, 3, 99, 100, 999, 1000, 4999, 5000, 9998, 9999])
+    cases["c1"] = run_case(pj, "c1", sq, dict(Is=Is, ts=ts, f=0.1, dt=0.05, Nt=10000),
+                           time_step=0.05, time_step_count=10000, external_flux=0.1,
+                           current_sources=Is, store_time_steps=ts)
+
+    # several bias points (W=4) on a small square array, stencil 3, capacitance
+    sq6 = pj.SquareArray(6, 6)
+    sq6.set_capacitance(1.0)
+    amps = np.array([0.3, 0.8, 1.2, 1.7])
+    Is = sq6.current_base(0.0)[:, None, None] * amps[None, :, None]
+    ts = np.arange(0, 800, 37)
+    cases["sq6_w4_c1"] = run_case(pj, "sq6", sq6, dict(Is=Is, ts=ts, f=0.25, C=1.0),
+                                  time_step=0.05, time_step_count=800, external_flux=0.25,
+                                  current_sources=Is, store_time_steps=ts)
+
+    # honeycomb 6x6, W=3
+    hc = pj.HoneycombArray(6, 6)
+    Is = hc.current_base(0.0)[:, None, None] * np.array([0.2, 0.9, 1.5])[None, :, None]
+    ts = np.arange(0, 1000, 50)
+    cases["hc6"] = run_case(pj, "hc6", hc, dict(Is=Is, ts=ts, f=0.05),
+                            time_step=0.05, time_step_count=1000, external_flux=0.05,
+                            current_sources=Is, store_time_steps=ts)
+
+    # triangular 6x6, W=2, vertical bias
+    tri = pj.TriangularArray(6, 6)
+    Is = tri.current_base(np.pi / 2)[:, None, None] * np.array([0.5, 1.4])[None, :, None]
+    ts = np.arange(0, 600, 25)
+    cases["tri6"] = run_case(pj, "tri6", tri, dict(Is=Is, ts=ts, f=0.15),
+                             time_step=0.05, time_step_count=600, external_flux=0.15,
+                             current_sources=Is, store_time_steps=ts)
+
+    # self inductance and capacitance: voltage includes L dI/dt
+    sqL = pj.SquareArray(8, 8)
+    sqL.set_inductance(1.0)
+    sqL.set_capacitance(1.0)
+    Is = sqL.current_base(0.0)[:, None, None] * np.array([0.7, 1.3])[None, :, None]
+    ts = np.arange(0, 1000, 40)
+    cases["sq8_L"] = run_case(pj, "sq8_L", sqL, dict(Is=Is, ts=ts, f=0.1, L=1.0, C=1.0),
+                              time_step=0.05, time_step_count=1000, external_flux=0.1,
+                              current_sources=Is, store_time_steps=ts)
+
+    # mutual inductance: sparse symmetric positive definite L
+    sqM = pj.SquareArray(5, 5)
+    Nj = sqM.junction_count()
+    import scipy.sparse
+    i = np.arange(Nj - 1)
+    Lm = scipy.sparse.diags(np.full(Nj, 0.8)) + scipy.sparse.coo_matrix(
+        (np.full(Nj - 1, 0.1), (i, i + 1)), shape=(Nj, Nj)) + scipy.sparse.coo_matrix(
+        (np.full(Nj - 1, 0.1), (i + 1, i)), shape=(Nj, Nj))
+    Lm = scipy.sparse.csc_matrix(Lm)
+    sqM.set_inductance(Lm)
+    Is = sqM.current_base(0.0)[:, None, None] * np.array([1.1])[None, :, None]
+    ts = np.arange(0, 500, 20)
+    cases["sq5_mutual"] = run_case(pj, "sq5_mutual", sqM,
+                                   dict(Is=Is, ts=ts, f=0.2, L_data=Lm.data, L_indices=Lm.indices,
+                                        L_indptr=Lm.indptr),
+                                   time_step=0.05, time_step_count=500, external_flux=0.2,
+                                   current_sources=Is, store_time_steps=ts)
+
+    # voltage sources on a subset of junctions, W=2
+    sqV = pj.SquareArray(5, 5)
+    Nj = sqV.junction_count()
+    Vs = np.zeros((Nj, 2, 1))
+    Vs[:4, 0, 0] = 0.3
+    Vs[4:8, 1, 0] = -0.2
+    ts = np.arange(0, 400, 10)
+    cases["sq5_vs"] = run_case(pj, "sq5_vs", sqV, dict(Vs=Vs, ts=ts, f=0.0),
+                               time_step=0.05, time_step_count=400, voltage_sources=Vs,
+                               store_time_steps=ts)
+
+    # stencil widths 4 and 5 with capacitance
+    for sw in (4, 5):
+        s = pj.SquareArray(5, 5)
+        s.set_capacitance(0.5)
+        Is = s.current_base(0.0)[:, None, None] * np.array([0.6, 1.4])[None, :, None]
+        ts = np.arange(0, 500, 25)
+        cases[f"sq5_s{sw}"] = run_case(pj, f"sq5_s{sw}", s, dict(Is=Is, ts=ts, f=0.1, C=0.5),
+                                       time_step=0.05, time_step_count=500, external_flux=0.1,
+                                       current_sources=Is, store_time_steps=ts, stencil_width=sw)
+
+    # time-dependent current pulse (Nj,1,Nt) and time-dependent flux (Nf,1,Nt)
+    s = pj.SquareArray(6, 6)
+    Nt = 400
+    t = np.arange(Nt) * 0.05
+    pulse = 2.0 * np.exp(-0.5 * ((t - 8.0) / 1.5) ** 2)
+    Is = s.current_base(0.0)[:, None, None] * pulse[None, None, :]
+    f = (0.1 + 0.05 * np.sin(0.2 * t))[None, None, :] * np.ones((s.face_count(), 1, 1))
+    ts = np.arange(0, Nt, 8)
+    cases["sq6_tdep"] = run_case(pj, "sq6_tdep", s, dict(Is=Is, f=f, ts=ts),
+                                 time_step=0.05, time_step_count=Nt, external_flux=f,
+                                 current_sources=Is, store_time_steps=ts)
+
+    # heterogeneous Ic, R, C and a non-zero initial condition; W=3 per-problem flux
+    s = pj.SquareArray(7, 6)
+    Nj, Nf = s.junction_count(), s.face_count()
+    Ic = np.exp(0.2 * rng.standard_normal(Nj))
+    R = 0.5 + rng.random(Nj)
+    C = 0.3 * rng.random(Nj)
+    s.set_critical_current(Ic)
+    s.set_resistance(R)
+    s.set_capacitance(C)
+    Is = s.current_base(0.0)[:, None, None] * np.array([0.4, 0.9, 1.6])[None, :, None]
+    f = np.array([0.0, 0.1, 0.33])[None, :, None]
+    th1 = 0.3 * rng.standard_normal((Nj, 3))
+    th2 = th1 - 0.01 * rng.standard_normal((Nj, 3))
+    ts = np.arange(0, 500, 20)
+    cases["sq76_hetero"] = run_case(pj, "sq76_hetero", s,
+                                    dict(Is=Is, f=f, ts=ts, Ic=Ic, R=R, C=C, th1=th1, th2=th2),
+                                    time_step=0.05, time_step_count=500, external_flux=f,
+                                    current_sources=Is, store_time_steps=ts,
+                                    config_at_minus_1=th1, config_at_minus_2=th2)
+
+    # T>0 with the reference's legacy global RNG seeded; Nj <= 500 (fresh draw each step)
+    s = pj.SquareArray(6, 6)
+    Is = s.current_base(0.0)[:, None, None] * np.array([0.5, 1.2])[None, :, None]
+    ts = np.arange(0, 200, 5)
+    cases["sq6_T"] = run_case(pj, "sq6_T", s, dict(Is=Is, ts=ts, f=0.1, T=0.05, seed=1234),
+                              seed=1234, time_step=0.05, time_step_count=200, external_flux=0.1,
+                              current_sources=Is, temperature=0.05, store_time_steps=ts)
+
+    # T>0 with Nj > 500: fresh normals every 3rd step, row-permuted otherwise
+    s = pj.SquareArray(17, 17)
+    Is = s.current_base(0.0)[:, None, None] * np.array([0.8])[None, :, None]
+    ts = np.arange(0, 60, 3)
+    cases["sq17_T"] = run_case(pj, "sq17_T", s, dict(Is=Is, ts=ts, f=0.2, T=0.02, seed=77),
+                               seed=77, time_step=0.05, time_step_count=60, external_flux=0.2,
+                               current_sources=Is, temperature=0.02, store_time_steps=ts)
+
+    # only theta stored, two steps (the bench-style IV measurement)
+    s = pj.SquareArray(9, 9)
+    s.set_capacitance(1.0)
+    Nt = 600
+    Is = s.current_base(0.0)[:, None, None] * np.linspace(0.0, 2.0, 8)[None, :, None]
+    ts = [Nt // 3, Nt - 1]
+    cases["sq9_iv"] = run_case(pj, "sq9_iv", s, dict(Is=Is, ts=np.array(ts), f=0.25, C=1.0),
+                               time_step=0.05, time_step_count=Nt, external_flux=0.25,
+                               current_sources=Is, store_time_steps=ts,
+                               store_current=False, store_voltage=False)
+
+    flat = {}
+    for cname, rec in cases.items():
+        for k, v in rec.items():
+            flat[f"{cname}__{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "trajectories.npz"), **flat)
+    print("trajectories:", sorted(cases))
+
+
+if __name__ == "__main__":
+    pj = _import_reference()
+    make_graphs(pj)
+    make_trajectories(pj)

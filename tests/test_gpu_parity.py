@@ -1,0 +1,107 @@
+"""GPU parity: libjja's HIP path against the reference's own trajectories (golden
+fixtures) and against the CPU oracle, through the drop-in API and the C ABI.
+
+Tolerance (FP64): per stored step, max |gpu - ref| <= 1e-9 * max |ref| for theta,
+current and voltage; vortex configurations exactly equal.
+"""
+import numpy as np
+import pytest
+
+import pyjjasim_amd as pa
+from cases import CASES, T_CASES, build, g, oracle_kwargs, traj
+from oracle import te_oracle as to
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+
+
+def assert_close_per_step(got, ref, rtol=RTOL, what=""):
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    for s in range(ref.shape[2]):
+        scale = np.abs(ref[:, :, s]).max()
+        err = np.abs(got[:, :, s] - ref[:, :, s]).max()
+        assert err <= rtol * max(scale, 1e-300), f"{what} step {s}: err {err:.3e} scale {scale:.3e}"
+
+
+def _compare_to_golden(case, res):
+    for key, getter in (("theta", res.get_theta), ("current", res.get_current),
+                        ("voltage", res.get_voltage)):
+        name = f"{case}__{key}"
+        if name in traj().files:
+            assert_close_per_step(getter(), g(case, key), what=f"{case}:{key}")
+    np.testing.assert_array_equal(res.get_vortex_configuration(), g(case, "n"))
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c not in T_CASES])
+def test_gpu_matches_reference_T0(case):
+    print(f"[gpu parity] {case}", flush=True)
+    c, kw, _ = build(case)
+    res = pa.TimeEvolutionProblem(c, **kw).compute()
+    _compare_to_golden(case, res)
+
+
+@pytest.mark.parametrize("case", T_CASES)
+def test_gpu_reference_noise_matches_reference(case):
+    """T > 0 with the host-replayed legacy RNG stream: the reference's exact draws."""
+    print(f"[gpu parity] {case} (reference noise)", flush=True)
+    c, kw, seed = build(case)
+    np.random.seed(seed)
+    res = pa.TimeEvolutionProblem(c, noise="reference", **kw).compute()
+    _compare_to_golden(case, res)
+
+
+def _bench_like(n=12, W=8, Nt=400, T=0.01, seed=99):
+    c = pa.SquareArray(n, n)
+    rng = np.random.default_rng(4)
+    c.set_critical_current(np.exp(0.1 * rng.standard_normal(c.junction_count())))
+    c.set_capacitance(1.0)
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.0, 2.0, W)[None, :, None]
+    kw = dict(time_step=0.05, time_step_count=Nt, external_flux=0.25, current_sources=Is,
+              temperature=T, store_time_steps=[Nt // 3, Nt - 1], store_current=False,
+              store_voltage=False)
+    return c, kw
+
+
+def test_gpu_philox_matches_oracle():
+    """Kernel noise (Philox4x32-10 + Box-Muller) equals the oracle's stream."""
+    print("[gpu parity] philox bench-like", flush=True)
+    c, kw = _bench_like()
+    res = pa.TimeEvolutionProblem(c, seed=99, **kw).compute()
+    ok = oracle_kwargs(kw)
+    ref = to.evolve(c, noise=to.philox_noise(99, 8, c.junction_count()), **ok)
+    assert_close_per_step(res.get_theta(), ref["theta"], what="theta")
+    np.testing.assert_array_equal(res.get_vortex_configuration(),
+                                  to.vortex(c.get_cycle_matrix(), ref["theta"]))
+
+
+def test_gpu_callable_sources_match_oracle():
+    """Callable Is(i) (Shapiro drive) evaluated per chunk on the host."""
+    print("[gpu parity] callable source", flush=True)
+    c = pa.SquareArray(8, 8)
+    base = c.current_base(0.0)
+    Idc = np.linspace(0.2, 1.6, 5)
+    Is = lambda i: base[:, None] * (Idc + 0.5 * np.sin(0.3 * i * 0.05))
+    kw = dict(time_step=0.05, time_step_count=700, external_flux=0.1, current_sources=Is,
+              store_time_steps=np.arange(0, 700, 50))
+    res = pa.TimeEvolutionProblem(c, steps_per_launch=128, **kw).compute()
+    ref = to.evolve(c, **oracle_kwargs(kw))
+    for key in ("theta", "current", "voltage"):
+        assert_close_per_step(getattr(res, "get_" + key)(), ref[key], what=key)
+
+
+def test_gpu_deterministic_across_launch_shapes():
+    """Same seed -> identical bits for any block size and chunking."""
+    print("[gpu parity] determinism", flush=True)
+    c, kw = _bench_like(n=10, W=4, Nt=300)
+    a = pa.TimeEvolutionProblem(c, seed=5, block_size=1024, **kw).compute().get_theta()
+    b = pa.TimeEvolutionProblem(c, seed=5, block_size=128, steps_per_launch=37, **kw).compute().get_theta()
+    assert np.array_equal(a, b)
+
+
+def test_gpu_rejects_custom_cpr():
+    c = pa.SquareArray(3, 3)
+    cpr = pa.CurrentPhaseRelation(lambda Ic, t: Ic * np.sin(t) ** 3, lambda Ic, t: 0 * t,
+                                  lambda Ic, t: 0 * t)
+    with pytest.raises(ValueError, match="current-phase relation"):
+        pa.TimeEvolutionProblem(c, current_phase_relation=cpr, time_step_count=5).compute()

@@ -1,0 +1,115 @@
+"""Planar graph builder vs the reference's face order, orientation and bias vectors
+(golden fixtures from tests/golden/make_golden.py) and closed forms."""
+import numpy as np
+import pytest
+
+import pyjjasim_amd as pa
+from cases import graphs
+
+X9 = [0, 1, 2, 0, 1, 2, 0, 1, 2]
+Y9 = [0, 0, 0, 1, 1, 1, 2, 2, 2]
+N1 = [4, 2, 7, 0, 5, 6, 4, 1, 8, 7, 7]
+N2 = [3, 1, 6, 3, 2, 3, 5, 0, 5, 4, 8]
+
+
+def _hole():
+    full = pa.SquareArray(8, 8)
+    fx, fy = full.get_node_coordinates()
+    return full.remove_nodes((fx > 2) & (fx < 5) & (fy > 2) & (fy < 5))
+
+
+BUILDERS = {
+    "sq_3x4": (lambda: pa.SquareArray(3, 4), True),
+    "sq_10x10": (lambda: pa.SquareArray(10, 10), True),
+    "sq_5x3_scaled": (lambda: pa.SquareArray(5, 3, 1.5, 0.5), True),
+    "hc_3x3": (lambda: pa.HoneycombArray(3, 3), True),
+    "hc_4x2": (lambda: pa.HoneycombArray(4, 2), True),
+    "hc_6x6": (lambda: pa.HoneycombArray(6, 6), True),
+    "tri_4x3": (lambda: pa.TriangularArray(4, 3), True),
+    "tri_3x5": (lambda: pa.TriangularArray(3, 5), True),
+    "tri_6x6": (lambda: pa.TriangularArray(6, 6), True),
+    "squid": (lambda: pa.SQUID(), False),
+    "custom": (lambda: pa.Circuit(pa.EmbeddedGraph(X9, Y9, N1, N2)), False),
+    "custom_dangling": (lambda: pa.Circuit(pa.EmbeddedGraph(X9 + [0.4], Y9 + [1.6], N1 + [9],
+                                                            N2 + [3])), False),
+    "hole_8": (_hole, False),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BUILDERS))
+def test_matrices_match_reference(name):
+    G = graphs()
+    build, lattice = BUILDERS[name]
+    c = build()
+    A = np.asarray(c.get_cycle_matrix().todense())
+    M = np.asarray(c.get_cut_matrix().todense())
+    assert np.array_equal(A, G[f"{name}__A"]), "face order or orientation differs"
+    assert np.array_equal(M, G[f"{name}__M"])
+    n1, n2 = c.get_junction_nodes()
+    assert np.array_equal(n1, G[f"{name}__node1"]) and np.array_equal(n2, G[f"{name}__node2"])
+    np.testing.assert_allclose(c.get_face_areas(), G[f"{name}__areas"], rtol=1e-13, atol=1e-13)
+    cx, cy = c.get_face_centroids()
+    np.testing.assert_allclose(cx, G[f"{name}__centroid_x"], atol=1e-12)
+    np.testing.assert_allclose(cy, G[f"{name}__centroid_y"], atol=1e-12)
+    if lattice:
+        for key, ang in (("cb0", 0.0), ("cb90", np.pi / 2), ("cb30", np.pi / 6)):
+            np.testing.assert_array_equal(c.current_base(ang), G[f"{name}__{key}"])
+
+
+@pytest.mark.parametrize("nx,ny", [(2, 2), (3, 5), (7, 4)])
+def test_square_closed_form(nx, ny):
+    """Face (fx, fy) is row fy*(nx-1)+fx: +1 on its bottom and right junctions,
+    -1 on top and left; horizontal junctions come first (embedded_graph.py:1106-1111)."""
+    c = pa.SquareArray(nx, ny)
+    A = np.asarray(c.get_cycle_matrix().todense())
+    Eh = (nx - 1) * ny
+    ref = np.zeros(((nx - 1) * (ny - 1), Eh + nx * (ny - 1)), dtype=int)
+    for fy in range(ny - 1):
+        for fx in range(nx - 1):
+            f = fy * (nx - 1) + fx
+            ref[f, fy * (nx - 1) + fx] = 1                 # bottom
+            ref[f, (fy + 1) * (nx - 1) + fx] = -1          # top
+            ref[f, Eh + fy * nx + fx + 1] = 1              # right
+            ref[f, Eh + fy * nx + fx] = -1                 # left
+    assert np.array_equal(A, ref)
+
+
+@pytest.mark.parametrize("c", [pa.SquareArray(6, 5), pa.HoneycombArray(4, 3),
+                               pa.TriangularArray(5, 4), pa.SQUID()])
+def test_topology_invariants(c):
+    A, M = c.get_cycle_matrix(), c.get_cut_matrix()
+    assert abs(A @ M.T).max() == 0               # cycles are orthogonal to cuts
+    assert c.node_count() + c.face_count() == c.junction_count() + 1   # Euler
+    assert np.all(c.get_face_areas() > 0)
+
+
+def test_current_base_conserves_current():
+    c = pa.HoneycombArray(5, 4)
+    Is = c.current_base(0.3)
+    node = c.current_base(0.3, type="node")
+    np.testing.assert_allclose(-c.get_cut_matrix() @ Is, node, atol=1e-12)
+
+
+def test_invalid_graphs():
+    with pytest.raises(pa.SelfLoopError):
+        pa.EmbeddedGraph([0, 1], [0, 0], [0], [0])
+    with pytest.raises(pa.NonSimpleError):
+        pa.EmbeddedGraph([0, 1], [0, 0], [0, 1], [1, 0])
+    with pytest.raises(pa.NotPlanarEmbeddingError):
+        # crossing diagonals of a square
+        pa.Circuit(pa.EmbeddedGraph([0, 1, 1, 0], [0, 0, 1, 1], [0, 1, 2, 3, 0, 1],
+                                    [1, 2, 3, 0, 2, 3]))
+
+
+def test_save_load_roundtrip(tmp_path):
+    c = pa.SquareArray(4, 3)
+    c.set_critical_current(np.linspace(0.5, 1.5, c.junction_count()))
+    c.set_capacitance(0.3)
+    c.set_inductance(0.7)
+    fn = tmp_path / "c.npy"
+    c.save(fn)
+    d = pa.Circuit.load(fn)
+    assert np.array_equal(d.get_critical_current(), c.get_critical_current())
+    assert np.array_equal(np.asarray(d.get_cycle_matrix().todense()),
+                          np.asarray(c.get_cycle_matrix().todense()))
+    assert abs(d.get_inductance() - c.get_inductance()).max() == 0

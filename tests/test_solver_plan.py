@@ -1,0 +1,85 @@
+"""Host preparation for the GPU: native Cholesky, nested-dissection stage schedule,
+cycle-matrix index arrays. The schedule is checked by emulating on the host
+exactly the stage algorithm the kernel runs, against a direct sparse solve."""
+import numpy as np
+import pytest
+import scipy.sparse
+import scipy.sparse.linalg
+
+import pyjjasim_amd as pa
+from pyjjasim_amd.solver_plan import CircuitPlan, cholesky, cycle_codes
+
+
+def _circuits():
+    sqL = pa.SquareArray(12, 9)
+    sqL.set_inductance(0.8)
+    het = pa.SquareArray(9, 9)
+    rng = np.random.default_rng(1)
+    het.set_resistance(0.5 + rng.random(het.junction_count()))
+    het.set_capacitance(rng.random(het.junction_count()))
+    return {"sq10": pa.SquareArray(10, 10), "hc7": pa.HoneycombArray(7, 5),
+            "tri8": pa.TriangularArray(8, 8), "squid": pa.SQUID(), "sqL": sqL, "het": het}
+
+
+@pytest.mark.parametrize("name", sorted(_circuits()))
+def test_staged_solve_matches_direct(name):
+    c = _circuits()[name]
+    plan = CircuitPlan(c, 0.05, leaf_size=8)
+    K = scipy.sparse.csc_matrix(plan.K)
+    b = np.random.default_rng(0).standard_normal((K.shape[0], 4))
+    want = scipy.sparse.linalg.spsolve(K, b)
+    got = plan.solve_plan.solve(b)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * abs(want).max())
+
+
+def test_cholesky_reproduces_matrix():
+    rng = np.random.default_rng(5)
+    n = 60
+    B = scipy.sparse.random(n, n, density=0.08, random_state=7)
+    K = (B @ B.T + scipy.sparse.identity(n) * 3).tocsc()
+    L = cholesky(K)
+    assert abs(L @ L.T - K).max() < 1e-12
+    assert abs(scipy.sparse.triu(L, 1)).max() == 0 if scipy.sparse.triu(L, 1).nnz else True
+    np.testing.assert_allclose(L.toarray(), np.linalg.cholesky(K.toarray()), atol=1e-12)
+
+
+def test_cholesky_rejects_indefinite():
+    K = scipy.sparse.csc_matrix(np.array([[1.0, 2.0], [2.0, 1.0]]))
+    with pytest.raises(ValueError, match="positive definite"):
+        cholesky(K)
+
+
+def test_schedule_structure():
+    c = pa.SquareArray(30, 30)
+    sp = CircuitPlan(c, 0.05, leaf_size=16).solve_plan
+    n = sp.n
+    assert sp.stage_ptr[0] == 0 and sp.stage_ptr[-1] == n
+    # off-block rows only reach earlier stages (lower) or later stages (upper)
+    stage_of = np.searchsorted(sp.stage_ptr, np.arange(n), side="right") - 1
+    rows = np.repeat(np.arange(n), np.diff(sp.lo_ptr))
+    assert np.all(stage_of[sp.lo_col] < stage_of[rows])
+    rows = np.repeat(np.arange(n), np.diff(sp.up_ptr))
+    assert np.all(stage_of[sp.up_col] > stage_of[rows])
+    # blocks stay inside a stage
+    assert np.all(stage_of[sp.row_block] == stage_of)
+    assert np.all(stage_of[sp.row_block + sp.row_bsize - 1] == stage_of)
+    assert sp.n_stages < 12
+
+
+def test_cycle_codes():
+    c = pa.HoneycombArray(3, 3)
+    A = c.get_cycle_matrix()
+    fp, fc, jc = cycle_codes(A, c.face_count(), c.junction_count())
+    dense = np.zeros(A.shape)
+    for f in range(A.shape[0]):
+        for e in range(fp[f], fp[f + 1]):
+            dense[f, fc[e] >> 1] = -1 if fc[e] & 1 else 1
+        assert np.all(np.diff(fc[fp[f]:fp[f + 1]] >> 1) > 0)   # ascending junctions
+    assert np.array_equal(dense, A.toarray())
+    dense2 = np.zeros(A.shape)
+    for j in range(A.shape[1]):
+        for k in range(2):
+            code = jc[2 * j + k]
+            if code >= 0:
+                dense2[code >> 1, j] = -1 if code & 1 else 1
+    assert np.array_equal(dense2, A.toarray())
