@@ -1,0 +1,187 @@
+"""Benchmark: batched IV sweep on one MI355X, FP64.
+
+Workload: SquareArray(100,100) (19,800 junctions, 9,801 faces), per-junction Ic
+log-normal (sigma 0.1, seeded), beta_C = 1, f = 0.25, W = 256 bias currents
+linspace(0, 2.0) * current_base(0), T = 0.01 with the seeded in-kernel Gaussian
+stream, dt = 0.05, theta stored at steps Nt//3 and Nt-1.
+
+A bench "step" is one time step of all 256 columns. The timed region covers
+``--steps`` steps (persistent kernel launches) with all inputs resident in HBM,
+after ``--warmup`` untimed steps. Reported:
+
+  value      junction-steps/s = Nj * W * steps / wall time
+  roofline   algorithmic HBM bytes per launch / mean launch time (HIP events on the
+             launch stream) against the 8 TB/s HBM peak
+  plain      the one-problem-at-a-time GPU loop (W = 1) on the same circuit
+  cpu_baseline  the FP64 numpy/scipy restatement of the reference scheme (same
+             library calls), timed on this host on a bounded sample of steps
+
+Usage: python bench.py [--gpus 1] [--steps 20000] [--warmup 1000]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "batched_iv_sweep_square100_W256"
+
+
+def make_problem(pa, n, W, Nt, T=0.01, seed=12345, steps_per_launch=None, block_size=None):
+    c = pa.SquareArray(n, n)
+    rng = np.random.default_rng(20261202)
+    c.set_critical_current(np.exp(0.1 * rng.standard_normal(c.junction_count())))
+    c.set_capacitance(1.0)
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.0, 2.0, W)[None, :, None]
+    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.25,
+                                   current_sources=Is, temperature=T,
+                                   store_time_steps=[Nt // 3, Nt - 1], store_current=False,
+                                   store_voltage=False, seed=seed,
+                                   steps_per_launch=steps_per_launch, block_size=block_size)
+    return c, prob
+
+
+def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events=True):
+    from pyjjasim_amd.engine import Engine
+    Nt = warmup + steps
+    c, prob = make_problem(pa, n, W, Nt, steps_per_launch=steps_per_launch, block_size=block_size)
+    th_mask = prob.store_time_steps.copy()
+    I_mask = np.zeros(Nt, dtype=bool)
+    eng = Engine(prob, th_mask, I_mask)
+    eng.advance(warmup)
+    torch.cuda.synchronize()
+    launches = []
+    stream = torch.cuda.current_stream()
+    t_start = time.perf_counter()
+    end = eng.t + steps
+    while eng.t < end:
+        k = min(eng.steps_per_launch, end - eng.t)
+        if events:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            eng.launch(k)
+            e1.record(stream)
+            launches.append((k, e0, e1))
+        else:
+            eng.launch(k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    launch_ms = [(k, e0.elapsed_time(e1)) for k, e0, e1 in launches]
+    th, _, nv = eng.results()
+    info = dict(Nj=c.junction_count(), Nf=c.face_count(), bytes_per_step=eng.bytes_per_step(),
+                nnz_factor=len(eng.host.solve_plan.lo_val) + len(eng.host.solve_plan.inv_cm),
+                n_stages=eng.host.solve_plan.n_stages, block_size=eng.block_size,
+                steps_per_launch=eng.steps_per_launch)
+    eng.close()
+    return wall, launch_ms, th, nv, c, info
+
+
+def cpu_baseline(pa, n, W, budget_s):
+    """Oracle (numpy/scipy, same calls as the reference) on a bounded sample of steps."""
+    from oracle import te_oracle as to
+    c, _ = make_problem(pa, n, W, 10)
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.0, 2.0, W)[None, :, None]
+
+    def timed(Nt):
+        np.random.seed(0)
+        t0 = time.perf_counter()
+        to.evolve(c, dt=0.05, Nt=Nt, f=0.25, Is=Is, T=0.01, store_time_steps=[Nt - 1],
+                  store_current=False, store_voltage=False)
+        return time.perf_counter() - t0
+    t_probe = timed(3)
+    steps = max(3, int(budget_s / max(t_probe / 3, 1e-6)))
+    t = timed(steps)
+    rate = c.junction_count() * W * steps / t
+    return dict(value=rate, unit="junction-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps of SquareArray({n},{n}) W={W} T=0.01 (includes one "
+                       f"SuperLU factorisation), single-threaded numpy/scipy; "
+                       f"{t:.1f} s total")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--n", type=int, default=100)
+    ap.add_argument("--W", type=int, default=256)
+    ap.add_argument("--steps-per-launch", type=int, default=None)
+    ap.add_argument("--block-size", type=int, default=None)
+    ap.add_argument("--plain-steps", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(columns and problems are independent; see the sharding notes in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import pyjjasim_amd as pa
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+
+    wall, launch_ms, th, nv, c, info = run_gpu(pa, torch, args.n, args.W, args.steps, args.warmup,
+                                               args.steps_per_launch, args.block_size)
+    Nj, W, K = info["Nj"], args.W, args.steps
+    value = Nj * W * K / wall
+    # dominant kernel: evolve_kernel; one launch advances k steps of the whole batch
+    tot_steps = sum(k for k, _ in launch_ms)
+    tot_ms = sum(ms for _, ms in launch_ms)
+    mean_launch_ms = tot_ms / len(launch_ms)
+    bytes_per_launch = info["bytes_per_step"] * tot_steps / len(launch_ms)
+    achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "evolve_kernel_square100_W256_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tj = json.load(fh)
+        if tj.get("steps_per_launch") == info["steps_per_launch"]:
+            traffic = tj.get("hbm_bytes_per_launch")
+    # integrity check at full size: device vortex counts equal the host count
+    A = c.get_cycle_matrix()
+    n_host = (-A @ np.round(th[:, :, -1] / (2.0 * np.pi))).astype(int)
+    assert np.array_equal(n_host, nv[:, :, -1]), "device vortex counts disagree with theta"
+    # plain: one problem at a time (W = 1) on the same circuit
+    pwall, pl, _, _, _, _ = run_gpu(pa, torch, args.n, 1, args.plain_steps, 100,
+                                    args.steps_per_launch, args.block_size, events=False)
+    plain_rate = Nj * 1 * args.plain_steps / pwall
+    out = {
+        "metric": "junction-steps/s (batched IV sweep, FP64)",
+        "value": value,
+        "unit": "junction-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Ic disorder, seeded in-kernel thermal noise)",
+        "config": {"workload": WORKLOAD, "lattice": f"SquareArray({args.n},{args.n})",
+                   "junctions": Nj, "faces": info["Nf"], "W": W, "dt": 0.05, "T": 0.01,
+                   "beta_C": 1.0, "f": 0.25, "Nt": args.warmup + K,
+                   "steps_per_launch": info["steps_per_launch"], "block_size": info["block_size"],
+                   "solve_stages": info["n_stages"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "evolve_kernel", "bytes_per_step": info["bytes_per_step"],
+                     "mean_launch_ms": mean_launch_ms},
+        "plain": {"value": plain_rate, "unit": "junction-steps/s",
+                  "note": "one problem (W=1) per run, same circuit and kernel",
+                  "batched_over_plain": value / plain_rate},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pa, args.n, W, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

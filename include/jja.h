@@ -58,48 +58,79 @@ const char* jja_factor_last_error(void);
 /* ------------------------------------------------------------------- plan */
 
 /*
- * One circuit: topology, per-junction coefficients and the stage schedule of
- * the triangular solves. All pointers are HOST arrays, copied by plan creation.
+ * One circuit: per-junction coefficients, cycle-matrix layouts and the stage
+ * schedule of the triangular solves, laid out for workgroups of `block_size`
+ * threads. All pointers are HOST arrays, copied by plan creation.
  *
- * Face positions: the solve works on faces renumbered by face_pos. Rows of the
- * factor are grouped into dense blocks; blocks of one stage are independent.
- * Forward solve of stage s, for each row r of the stage:
- *     t_r = b_r - sum_e lo_val[e] * z[lo_col[e]]           (e in lo_ptr[r]..)
- *     z_r = sum_{c <= a} inv(a, c) * t_{block+c}             (a = r - block)
- * Backward solve, stages in reverse:
- *     t_r = z_r - sum_e up_val[e] * x[up_col[e]]
- *     x_r = sum_{c >= a} inv(c, a) * t_{block+c}
- * where inv is the inverse of the block's diagonal part of L. Packed storage of
- * a block of size B at offset o:  inv_cm[o + c*B - c*(c-1)/2 + (a - c)] = inv(a, c)
- * (column-major lower), inv_rm[o + c*(c+1)/2 + a] = inv(c, a) (row-major lower).
+ * Faces are renumbered to solve positions 0..Nf-1 (face_of_pos). Work that a
+ * workgroup spreads over its threads is a list of slices (jja_slice):
+ *   - G = flags & 0xff consecutive threads share one row (G a power of two
+ *     <= 64); thread tid works on row slice_rows[rowbase + tid/G] (-1: none) and
+ *     takes that row's entries tid%G, tid%G + G, ...;
+ *   - every address is known without reading an index first. Face slices:
+ *     code `it` (it < n_iter <= 8) of thread tid at base + it*block_size + tid.
+ *     Solve slices hold n_iter <= 8 entries per thread: values in pairs, value
+ *     `it` of thread tid at base + (it/2)*2*block_size + 2*tid + it%2 (16-byte
+ *     loads, ceil(n_iter/2) pairs); LDS byte offsets lane-major in groups of
+ *     four, offset `it` at ibase + tid*NQ*4 + it with NQ = ceil(n_iter/4).
+ *     An offset addresses the workgroup's LDS: z at 8*position, the stage
+ *     scratch t at 8*(n_faces + stage row). Face codes pad with -1.
+ *     The solve program ends with 4 records that are never executed (flags 0):
+ *     the kernel loads up to four records ahead.
+ * A row may span several consecutive slices; only the last one (flag FINISH)
+ * reduces over the G lanes and writes the row. Row slots are filled from the
+ * front: flags >> 16 is the number of slots holding a row, and threads of the
+ * other slots load nothing.
+ *
+ * Face phase (G = 1, entries = junction codes 2*j + (A[f,j] < 0), ascending j):
+ *     b_p = sum over codes of +-u_j,  u = x/c0 - theta_s.
+ * The rows of the factor L of the reordered face matrix form dense blocks;
+ * blocks of one stage are independent. The solve program lists the slices of
+ * the forward then the backward solve in execution order; for each stage:
+ *     off-block (flag BLOCK clear): t[row - r0] = z[row] - sum val * z[idx]
+ *     block     (flag BLOCK set):   z[row]      = sum val * t[idx]
+ * where the block entries are rows (forward) or columns (backward) of the
+ * inverse of the row's dense diagonal block and idx is a stage-local row.
+ * Flag BARRIER ends a phase: the workgroup synchronises after the slice.
  */
+#define JJA_SLICE_BLOCK 0x100
+#define JJA_SLICE_FINISH 0x200
+#define JJA_SLICE_BARRIER 0x400
+
+typedef struct jja_slice {
+  int64_t base;    /* first value (face slices: first code) */
+  int64_t ibase;   /* first index (solve slices) */
+  int32_t rowbase; /* first row slot in slice_rows */
+  int32_t r0;      /* first position of the slice's stage */
+  int32_t n_iter;  /* entries per thread, 0..8 */
+  int32_t flags;   /* G | JJA_SLICE_* | rows << 16 */
+} jja_slice;
+
 typedef struct jja_problem_desc {
   int32_t n_junctions;
   int32_t n_faces;
-  int32_t stencil_width; /* 3, 4 or 5 (time_evolution.py:401-408) */
-  int32_t n_stages;
-  /* cycle matrix A by face, ascending junction: code = 2*j + (A[f,j] < 0) */
-  const int32_t* face_ptr;  /* [n_faces+1] */
-  const int32_t* face_code; /* [face_ptr[n_faces]] */
-  /* A^T by junction, two slots, ascending face: code = 2*f + (A[f,j] < 0), -1 empty */
-  const int32_t* junc_code; /* [2*n_junctions] */
+  int32_t stencil_width;    /* 3, 4 or 5 (time_evolution.py:401-408) */
+  int32_t block_size;       /* threads per workgroup the layouts are built for */
   const double* Ic;         /* [Nj] critical current */
   const double* Rv;         /* [Nj] 1/(dt R) */
   const double* coef;       /* [stencil_width][Nj] c_k = S1[k] Rv + S2[k] Cv (time_evolution.py:477-499) */
-  const int32_t* face_pos;  /* [Nf] */
-  const int32_t* stage_ptr; /* [n_stages+1] */
-  const int64_t* lo_ptr;    /* [Nf+1] */
-  const int32_t* lo_col;
-  const double* lo_val;
-  const int64_t* up_ptr;    /* [Nf+1] */
-  const int32_t* up_col;
-  const double* up_val;
-  const int32_t* row_block; /* [Nf] first row of the row's block */
-  const int32_t* row_bsize; /* [Nf] block size */
-  const int64_t* row_boff;  /* [Nf] block offset into inv_cm / inv_rm */
-  int64_t n_inv;
-  const double* inv_cm;
-  const double* inv_rm;
+  /* A^T by junction: two slots, ascending face index, code = 2*position + (A[f,j] < 0), -1 empty */
+  const int32_t* junc_code; /* [2*Nj] */
+  const int32_t* face_of_pos; /* [Nf] */
+  int32_t n_face_slices;
+  int32_t n_solve_slices;
+  const jja_slice* face_slices;  /* [n_face_slices] */
+  const jja_slice* solve_slices; /* [n_solve_slices], execution order */
+  int64_t n_face_entries;
+  const int32_t* face_code;      /* [n_face_entries] */
+  int64_t n_slice_rows;
+  const int32_t* slice_rows;     /* [n_slice_rows] */
+  int64_t n_entries;
+  const double* ent_val;         /* [n_entries] */
+  int64_t n_index;
+  const uint32_t* ent_off;       /* [n_index] LDS byte offsets */
+  int32_t max_stage_rows;        /* size of the stage scratch t */
+  int32_t pad_;
 } jja_problem_desc;
 
 /* Create a plan for n_problems circuits on the current HIP device. */
@@ -135,7 +166,11 @@ typedef struct jja_problem_run {
   double dt;
   double* theta;         /* [stencil_width-1][W][Nj]; theta at step m in slot m mod (sw-1) */
   double* theta_s;       /* [W][Nj] voltage-source phase; NULL when Vs.zero */
+  double* u;             /* [W][Nj] scratch: x/c0 - theta_s of the step being solved */
   jja_source Is, Vs, f, T;
+  /* noise amplitude sqrt(2 T Rv) when T is constant in time (sigma.base != NULL);
+   * otherwise the kernel forms it from T */
+  jja_source sigma;
   const double* noise;   /* optional [n_steps][W][Nj] standard normals (index t - noise_t0) */
   int64_t noise_t0;
   uint64_t seed;         /* Philox key when noise == NULL */
@@ -149,11 +184,12 @@ typedef struct jja_problem_run {
 /*
  * Advance every column of every problem by n_steps steps, starting at absolute
  * step step0, on `stream` (a hipStream_t; NULL = default stream). One
- * workgroup integrates one column for all n_steps steps (persistent launch).
- * block_size: threads per workgroup (64..1024, multiple of 64; 0 = default).
+ * workgroup of the plan's block_size threads integrates one column for all
+ * n_steps steps (persistent launch). All problems of a plan share one
+ * block_size.
  */
 int jja_evolve(jja_plan* plan, const jja_problem_run* runs, int32_t n_steps, int64_t step0,
-               int32_t block_size, void* stream);
+               void* stream);
 
 /*
  * Vortex configuration n[f] = -sum_j A[f,j] rint(theta_j / (2 pi))

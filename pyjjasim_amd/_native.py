@@ -24,17 +24,29 @@ i64p = C.POINTER(C.c_int64)
 f64p = C.POINTER(C.c_double)
 
 
+class Slice(C.Structure):
+    _fields_ = [("base", C.c_int64), ("ibase", C.c_int64), ("rowbase", C.c_int32),
+                ("r0", C.c_int32), ("n_iter", C.c_int32), ("flags", C.c_int32)]
+
+
+SLICE_BLOCK = 0x100
+SLICE_FINISH = 0x200
+SLICE_BARRIER = 0x400
+
+
 class ProblemDesc(C.Structure):
     _fields_ = [
         ("n_junctions", C.c_int32), ("n_faces", C.c_int32),
-        ("stencil_width", C.c_int32), ("n_stages", C.c_int32),
-        ("face_ptr", i32p), ("face_code", i32p), ("junc_code", i32p),
+        ("stencil_width", C.c_int32), ("block_size", C.c_int32),
         ("Ic", f64p), ("Rv", f64p), ("coef", f64p),
-        ("face_pos", i32p), ("stage_ptr", i32p),
-        ("lo_ptr", i64p), ("lo_col", i32p), ("lo_val", f64p),
-        ("up_ptr", i64p), ("up_col", i32p), ("up_val", f64p),
-        ("row_block", i32p), ("row_bsize", i32p), ("row_boff", i64p),
-        ("n_inv", C.c_int64), ("inv_cm", f64p), ("inv_rm", f64p),
+        ("junc_code", i32p), ("face_of_pos", i32p),
+        ("n_face_slices", C.c_int32), ("n_solve_slices", C.c_int32),
+        ("face_slices", C.c_void_p), ("solve_slices", C.c_void_p),
+        ("n_face_entries", C.c_int64), ("face_code", i32p),
+        ("n_slice_rows", C.c_int64), ("slice_rows", i32p),
+        ("n_entries", C.c_int64), ("ent_val", f64p), ("n_index", C.c_int64),
+        ("ent_off", C.POINTER(C.c_uint32)),
+        ("max_stage_rows", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -49,8 +61,8 @@ class Source(C.Structure):
 class ProblemRun(C.Structure):
     _fields_ = [
         ("W", C.c_int32), ("column_offset", C.c_int32), ("dt", C.c_double),
-        ("theta", C.c_void_p), ("theta_s", C.c_void_p),
-        ("Is", Source), ("Vs", Source), ("f", Source), ("T", Source),
+        ("theta", C.c_void_p), ("theta_s", C.c_void_p), ("u", C.c_void_p),
+        ("Is", Source), ("Vs", Source), ("f", Source), ("T", Source), ("sigma", Source),
         ("noise", C.c_void_p), ("noise_t0", C.c_int64), ("seed", C.c_uint64),
         ("th_slot", C.c_void_p), ("I_slot", C.c_void_p), ("slot_t0", C.c_int64),
         ("th_out", C.c_void_p), ("I_out", C.c_void_p),
@@ -85,8 +97,7 @@ def lib():
     L.jja_plan_destroy.argtypes = [C.c_void_p]
     L.jja_plan_lds_bytes.argtypes = [C.c_void_p, C.c_int32]
     L.jja_plan_lds_bytes.restype = C.c_int64
-    L.jja_evolve.argtypes = [C.c_void_p, C.POINTER(ProblemRun), C.c_int32, C.c_int64, C.c_int32,
-                             C.c_void_p]
+    L.jja_evolve.argtypes = [C.c_void_p, C.POINTER(ProblemRun), C.c_int32, C.c_int64, C.c_void_p]
     L.jja_vortex.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     L.jja_last_error.restype = C.c_char_p
     _lib = L

@@ -2,20 +2,26 @@
 // time evolution of Josephson junction arrays.
 //
 // One workgroup integrates one column (problem w) of one circuit for a whole
-// chunk of steps: columns never exchange data, so the launch needs no grid-wide
+// chunk of steps: columns never exchange data, so a launch needs no grid-wide
 // synchronisation. Per step (stencil 3 shown; time_evolution.py:523-580):
 //
 //   junctions  x_j = eta - Is + Ic sin(2 th1 - th2) + c1 th1 + c2 th2
-//   faces      b_f = sum_j A_fj (x_j / c0_j - ths_j) - 2 pi f_f      (LDS)
-//   solve      J = K^-1 b : staged forward/backward triangular solves  (LDS)
+//              u_j = x_j / c0_j - ths_j                            (scratch)
+//   faces      b_f = sum_j A_fj u_j - 2 pi f_f                      (LDS)
+//   solve      J = K^-1 b : staged forward/backward triangular solves (LDS)
 //   junctions  y_j = sum_f A_fj J_f ; th_new = (y - x) / c0 ; I = y + Is
 //
 // The face vector lives in LDS for the whole step. The oldest phase slot of the
-// history holds x between the first and the last junction phase, then the new
-// phase, so the state needs stencil_width-1 vectors per column and nothing else.
-// Arithmetic follows the reference's operation order term by term (no FMA
-// contraction in the step expressions), so results agree with the CPU scheme to
-// rounding of sin and of the linear solve.
+// history holds x between the junction passes, then the new phase, so the
+// state is stencil_width-1 vectors per column. The junction pass that finishes
+// step t also prepares step t+1. Arithmetic of the step expressions follows the
+// reference's operation order (no FMA contraction), so results agree with the
+// CPU scheme to the rounding of sin and of the linear solve.
+//
+// The solve is a linear program of slices whose addresses do not depend on any
+// data: each thread prefetches the entries of the next slice (also across the
+// phase barriers, which only order LDS traffic) while it processes the current
+// one, so global-memory latency overlaps the LDS gathers and reductions.
 
 #include <hip/hip_runtime.h>
 
@@ -44,30 +50,57 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr double kTwoPi = 6.283185307179586;  // 2 * np.pi
+constexpr int kMaxIter = 8;                   // entries per thread and slice
+
+// Pointers read out of the descriptor structs are generic; casting them to the
+// global (1) or constant (4) address space lets the compiler use global and
+// scalar loads instead of flat ones, which would also stall on LDS traffic.
+#define GAS __attribute__((address_space(1)))
+#define CAS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const GAS T* gbl(const T* p) {
+  return (const GAS T*)p;
+}
+template <typename T>
+__device__ __forceinline__ GAS T* gblw(T* p) {
+  return (GAS T*)p;
+}
+
+typedef double vdouble2 __attribute__((ext_vector_type(2)));
+typedef uint32_t vuint2 __attribute__((ext_vector_type(2)));
+typedef uint32_t vuint4 __attribute__((ext_vector_type(4)));
+typedef int32_t vint2 __attribute__((ext_vector_type(2)));
+
+struct RecWords {
+  vuint4 a, b;
+};
+static_assert(sizeof(RecWords) == sizeof(jja_slice), "record layout");
+
+// Uniform load of one program record through the scalar cache.
+__device__ __forceinline__ jja_slice load_rec(const jja_slice* prog, int32_t k) {
+  const CAS vuint4* q = (const CAS vuint4*)(prog + k);
+  RecWords w;
+  w.a = q[0];
+  w.b = q[1];
+  return __builtin_bit_cast(jja_slice, w);
+}
 
 // Device-resident description of one circuit (all pointers device memory).
 struct DevProblem {
-  int32_t Nj, Nf, sw, n_stages;
-  int32_t max_stage_rows, pad_;
-  const int32_t* face_ptr;
-  const int32_t* face_code;
-  const int32_t* junc_code;
+  int32_t Nj, Nf, sw, BS;
+  int32_t n_face_slices, n_solve_slices, max_stage_rows, pad_;
   const double* Ic;
   const double* Rv;
   const double* coef;
-  const int32_t* face_pos;
-  const int32_t* stage_ptr;
-  const int64_t* lo_ptr;
-  const int32_t* lo_col;
-  const double* lo_val;
-  const int64_t* up_ptr;
-  const int32_t* up_col;
-  const double* up_val;
-  const int32_t* row_block;
-  const int32_t* row_bsize;
-  const int64_t* row_boff;
-  const double* inv_cm;
-  const double* inv_rm;
+  const int32_t* junc_code;
+  const int32_t* face_of_pos;
+  const jja_slice* face_slices;
+  const jja_slice* solve_slices;
+  const int32_t* face_code;
+  const int32_t* slice_rows;
+  const double* ent_val;
+  const uint32_t* ent_off;
+  const double* zero;  // 64 zero bytes
 };
 
 struct WorkItem {
@@ -77,14 +110,23 @@ struct WorkItem {
 
 __device__ __forceinline__ double src_at(const jja_source& s, int64_t i, int64_t w, int64_t t) {
   const int64_t tt = t - s.t0;
-  double v = s.base[i * s.s_i + w * s.s_w + tt * s.s_t];
-  if (s.amp) v = v * s.amp[w * s.a_w + tt * s.a_t];
+  double v = gbl(s.base)[i * s.s_i + w * s.s_w + tt * s.s_t];
+  if (s.amp) v = v * gbl(s.amp)[w * s.a_w + tt * s.a_t];
   return v;
 }
 
-// Philox4x32-10 (Salmon et al., SC'11) + Box-Muller; counter (j>>1, w, step, 0).
-__device__ __forceinline__ double philox_normal(uint64_t seed, uint32_t j, uint32_t w,
-                                                uint64_t step) {
+// Workgroup barrier that orders LDS traffic only: global loads in flight (the
+// prefetched solve entries) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Philox4x32-10 (Salmon et al., SC'11) + Box-Muller for the junction pair
+// (j, j+1), j even; counter (j>>1, w, step, 0), key = seed.
+__device__ __forceinline__ void normal_pair(uint64_t seed, uint32_t j, uint32_t w, uint64_t step,
+                                            double* z0, double* z1) {
   uint32_t c0 = j >> 1, c1 = w, c2 = static_cast<uint32_t>(step), c3 = 0u;
   uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
 #pragma unroll
@@ -106,15 +148,249 @@ __device__ __forceinline__ double philox_normal(uint64_t seed, uint32_t j, uint3
   const double u2 = static_cast<double>(b) * 0x1.0p-53;
   const double rho = sqrt(-2.0 * log(u1));
   const double ang = kTwoPi * u2;
-  return (j & 1u) ? rho * sin(ang) : rho * cos(ang);
+  *z0 = rho * cos(ang);
+  *z1 = rho * sin(ang);
 }
 
-__device__ __forceinline__ int64_t col_start(int32_t c, int32_t B) {
-  return static_cast<int64_t>(c) * B - (static_cast<int64_t>(c) * (c - 1)) / 2;
+// One 64-bit DPP move (two 32-bit halves); rows outside ROW_MASK and lanes
+// without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __builtin_bit_cast(double, static_cast<long long>(static_cast<unsigned>(lo)) |
+                                        (static_cast<long long>(hi) << 32));
 }
 
-__device__ __forceinline__ int64_t row_start(int32_t c) {
-  return (static_cast<int64_t>(c) * (c + 1)) / 2;
+// Sum over each aligned group of G lanes (G a power of two <= 64), left in the
+// group's last lane. Fixed order (shift-add within 16-lane rows, then row
+// broadcasts), so the bits do not depend on anything but the inputs.
+__device__ __forceinline__ double group_sum_last(double v, int G) {
+  if (G >= 2) v += dpp_f64<0x111, 0xf>(v);   // row_shr:1
+  if (G >= 4) v += dpp_f64<0x112, 0xf>(v);   // row_shr:2
+  if (G >= 8) v += dpp_f64<0x114, 0xf>(v);   // row_shr:4
+  if (G >= 16) v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  if (G >= 32) v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  if (G >= 64) v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// Entries of one solve record for one thread, held in registers.
+struct SliceRegs {
+  double v[kMaxIter];
+  uint32_t off[kMaxIter];  // LDS byte offsets
+  int32_t row;
+};
+
+// Issue the loads of one record: always the same 7 vector loads, without
+// branches, so that the compiler can keep two records in flight while it
+// processes a third. Lanes without a row and entries past n_iter read the
+// plan's zero page (value 0, LDS offset 0).
+template <int BS>
+__device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice& m, int tid,
+                                           SliceRegs& r) {
+  const int G = m.flags & 0xff;
+  const bool live = tid / G < (m.flags >> 16);
+  r.row = gbl(P.slice_rows)[m.rowbase + tid / G];
+  const int np = (m.n_iter + 1) >> 1, nq = (m.n_iter + 3) >> 2;
+  const GAS vdouble2* __restrict__ vp = gbl(reinterpret_cast<const vdouble2*>(P.ent_val + m.base)) + tid;
+  const GAS vuint4* __restrict__ op = gbl(reinterpret_cast<const vuint4*>(P.ent_off + m.ibase)) + tid * nq;
+  const GAS vdouble2* __restrict__ zd = gbl(reinterpret_cast<const vdouble2*>(P.zero));
+  const GAS vuint4* __restrict__ zo = gbl(reinterpret_cast<const vuint4*>(P.zero));
+#pragma unroll
+  for (int q = 0; q < kMaxIter / 2; ++q) {
+    const vdouble2 d = *((live && q < np) ? vp + q * BS : zd);
+    r.v[2 * q] = d.x;
+    r.v[2 * q + 1] = d.y;
+  }
+#pragma unroll
+  for (int q = 0; q < kMaxIter / 4; ++q) {
+    const vuint4 o = *((live && q < nq) ? op + q : zo);
+    r.off[4 * q] = o.x;
+    r.off[4 * q + 1] = o.y;
+    r.off[4 * q + 2] = o.z;
+    r.off[4 * q + 3] = o.w;
+  }
+}
+
+__device__ __forceinline__ double lds_at(const double* lds, uint32_t byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(lds) + byte_off);
+}
+
+// Gather, accumulate, and (flag FINISH) reduce and write one record's rows.
+__device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t Nf,
+                                              const jja_slice& m, const SliceRegs& r, double& acc) {
+  double zv[kMaxIter];
+#pragma unroll
+  for (int i = 0; i < kMaxIter; ++i) zv[i] = lds_at(lds, r.off[i]);
+#pragma unroll
+  for (int i = 0; i < kMaxIter; ++i) acc = __builtin_fma(r.v[i], zv[i], acc);
+  if (m.flags & JJA_SLICE_FINISH) {
+    const int G = m.flags & 0xff;
+    acc = group_sum_last(acc, G);
+    if (r.row >= 0 && (static_cast<int>(threadIdx.x) & (G - 1)) == G - 1) {
+      if (m.flags & JJA_SLICE_BLOCK)
+        lds[r.row] = acc;
+      else
+        lds[Nf + r.row - m.r0] = lds[r.row] - acc;
+    }
+    acc = 0.0;
+  }
+  if (m.flags & JJA_SLICE_BARRIER) lds_barrier();
+}
+
+// The whole forward + backward solve: a linear walk over the record program,
+// software-pipelined in two register buffers used in turn (the next record is
+// in flight while one is processed; no register copies, so the wait before a
+// record covers only its own loads). The program carries four never-executed
+// records at its end.
+template <int BS>
+__device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ lds) {
+  const int tid = threadIdx.x;
+  const int32_t nk = P.n_solve_slices;
+  const jja_slice* __restrict__ prog = P.solve_slices;
+  jja_slice mA = load_rec(prog, 0), mB;
+  SliceRegs A, B;
+  load_slice<BS>(P, mA, tid, A);
+  double acc = 0.0;
+  for (int32_t k = 0;; k += 2) {
+    mB = load_rec(prog, k + 1);
+    load_slice<BS>(P, mB, tid, B);
+    if (k >= nk) break;
+    process_slice(lds, P.Nf, mA, A, acc);
+    mA = load_rec(prog, k + 2);
+    load_slice<BS>(P, mA, tid, A);
+    if (k + 1 >= nk) break;
+    process_slice(lds, P.Nf, mB, B, acc);
+  }
+}
+
+// Face phase: b_p = sum +-u_j in ascending j, minus 2 pi f (time_evolution.py:560-569).
+// A face longer than one record continues in the next record (flag FINISH on
+// its last one).
+template <int BS>
+__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_problem_run& R,
+                                           const GAS double* __restrict__ u, double* __restrict__ zs,
+                                           int64_t w, int64_t t) {
+  const int tid = threadIdx.x;
+  double acc = 0.0;
+  for (int32_t k = 0; k < P.n_face_slices; ++k) {
+    const jja_slice m = load_rec(P.face_slices, k);
+    const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + tid;
+    const bool live = tid < (m.flags >> 16);
+    int32_t cd[kMaxIter];
+#pragma unroll
+    for (int i = 0; i < kMaxIter; ++i) cd[i] = (live && i < m.n_iter) ? code[i * BS] : -1;
+    double uu[kMaxIter];
+#pragma unroll
+    for (int i = 0; i < kMaxIter; ++i) uu[i] = cd[i] >= 0 ? u[cd[i] >> 1] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kMaxIter; ++i)
+      if (cd[i] >= 0) acc = (cd[i] & 1) ? acc - uu[i] : acc + uu[i];
+    if ((m.flags & JJA_SLICE_FINISH) && live) {
+      const int32_t pos = gbl(P.slice_rows)[m.rowbase + tid];
+      if (pos >= 0) {
+        if (!R.f.zero) acc = acc - kTwoPi * src_at(R.f, gbl(P.face_of_pos)[pos], w, t);
+        zs[pos] = acc;
+      }
+      acc = 0.0;
+    }
+  }
+}
+
+// x of junction j for step t from the phases of the previous steps
+// (time_evolution.py:533-558), in the reference's operation order.
+__device__ __forceinline__ double junction_x(const DevProblem& P, const jja_problem_run& R, int32_t j,
+                                             int64_t w, int64_t t, double t1, double t2, double t3,
+                                             double t4, double z) {
+  const size_t Nj = static_cast<size_t>(P.Nj);
+  const GAS double* __restrict__ c = gbl(P.coef);
+  const double Ic = gbl(P.Ic)[j];
+  double X;
+  if (P.sw == 3) {
+    X = Ic * sin(2.0 * t1 - t2) + c[Nj + j] * t1 + c[2 * Nj + j] * t2;
+  } else if (P.sw == 4) {
+    X = Ic * sin(3.0 * t1 - 3.0 * t2 + t3) + c[Nj + j] * t1 + c[2 * Nj + j] * t2 +
+        c[3 * Nj + j] * t3;
+  } else {
+    X = Ic * sin(4.0 * t1 - 6.0 * t2 + 4.0 * t3 - t4) + c[Nj + j] * t1 + c[2 * Nj + j] * t2 +
+        c[3 * Nj + j] * t3 + c[4 * Nj + j] * t4;
+  }
+  double eta = 0.0;
+  if (!R.T.zero) {
+    const double sig = R.sigma.base ? src_at(R.sigma, j, w, t)
+                                    : sqrt(2.0 * src_at(R.T, j, w, t) * gbl(P.Rv)[j]);
+    eta = sig * z;
+  }
+  return R.Is.zero ? eta + X : (eta - src_at(R.Is, j, w, t)) + X;
+}
+
+// Junction pass. With `update`: finish step t (time_evolution.py:560-580):
+// y = A^T J from the face vector, theta_t = (y - x)/c0, outputs, voltage-source
+// phase. With `prep`: write x (history slot) and u (scratch) of step tx.
+// Each thread takes the junction pair (2p, 2p+1): one Philox draw gives both
+// deviates.
+template <int BS>
+__device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_problem_run& R,
+                                              GAS double* __restrict__ th, GAS double* __restrict__ ths,
+                                              GAS double* __restrict__ u, const double* __restrict__ zs,
+                                              int64_t w, size_t slot_stride, int64_t t, bool update,
+                                              int64_t tx, bool prep, uint32_t noise_col) {
+  const int32_t Nj = P.Nj;
+  const int32_t S = P.sw - 1;
+  auto slot = [&](int64_t m) -> size_t {
+    int64_t r = m % S;
+    return static_cast<size_t>(r < 0 ? r + S : r) * slot_stride;
+  };
+  const size_t sn = slot(t);
+  const size_t sx = slot(tx), sx1 = slot(tx - 1), sx2 = slot(tx - 2);
+  const size_t sx3 = P.sw >= 4 ? slot(tx - 3) : 0, sx4 = P.sw >= 5 ? slot(tx - 4) : 0;
+  const int32_t tho = (update && R.th_slot) ? gbl(R.th_slot)[t - R.slot_t0] : -1;
+  const int32_t Io = (update && R.I_slot) ? gbl(R.I_slot)[t - R.slot_t0] : -1;
+  const GAS double* __restrict__ c0 = gbl(P.coef);
+  const GAS vint2* __restrict__ jcode = gbl(reinterpret_cast<const vint2*>(P.junc_code));
+  GAS double* __restrict__ th_out = gblw(R.th_out);
+  GAS double* __restrict__ I_out = gblw(R.I_out);
+  for (int32_t p = threadIdx.x; 2 * p < Nj; p += BS) {
+    double z[2] = {0.0, 0.0};
+    if (prep && !R.T.zero && !R.noise)
+      normal_pair(R.seed, static_cast<uint32_t>(2 * p), noise_col, static_cast<uint64_t>(tx), &z[0], &z[1]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int32_t j = 2 * p + h;
+      if (j >= Nj) break;
+      double tn = 0.0;
+      if (update) {
+        const vint2 code = jcode[j];
+        double y = 0.0;
+        if (code.x >= 0) y = (code.x & 1) ? y - zs[code.x >> 1] : y + zs[code.x >> 1];
+        if (code.y >= 0) y = (code.y & 1) ? y - zs[code.y >> 1] : y + zs[code.y >> 1];
+        const double x = th[sn + j];
+        tn = (y - x) / c0[j];
+        th[sn + j] = tn;
+        if (tho >= 0) th_out[static_cast<size_t>(tho) * slot_stride + static_cast<size_t>(w) * Nj + j] = tn;
+        if (Io >= 0)
+          I_out[static_cast<size_t>(Io) * slot_stride + static_cast<size_t>(w) * Nj + j] =
+              y + src_at(R.Is, j, w, t);
+        if (ths) ths[j] += src_at(R.Vs, j, w, t) * R.dt;
+      }
+      if (prep) {
+        const double t1 = update ? tn : th[sx1 + j];
+        const double t2 = th[sx2 + j];
+        const double t3 = P.sw >= 4 ? th[sx3 + j] : 0.0;
+        const double t4 = P.sw >= 5 ? th[sx4 + j] : 0.0;
+        double zz = z[h];
+        if (!R.T.zero && R.noise)
+          zz = gbl(R.noise)[static_cast<size_t>(tx - R.noise_t0) * slot_stride + static_cast<size_t>(w) * Nj + j];
+        const double x = junction_x(P, R, j, w, tx, t1, t2, t3, t4, zz);
+        th[sx + j] = x;
+        double uj = x / c0[j];
+        if (ths) uj = uj - ths[j];
+        u[j] = uj;
+      }
+    }
+  }
 }
 
 template <int BS>
@@ -126,160 +402,54 @@ __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict
   const WorkItem wi = work[blockIdx.x];
   const DevProblem& P = probs[wi.problem];
   const jja_problem_run& R = runs[wi.problem];
-  const int tid = threadIdx.x;
-  const int32_t Nj = P.Nj, Nf = P.Nf, sw = P.sw, S = sw - 1;
   const int64_t w = wi.column;
-  const int64_t W = R.W;
-  double* __restrict__ zs = lds;        // face vector, by solve position
-  double* __restrict__ ts = lds + Nf;   // per-stage scratch
-  const size_t slot_stride = static_cast<size_t>(W) * Nj;
-  double* __restrict__ th = R.theta + static_cast<size_t>(w) * Nj;
-  double* __restrict__ ths = R.theta_s ? R.theta_s + static_cast<size_t>(w) * Nj : nullptr;
-  const double* __restrict__ c0 = P.coef;
-  const double* __restrict__ c1 = P.coef + Nj;
-  const double* __restrict__ c2 = P.coef + 2 * static_cast<size_t>(Nj);
-  const double* __restrict__ c3 = P.coef + 3 * static_cast<size_t>(Nj);
-  const double* __restrict__ c4 = P.coef + 4 * static_cast<size_t>(Nj);
+  double* __restrict__ zs = lds;          // face vector, by solve position
+  double* __restrict__ ts = lds + P.Nf;   // stage scratch
+  const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
+  GAS double* __restrict__ th = gblw(R.theta) + static_cast<size_t>(w) * P.Nj;
+  GAS double* __restrict__ ths = R.theta_s ? gblw(R.theta_s) + static_cast<size_t>(w) * P.Nj : nullptr;
+  GAS double* __restrict__ u = gblw(R.u) + static_cast<size_t>(w) * P.Nj;
   const uint32_t noise_col = static_cast<uint32_t>(R.column_offset + w);
 
+  junction_pass<BS>(P, R, th, ths, u, zs, w, slot_stride, step0, false, step0, true, noise_col);
+  __syncthreads();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
-    // history slot of phase at step m is m mod S (m >= -S)
-    auto slot = [&](int64_t m) -> size_t {
-      int64_t r = m % S;
-      return static_cast<size_t>(r < 0 ? r + S : r) * slot_stride;
-    };
-    const size_t s_new = slot(t), s1 = slot(t - 1), s2 = slot(t - 2);
-    const size_t s3 = sw >= 4 ? slot(t - 3) : 0, s4 = sw >= 5 ? slot(t - 4) : 0;
-
-    // ---- junction phase: x_j into the oldest slot (time_evolution.py:533-558)
-    for (int32_t j = tid; j < Nj; j += BS) {
-      const double t1 = th[s1 + j];
-      const double t2 = th[s2 + j];
-      double X;
-      if (sw == 3) {
-        X = P.Ic[j] * sin(2.0 * t1 - t2) + c1[j] * t1 + c2[j] * t2;
-      } else if (sw == 4) {
-        const double t3 = th[s3 + j];
-        X = P.Ic[j] * sin(3.0 * t1 - 3.0 * t2 + t3) + c1[j] * t1 + c2[j] * t2 + c3[j] * t3;
-      } else {
-        const double t3 = th[s3 + j];
-        const double t4 = th[s4 + j];
-        X = P.Ic[j] * sin(4.0 * t1 - 6.0 * t2 + 4.0 * t3 - t4) + c1[j] * t1 + c2[j] * t2 +
-            c3[j] * t3 + c4[j] * t4;
-      }
-      double eta = 0.0;
-      if (!R.T.zero) {
-        const double z = R.noise ? R.noise[static_cast<size_t>(t - R.noise_t0) * slot_stride +
-                                           static_cast<size_t>(w) * Nj + j]
-                                 : philox_normal(R.seed, static_cast<uint32_t>(j), noise_col,
-                                                 static_cast<uint64_t>(t));
-        eta = sqrt(2.0 * src_at(R.T, j, w, t) * P.Rv[j]) * z;
-      }
-      const double x = R.Is.zero ? eta + X : (eta - src_at(R.Is, j, w, t)) + X;
-      th[s_new + j] = x;
-    }
-    __syncthreads();
-
-    // ---- face phase: b = A (x/c0 - ths) - 2 pi f (time_evolution.py:560-569)
-    for (int32_t f = tid; f < Nf; f += BS) {
-      double acc = 0.0;
-      for (int32_t e = P.face_ptr[f]; e < P.face_ptr[f + 1]; ++e) {
-        const int32_t code = P.face_code[e];
-        const int32_t j = code >> 1;
-        double u = th[s_new + j] / c0[j];
-        if (ths) u = u - ths[j];
-        acc = (code & 1) ? acc - u : acc + u;
-      }
-      if (!R.f.zero) acc = acc - kTwoPi * src_at(R.f, f, w, t);
-      zs[P.face_pos[f]] = acc;
-    }
-    __syncthreads();
-
-    // ---- forward solve L z = b, stage by stage
-    for (int32_t s = 0; s < P.n_stages; ++s) {
-      const int32_t r0 = P.stage_ptr[s], r1 = P.stage_ptr[s + 1];
-      for (int32_t r = r0 + tid; r < r1; r += BS) {
-        double acc = zs[r];
-        for (int64_t e = P.lo_ptr[r]; e < P.lo_ptr[r + 1]; ++e)
-          acc = __builtin_fma(-P.lo_val[e], zs[P.lo_col[e]], acc);
-        ts[r - r0] = acc;
-      }
-      __syncthreads();
-      for (int32_t r = r0 + tid; r < r1; r += BS) {
-        const int32_t b0 = P.row_block[r], B = P.row_bsize[r], a = r - b0;
-        const double* __restrict__ inv = P.inv_cm + P.row_boff[r];
-        double z = 0.0;
-        for (int32_t c = 0; c <= a; ++c)
-          z = __builtin_fma(inv[col_start(c, B) + (a - c)], ts[b0 + c - r0], z);
-        (void)B;
-        zs[r] = z;
-      }
-      __syncthreads();
-    }
-    // ---- backward solve L^T J = z
-    for (int32_t s = P.n_stages - 1; s >= 0; --s) {
-      const int32_t r0 = P.stage_ptr[s], r1 = P.stage_ptr[s + 1];
-      for (int32_t r = r0 + tid; r < r1; r += BS) {
-        double acc = zs[r];
-        for (int64_t e = P.up_ptr[r]; e < P.up_ptr[r + 1]; ++e)
-          acc = __builtin_fma(-P.up_val[e], zs[P.up_col[e]], acc);
-        ts[r - r0] = acc;
-      }
-      __syncthreads();
-      for (int32_t r = r0 + tid; r < r1; r += BS) {
-        const int32_t b0 = P.row_block[r], B = P.row_bsize[r], a = r - b0;
-        const double* __restrict__ inv = P.inv_rm + P.row_boff[r];
-        double z = 0.0;
-        for (int32_t c = a; c < B; ++c)
-          z = __builtin_fma(inv[row_start(c) + a], ts[b0 + c - r0], z);
-        zs[r] = z;
-      }
-      __syncthreads();
-    }
-
-    // ---- junction update (time_evolution.py:570-580)
-    const int32_t tho = R.th_slot ? R.th_slot[t - R.slot_t0] : -1;
-    const int32_t Io = R.I_slot ? R.I_slot[t - R.slot_t0] : -1;
-    for (int32_t j = tid; j < Nj; j += BS) {
-      double y = 0.0;
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int32_t code = P.junc_code[2 * static_cast<size_t>(j) + k];
-        if (code >= 0) {
-          const double J = zs[P.face_pos[code >> 1]];
-          y = (code & 1) ? y - J : y + J;
-        }
-      }
-      const double x = th[s_new + j];
-      const double tn = (y - x) / c0[j];
-      th[s_new + j] = tn;
-      if (tho >= 0) R.th_out[static_cast<size_t>(tho) * slot_stride + static_cast<size_t>(w) * Nj + j] = tn;
-      if (Io >= 0)
-        R.I_out[static_cast<size_t>(Io) * slot_stride + static_cast<size_t>(w) * Nj + j] =
-            y + src_at(R.Is, j, w, t);
-      if (ths) ths[j] += src_at(R.Vs, j, w, t) * R.dt;
-    }
+    face_phase<BS>(P, R, u, zs, w, t);
+    lds_barrier();
+    solve<BS>(P, lds);
+    // finish step t and prepare step t+1 in one pass
+    junction_pass<BS>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     __syncthreads();
   }
 }
 
-__global__ void vortex_kernel(const DevProblem* __restrict__ probs, int32_t p,
-                              const double* __restrict__ theta, int32_t cols,
-                              int32_t* __restrict__ n_out) {
+// n[col][f] = -sum_j A[f,j] rint(theta_j / 2pi) (time_evolution.py:734-755): one
+// workgroup per phase vector, one thread per face position.
+template <int BS>
+__global__ __launch_bounds__(BS) void vortex_kernel(const DevProblem* __restrict__ probs, int32_t p,
+                                                    const double* __restrict__ theta,
+                                                    int32_t* __restrict__ n_out) {
   const DevProblem& P = probs[p];
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= static_cast<int64_t>(P.Nf) * cols) return;
-  const int64_t col = idx / P.Nf;
-  const int32_t f = static_cast<int32_t>(idx - col * P.Nf);
-  const double* th = theta + col * P.Nj;
+  const int64_t col = blockIdx.x;
+  const GAS double* __restrict__ th = gbl(theta) + col * P.Nj;
+  const int tid = threadIdx.x;
   int64_t acc = 0;
-  for (int32_t e = P.face_ptr[f]; e < P.face_ptr[f + 1]; ++e) {
-    const int32_t code = P.face_code[e];
-    const int64_t k = static_cast<int64_t>(rint(th[code >> 1] / kTwoPi));
-    acc += (code & 1) ? k : -k;
+  for (int32_t k = 0; k < P.n_face_slices; ++k) {
+    const jja_slice m = load_rec(P.face_slices, k);
+    const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + tid;
+    for (int32_t it = 0; it < m.n_iter; ++it) {
+      const int32_t cd = code[static_cast<int64_t>(it) * BS];
+      if (cd < 0) continue;
+      const int64_t kk = static_cast<int64_t>(rint(th[cd >> 1] / kTwoPi));
+      acc += (cd & 1) ? kk : -kk;
+    }
+    if (m.flags & JJA_SLICE_FINISH) {
+      const int32_t pos = gbl(P.slice_rows)[m.rowbase + tid];
+      if (pos >= 0) gblw(n_out)[col * P.Nf + gbl(P.face_of_pos)[pos]] = static_cast<int32_t>(acc);
+      acc = 0;
+    }
   }
-  n_out[idx] = static_cast<int32_t>(acc);
 }
 
 struct Plan {
@@ -292,6 +462,7 @@ struct Plan {
   WorkItem* d_work = nullptr;
   size_t work_cap = 0;
   int64_t lds_limit = 0;
+  int32_t block_size = 1024;
 };
 
 template <typename T>
@@ -312,53 +483,79 @@ void free_plan(Plan* plan) {
   delete plan;
 }
 
-int validate(const jja_problem_desc& d, int64_t* max_stage_rows) {
-  if (d.n_junctions <= 0 || d.n_faces < 0 || d.stencil_width < 3 || d.stencil_width > 5 ||
-      d.n_stages < 0)
+bool pow2_le(int32_t g, int32_t limit) { return g >= 1 && g <= limit && (g & (g - 1)) == 0; }
+
+// Every index the kernels follow is checked here, once, on the host.
+int validate(const jja_problem_desc& d) {
+  const int32_t Nj = d.n_junctions, Nf = d.n_faces, BS = d.block_size, ms = d.max_stage_rows;
+  if (Nj <= 0 || Nf < 0 || d.stencil_width < 3 || d.stencil_width > 5 || d.n_face_slices < 0 ||
+      d.n_solve_slices < 0 || ms < 0)
     return fail(JJA_EINVAL, "jja_plan_create: bad sizes");
-  if (!d.face_ptr || !d.junc_code || !d.Ic || !d.Rv || !d.coef)
-    return fail(JJA_EINVAL, "jja_plan_create: missing topology or coefficients");
-  const int32_t Nf = d.n_faces, Nj = d.n_junctions;
-  for (int32_t f = 0; f < Nf; ++f)
-    if (d.face_ptr[f + 1] < d.face_ptr[f]) return fail(JJA_EINVAL, "face_ptr not monotone");
-  for (int32_t e = 0; e < d.face_ptr[Nf]; ++e)
-    if (d.face_code[e] < 0 || (d.face_code[e] >> 1) >= Nj)
-      return fail(JJA_EINVAL, "face_code out of range");
+  if (BS != 64 && BS != 128 && BS != 256 && BS != 512 && BS != 1024)
+    return fail(JJA_EINVAL, "jja_plan_create: block_size must be 64, 128, 256, 512 or 1024");
+  if (!d.Ic || !d.Rv || !d.coef || !d.junc_code)
+    return fail(JJA_EINVAL, "jja_plan_create: missing junction arrays");
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
     if (d.junc_code[e] < -1 || (d.junc_code[e] >> 1) >= Nf)
       return fail(JJA_EINVAL, "junc_code out of range");
-  if (Nf > 0) {
-    if (!d.face_pos || !d.stage_ptr || !d.lo_ptr || !d.up_ptr || !d.row_block ||
-        !d.row_bsize || !d.row_boff)
-      return fail(JJA_EINVAL, "jja_plan_create: missing solve schedule");
-    if (d.stage_ptr[0] != 0 || d.stage_ptr[d.n_stages] != Nf)
-      return fail(JJA_EINVAL, "stage_ptr must span all faces");
-    std::vector<char> seen(Nf, 0);
-    for (int32_t f = 0; f < Nf; ++f) {
-      const int32_t q = d.face_pos[f];
-      if (q < 0 || q >= Nf || seen[q]) return fail(JJA_EINVAL, "face_pos is not a permutation");
-      seen[q] = 1;
+  if (Nf == 0) return (d.n_face_slices || d.n_solve_slices) ? fail(JJA_EINVAL, "slices without faces") : JJA_OK;
+  if (!d.face_of_pos || !d.face_code || !d.face_slices || !d.slice_rows ||
+      (d.n_solve_slices && (!d.solve_slices || !d.ent_val || !d.ent_off)))
+    return fail(JJA_EINVAL, "jja_plan_create: missing face or solve arrays");
+  if (Nf > 65536 || ms > 65536) return fail(JJA_EINVAL, "face vector exceeds 16-bit indices");
+  std::vector<char> seen(Nf, 0);
+  for (int32_t q = 0; q < Nf; ++q) {
+    const int32_t f = d.face_of_pos[q];
+    if (f < 0 || f >= Nf || seen[f]) return fail(JJA_EINVAL, "face_of_pos is not a permutation");
+    seen[f] = 1;
+  }
+  auto check_slice = [&](const jja_slice& m, int64_t n_ent, bool paired) -> int {
+    const int32_t G = m.flags & 0xff;
+    if (!pow2_le(G, std::min(64, BS))) return fail(JJA_EINVAL, "slice lanes per row must be a power of two <= 64");
+    if (m.n_iter < 0 || m.n_iter > kMaxIter) return fail(JJA_EINVAL, "slice n_iter must be 0..8");
+    // face records: n_iter*BS codes; solve records: ceil(n_iter/2) pairs of BS doubles
+    const int64_t extent = paired ? 2 * static_cast<int64_t>((m.n_iter + 1) / 2) * BS
+                                  : static_cast<int64_t>(m.n_iter) * BS;
+    if (m.base < 0 || m.base + extent > n_ent) return fail(JJA_EINVAL, "slice entries out of range");
+    if (m.rowbase < 0 || m.rowbase + BS / G > d.n_slice_rows) return fail(JJA_EINVAL, "slice rows out of range");
+    if ((m.flags >> 16) < 0 || (m.flags >> 16) > BS / G) return fail(JJA_EINVAL, "slice row count out of range");
+    if (m.flags & 0xf800) return fail(JJA_EINVAL, "unknown slice flags");
+    return JJA_OK;
+  };
+  for (int32_t k = 0; k < d.n_face_slices; ++k) {
+    const jja_slice& m = d.face_slices[k];
+    if (int rc = check_slice(m, d.n_face_entries, false)) return rc;
+    if ((m.flags & 0xff) != 1) return fail(JJA_EINVAL, "face slices use one lane per row");
+    for (int32_t i = 0; i < BS; ++i) {
+      const int32_t r = d.slice_rows[m.rowbase + i];
+      if (r < -1 || r >= Nf) return fail(JJA_EINVAL, "face slice row out of range");
     }
-    int64_t ms = 0;
-    for (int32_t s = 0; s < d.n_stages; ++s) {
-      const int32_t r0 = d.stage_ptr[s], r1 = d.stage_ptr[s + 1];
-      if (r1 < r0) return fail(JJA_EINVAL, "stage_ptr not monotone");
-      ms = std::max<int64_t>(ms, r1 - r0);
-      for (int32_t r = r0; r < r1; ++r) {
-        const int32_t b0 = d.row_block[r], B = d.row_bsize[r];
-        if (b0 < r0 || B <= 0 || r < b0 || r >= b0 + B || b0 + B > r1)
-          return fail(JJA_EINVAL, "block rows must stay inside their stage");
-        const int64_t need = d.row_boff[r] + static_cast<int64_t>(B) * (B + 1) / 2;
-        if (d.row_boff[r] < 0 || need > d.n_inv) return fail(JJA_EINVAL, "row_boff out of range");
-        for (int64_t e = d.lo_ptr[r]; e < d.lo_ptr[r + 1]; ++e)
-          if (d.lo_col[e] < 0 || d.lo_col[e] >= r0) return fail(JJA_EINVAL, "lo_col must point to earlier stages");
-        for (int64_t e = d.up_ptr[r]; e < d.up_ptr[r + 1]; ++e)
-          if (d.up_col[e] < r1 || d.up_col[e] >= Nf) return fail(JJA_EINVAL, "up_col must point to later stages");
-      }
+    for (int64_t e = m.base; e < m.base + static_cast<int64_t>(m.n_iter) * BS; ++e)
+      if (d.face_code[e] < -1 || (d.face_code[e] >> 1) >= Nj) return fail(JJA_EINVAL, "face_code out of range");
+  }
+  // the four records after the program are loaded ahead, never executed
+  for (int32_t k = d.n_solve_slices; k < d.n_solve_slices + 4; ++k)
+    if (int rc = check_slice(d.solve_slices[k], d.n_entries, true)) return rc;
+  for (int32_t k = 0; k < d.n_solve_slices; ++k) {
+    const jja_slice& m = d.solve_slices[k];
+    if (int rc = check_slice(m, d.n_entries, true)) return rc;
+    const bool block = m.flags & JJA_SLICE_BLOCK;
+    const int32_t G = m.flags & 0xff;
+    for (int32_t i = 0; i < BS / G; ++i) {
+      const int32_t r = d.slice_rows[m.rowbase + i];
+      if (r == -1) continue;
+      if (r < 0 || r >= Nf || (!block && (r - m.r0 < 0 || r - m.r0 >= ms)))
+        return fail(JJA_EINVAL, "solve slice row out of range");
     }
-    *max_stage_rows = ms;
-  } else {
-    *max_stage_rows = 0;
+    const int64_t NQ = (m.n_iter + 3) / 4;
+    if (m.ibase < 0 || (m.ibase % 4) || m.ibase + 4 * NQ * BS > d.n_index || (m.base % 2))
+      return fail(JJA_EINVAL, "solve record offsets out of range or misaligned");
+    // off-block records read z (bytes [0, 8 Nf)), block records read t
+    const int64_t lo = block ? 8LL * Nf : 0, hi = block ? 8LL * (Nf + ms) : 8LL * Nf;
+    for (int64_t e = m.ibase; e < m.ibase + 4 * NQ * BS; ++e) {
+      const int64_t o = d.ent_off[e];
+      if (o < lo || o + 8 > hi || (o & 7)) return fail(JJA_EINVAL, "solve LDS offset out of range");
+    }
   }
   return JJA_OK;
 }
@@ -375,6 +572,15 @@ int launch_evolve(Plan* plan, int32_t n_work, int64_t lds, int32_t n_steps, int6
   return JJA_OK;
 }
 
+template <int BS>
+int launch_vortex(Plan* plan, int32_t p, const double* theta, int32_t cols, int32_t* n_out,
+                  hipStream_t stream) {
+  hipLaunchKernelGGL(vortex_kernel<BS>, dim3(cols), dim3(BS), 0, stream, plan->d_problems, p, theta,
+                     n_out);
+  JJA_HIP(hipGetLastError());
+  return JJA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -384,7 +590,11 @@ const char* jja_last_error(void) { return g_error.c_str(); }
 int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan** out) {
   if (!out || n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_create: invalid arguments");
   *out = nullptr;
+  for (int32_t p = 1; p < n_problems; ++p)
+    if (descs[p].block_size != descs[0].block_size)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size");
   Plan* plan = new Plan();
+  plan->block_size = descs[0].block_size;
   int rc = JJA_OK;
   hipDeviceProp_t prop;
   if (hipGetDevice(&plan->device) != hipSuccess ||
@@ -395,43 +605,37 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
   plan->lds_limit = static_cast<int64_t>(prop.sharedMemPerBlock);
   for (int32_t p = 0; p < n_problems && rc == JJA_OK; ++p) {
     const jja_problem_desc& d = descs[p];
-    int64_t ms = 0;
-    rc = validate(d, &ms);
+    rc = validate(d);
     if (rc) break;
     DevProblem dp{};
     dp.Nj = d.n_junctions;
     dp.Nf = d.n_faces;
     dp.sw = d.stencil_width;
-    dp.n_stages = d.n_stages;
-    dp.max_stage_rows = static_cast<int32_t>(ms);
+    dp.BS = d.block_size;
+    dp.n_face_slices = d.n_face_slices;
+    dp.n_solve_slices = d.n_solve_slices;
+    dp.max_stage_rows = d.max_stage_rows;
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
-    const size_t nnzA = static_cast<size_t>(d.face_ptr[Nf]);
-    const size_t nlo = Nf ? static_cast<size_t>(d.lo_ptr[Nf]) : 0;
-    const size_t nup = Nf ? static_cast<size_t>(d.up_ptr[Nf]) : 0;
-    if ((rc = upload(plan, d.face_ptr, Nf + 1, &dp.face_ptr))) break;
-    if ((rc = upload(plan, d.face_code, nnzA, &dp.face_code))) break;
-    if ((rc = upload(plan, d.junc_code, 2 * Nj, &dp.junc_code))) break;
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
     if ((rc = upload(plan, d.coef, Nj * d.stencil_width, &dp.coef))) break;
+    if ((rc = upload(plan, d.junc_code, 2 * Nj, &dp.junc_code))) break;
     if (Nf) {
-      if ((rc = upload(plan, d.face_pos, Nf, &dp.face_pos))) break;
-      if ((rc = upload(plan, d.stage_ptr, static_cast<size_t>(d.n_stages) + 1, &dp.stage_ptr))) break;
-      if ((rc = upload(plan, d.lo_ptr, Nf + 1, &dp.lo_ptr))) break;
-      if ((rc = upload(plan, d.lo_col, nlo, &dp.lo_col))) break;
-      if ((rc = upload(plan, d.lo_val, nlo, &dp.lo_val))) break;
-      if ((rc = upload(plan, d.up_ptr, Nf + 1, &dp.up_ptr))) break;
-      if ((rc = upload(plan, d.up_col, nup, &dp.up_col))) break;
-      if ((rc = upload(plan, d.up_val, nup, &dp.up_val))) break;
-      if ((rc = upload(plan, d.row_block, Nf, &dp.row_block))) break;
-      if ((rc = upload(plan, d.row_bsize, Nf, &dp.row_bsize))) break;
-      if ((rc = upload(plan, d.row_boff, Nf, &dp.row_boff))) break;
-      if ((rc = upload(plan, d.inv_cm, static_cast<size_t>(d.n_inv), &dp.inv_cm))) break;
-      if ((rc = upload(plan, d.inv_rm, static_cast<size_t>(d.n_inv), &dp.inv_rm))) break;
+      if ((rc = upload(plan, d.face_of_pos, Nf, &dp.face_of_pos))) break;
+      if ((rc = upload(plan, d.face_slices, static_cast<size_t>(d.n_face_slices), &dp.face_slices))) break;
+      if ((rc = upload(plan, d.solve_slices, static_cast<size_t>(d.n_solve_slices) + 4, &dp.solve_slices))) break;
+      if ((rc = upload(plan, d.face_code, static_cast<size_t>(d.n_face_entries), &dp.face_code))) break;
+      if ((rc = upload(plan, d.slice_rows, static_cast<size_t>(d.n_slice_rows), &dp.slice_rows))) break;
+      if ((rc = upload(plan, d.ent_val, static_cast<size_t>(d.n_entries), &dp.ent_val))) break;
+      if ((rc = upload(plan, d.ent_off, static_cast<size_t>(d.n_index), &dp.ent_off))) break;
+    }
+    {
+      static const double zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if ((rc = upload(plan, zeros, 8, &dp.zero))) break;
     }
     plan->problems.push_back(dp);
-    const int64_t bytes = (static_cast<int64_t>(d.n_faces) + ms) * static_cast<int64_t>(sizeof(double));
-    plan->lds_bytes.push_back(bytes);
+    plan->lds_bytes.push_back((static_cast<int64_t>(d.n_faces) + d.max_stage_rows) *
+                              static_cast<int64_t>(sizeof(double)));
   }
   if (rc == JJA_OK) {
     const DevProblem* dprob = nullptr;
@@ -468,7 +672,7 @@ int64_t jja_plan_lds_bytes(const jja_plan* p, int32_t i) {
 }
 
 int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_t step0,
-               int32_t block_size, void* stream_ptr) {
+               void* stream_ptr) {
   Plan* plan = reinterpret_cast<Plan*>(p);
   if (!plan || !runs || n_steps < 0 || step0 < 0) return fail(JJA_EINVAL, "jja_evolve: invalid arguments");
   if (n_steps == 0) return JJA_OK;
@@ -477,8 +681,7 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
   int64_t lds = 0;
   for (int32_t i = 0; i < np; ++i) {
     const jja_problem_run& r = runs[i];
-    const DevProblem& d = plan->problems[i];
-    if (r.W <= 0 || !r.theta) return fail(JJA_EINVAL, "jja_evolve: run needs W > 0 and theta");
+    if (r.W <= 0 || !r.theta || !r.u) return fail(JJA_EINVAL, "jja_evolve: run needs W > 0, theta and u");
     if (r.dt <= 0) return fail(JJA_EINVAL, "jja_evolve: dt must be positive");
     if (!r.Is.base || !r.Vs.base || !r.f.base || !r.T.base)
       return fail(JJA_EINVAL, "jja_evolve: every source needs a base array");
@@ -487,11 +690,9 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
       return fail(JJA_EUNSUPPORTED, "jja_evolve: face vector of problem " + std::to_string(i) +
                                         " needs " + std::to_string(plan->lds_bytes[i]) +
                                         " bytes of LDS; limit " + std::to_string(plan->lds_limit));
-    (void)d;
     lds = std::max(lds, plan->lds_bytes[i]);
     for (int32_t w = 0; w < r.W; ++w) work.push_back({i, w});
   }
-  if (work.empty()) return JJA_OK;
   if (work.size() > plan->work_cap) {
     if (plan->d_work) {
       auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_work));
@@ -513,14 +714,12 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
                          hipMemcpyHostToDevice, stream));
   const int32_t n_work = static_cast<int32_t>(work.size());
   if (lds < 8) lds = 8;
-  switch (block_size) {
+  switch (plan->block_size) {
     case 64: return launch_evolve<64>(plan, n_work, lds, n_steps, step0, stream);
     case 128: return launch_evolve<128>(plan, n_work, lds, n_steps, step0, stream);
     case 256: return launch_evolve<256>(plan, n_work, lds, n_steps, step0, stream);
     case 512: return launch_evolve<512>(plan, n_work, lds, n_steps, step0, stream);
-    case 0:
-    case 1024: return launch_evolve<1024>(plan, n_work, lds, n_steps, step0, stream);
-    default: return fail(JJA_EINVAL, "jja_evolve: block_size must be 64, 128, 256, 512 or 1024");
+    default: return launch_evolve<1024>(plan, n_work, lds, n_steps, step0, stream);
   }
 }
 
@@ -529,14 +728,15 @@ int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_
   Plan* plan = reinterpret_cast<Plan*>(p);
   if (!plan || i < 0 || i >= static_cast<int32_t>(plan->problems.size()) || !theta || !n_out || cols < 0)
     return fail(JJA_EINVAL, "jja_vortex: invalid arguments");
-  const int64_t total = static_cast<int64_t>(plan->problems[i].Nf) * cols;
-  if (total == 0) return JJA_OK;
-  const int threads = 256;
-  const int64_t blocks = (total + threads - 1) / threads;
-  hipLaunchKernelGGL(vortex_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(threads), 0,
-                     static_cast<hipStream_t>(stream_ptr), plan->d_problems, i, theta, cols, n_out);
-  JJA_HIP(hipGetLastError());
-  return JJA_OK;
+  if (cols == 0 || plan->problems[i].Nf == 0) return JJA_OK;
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  switch (plan->block_size) {
+    case 64: return launch_vortex<64>(plan, i, theta, cols, n_out, stream);
+    case 128: return launch_vortex<128>(plan, i, theta, cols, n_out, stream);
+    case 256: return launch_vortex<256>(plan, i, theta, cols, n_out, stream);
+    case 512: return launch_vortex<512>(plan, i, theta, cols, n_out, stream);
+    default: return launch_vortex<1024>(plan, i, theta, cols, n_out, stream);
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,255 @@
+"""Device engine behind TimeEvolutionProblem.compute(): one libjja plan plus the
+device-resident state, inputs and outputs of one time evolution.
+
+``Engine(problem, th_mask, I_mask)`` performs the reference's set-up
+(time_evolution.py:466-521) on the host and uploads everything once;
+``advance(n)`` integrates n further steps with persistent kernel launches;
+``results()`` downloads the stored steps in the reference's layout and the vortex
+counts computed on the device. Device memory is owned by torch tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+from .circuit import DefaultCPR
+from .solver_plan import CircuitPlan
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyjjasim_amd: the time evolution needs a HIP (ROCm) device; "
+                           "none is visible")
+    return torch
+
+
+class DeviceSource:
+    """Device copy of one input (f, Is, Vs or T) plus its jja_source descriptor.
+
+    Arrays are uploaded un-broadcast; numpy's broadcast strides (0 on broadcast
+    axes) become the descriptor's element strides. Callables are evaluated on the
+    host for each launch's chunk of steps.
+    """
+
+    def __init__(self, torch, value, n, W, Nt, zero):
+        self.callable = callable(value)
+        self.value = value
+        self.n, self.W = n, W
+        self.src = _native.Source()
+        self.src.zero = int(bool(zero))
+        self.tensor = None
+        if not self.callable:
+            arr = np.ascontiguousarray(np.array(value, dtype=np.float64))
+            view = np.broadcast_to(arr, (n, W, Nt))
+            self.tensor = torch.from_numpy(arr.reshape(-1).copy()).cuda()
+            self.src.base = self.tensor.data_ptr()
+            self.src.s_i, self.src.s_w, self.src.s_t = (s // 8 for s in view.strides)
+            self.src.t0 = 0
+            self.bytes_per_step = 8 * int(np.prod([d for d, st in zip((n, W), view.strides[:2]) if st]))
+        else:
+            self.bytes_per_step = 8 * n * W
+
+    def fill_chunk(self, torch, t0, n_steps):
+        if not self.callable:
+            return
+        buf = np.empty((n_steps, self.W, self.n))
+        for k in range(n_steps):
+            buf[k] = np.broadcast_to(self.value(t0 + k), (self.n, self.W)).T
+        self.tensor = torch.from_numpy(buf.reshape(-1)).cuda()
+        self.src.base = self.tensor.data_ptr()
+        self.src.s_i, self.src.s_w, self.src.s_t = 1, self.n, self.n * self.W
+        self.src.t0 = t0
+
+
+def reference_noise(Nj, W):
+    """The reference's legacy-RNG draw order (time_evolution.py:533-537)."""
+    state = {}
+
+    def draw(i):
+        if Nj > 500:
+            if i % 3 == 0:
+                state["r"] = np.random.randn(Nj, W)
+            else:
+                state["r"] = state["r"][np.random.permutation(Nj), :]
+        else:
+            state["r"] = np.random.randn(Nj, W)
+        return state["r"]
+    return draw
+
+
+def default_block_size(Nj, Nf):
+    work = max(Nj, Nf)
+    for bs in (64, 128, 256, 512):
+        if work <= 2 * bs:
+            return bs
+    return 1024
+
+
+class Engine:
+    def __init__(self, problem, th_mask, I_mask):
+        if not isinstance(problem.current_phase_relation, DefaultCPR):
+            raise ValueError("the GPU time evolution evaluates the default current-phase "
+                             "relation Ic*sin(theta) only; got a custom CurrentPhaseRelation")
+        torch = self.torch = _torch()
+        self.problem = problem
+        circuit = problem.circuit
+        Nj, Nf, W, Nt = circuit._Nj(), circuit._Nf(), problem.problem_count, problem._Nt()
+        self.Nj, self.Nf, self.W, self.Nt = Nj, Nf, W, Nt
+        dt = problem._dt()
+        sw = problem.stencil_width
+        S = self.S = sw - 1
+        self.block_size = problem.block_size or default_block_size(Nj, Nf)
+        self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
+                                block_size=self.block_size)
+        self.lib = lib = _native.lib()
+        self._desc = self.host.desc()
+        self.handle = C.c_void_p()
+        _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
+        if lib.jja_plan_lds_bytes(self.handle, 0) == 0:
+            self.close()
+            raise _native.NativeError(
+                f"circuit with {Nf} faces needs {self.host.lds_bytes()} bytes of LDS per "
+                "column; larger than one workgroup's LDS")
+        dev = self.dev = torch.device("cuda")
+        f64 = torch.float64
+        cfgs = [problem.config_at_minus_1, problem.config_at_minus_2]
+        if sw >= 4:
+            cfgs.append(problem.config_at_minus_3)
+        if sw >= 5:
+            cfgs.append(problem.config_at_minus_4)
+        self.cfgs = cfgs
+        hist = np.zeros((S, W, Nj))
+        for k, cfg in enumerate(cfgs):          # theta at step m sits in slot m mod S
+            hist[(-(k + 1)) % S] = np.asarray(cfg, dtype=np.float64).reshape(Nj, W).T
+        self.theta = torch.from_numpy(hist).to(dev)
+        self.srcs = {}
+        for name, val, n, td in (("Is", problem.current_sources, Nj, problem._Is_is_timedep),
+                                 ("Vs", problem.voltage_sources, Nj, problem._Vs_is_timedep),
+                                 ("f", problem.external_flux, Nf, problem._f_is_timedep),
+                                 ("T", problem.temperature, Nj, problem._T_is_timedep)):
+            sample = np.broadcast_to(val(0), (n, W)) if callable(val) else val[:, :, 0]
+            zero = (not td) and bool(np.allclose(sample, 0))       # time_evolution.py:509-519
+            self.srcs[name] = DeviceSource(torch, problem._raw[name], n, W, Nt, zero)
+        self.theta_s = torch.zeros((W, Nj), dtype=f64, device=dev) \
+            if not self.srcs["Vs"].src.zero else None
+        self.u = torch.zeros((W, Nj), dtype=f64, device=dev)
+        # noise amplitude sqrt(2 T Rv), formed once when T is constant in time
+        # (the same expression as time_evolution.py:538, correctly rounded sqrt)
+        self.sigma = None
+        T_src = self.srcs["T"]
+        if not T_src.src.zero and not T_src.callable and not problem._T_is_timedep:
+            Traw = np.broadcast_to(np.array(problem._raw["T"], dtype=np.float64), (Nj, W, Nt))[:, :, 0]
+            sig = np.sqrt(2.0 * Traw * self.host.Rv[:, None])
+            self.sigma = DeviceSource(torch, sig[:, :, None] if np.ptp(sig, axis=1).any() else sig[:, :1, None],
+                                      Nj, W, Nt, False)
+        self.th_mask, self.I_mask = th_mask, I_mask
+        th_slot = np.where(th_mask, np.cumsum(th_mask) - 1, -1).astype(np.int32)
+        I_slot = np.where(I_mask, np.cumsum(I_mask) - 1, -1).astype(np.int32)
+        self.n_th, self.n_I = int(th_mask.sum()), int(I_mask.sum())
+        self.th_slot = torch.from_numpy(th_slot).to(dev)
+        self.I_slot = torch.from_numpy(I_slot).to(dev)
+        self.th_out = torch.empty((max(self.n_th, 1), W, Nj), dtype=f64, device=dev)
+        self.I_out = torch.empty((max(self.n_I, 1), W, Nj), dtype=f64, device=dev)
+        run = self.run = _native.ProblemRun()
+        run.W = W
+        run.column_offset = 0
+        run.dt = dt
+        run.theta = self.theta.data_ptr()
+        run.theta_s = self.theta_s.data_ptr() if self.theta_s is not None else None
+        run.u = self.u.data_ptr()
+        if self.sigma is not None:
+            run.sigma = self.sigma.src
+        run.th_slot = self.th_slot.data_ptr()
+        run.I_slot = self.I_slot.data_ptr()
+        run.slot_t0 = 0
+        run.th_out = self.th_out.data_ptr()
+        run.I_out = self.I_out.data_ptr()
+        T_zero = self.srcs["T"].src.zero
+        self.ref_noise = (not T_zero) and problem.noise == "reference"
+        if not T_zero and problem.noise == "philox":
+            run.seed = int(problem.seed) if problem.seed is not None \
+                else int(np.random.randint(0, 2 ** 62, dtype=np.int64))
+        self.draw = reference_noise(Nj, W) if self.ref_noise else None
+        self.host_chunks = self.ref_noise or any(s.callable for s in self.srcs.values())
+        spl = problem.steps_per_launch
+        self.steps_per_launch = spl if spl else (500 if self.host_chunks else 2000)
+        self.t = 0
+        self._noise_buf = None
+
+    def launch(self, n_steps, stream=None):
+        """One persistent launch of n_steps steps starting at the current step."""
+        torch = self.torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        for s in self.srcs.values():
+            s.fill_chunk(torch, self.t, n_steps)
+        if self.ref_noise:
+            z = np.empty((n_steps, self.W, self.Nj))
+            for k in range(n_steps):
+                z[k] = self.draw(self.t + k).T
+            self._noise_buf = torch.from_numpy(z.reshape(-1)).to(self.dev)
+            self.run.noise = self._noise_buf.data_ptr()
+            self.run.noise_t0 = self.t
+        run = self.run
+        run.Is, run.Vs, run.f, run.T = (self.srcs[k].src for k in ("Is", "Vs", "f", "T"))
+        _native.check(self.lib.jja_evolve(self.handle, C.byref(run), n_steps, self.t, stream))
+        if self.host_chunks:
+            torch.cuda.current_stream().synchronize()      # chunk buffers are reused
+        self.t += n_steps
+
+    def advance(self, n_steps):
+        if self.t + n_steps > self.Nt:
+            raise ValueError("advance beyond time_step_count")
+        end = self.t + n_steps
+        while self.t < end:
+            self.launch(min(self.steps_per_launch, end - self.t))
+
+    def results(self):
+        torch = self.torch
+        Nj, W, S = self.Nj, self.W, self.S
+        stream = torch.cuda.current_stream().cuda_stream
+        n_dev = None
+        if self.n_th and self.Nf:
+            n_dev = torch.empty((self.n_th, W, self.Nf), dtype=torch.int32, device=self.dev)
+            _native.check(self.lib.jja_vortex(self.handle, 0, self.th_out.data_ptr(),
+                                              self.n_th * W, n_dev.data_ptr(), stream))
+        torch.cuda.synchronize()
+        th = self.th_out[:self.n_th].cpu().numpy().transpose(2, 1, 0) if self.n_th \
+            else np.zeros((Nj, W, 0))
+        I = self.I_out[:self.n_I].cpu().numpy().transpose(2, 1, 0) if self.n_I \
+            else np.zeros((Nj, W, 0))
+        n = None
+        if n_dev is not None:
+            n = n_dev.cpu().numpy().astype(np.int64).transpose(2, 1, 0)
+        elif self.n_th:
+            n = np.zeros((0, W, self.n_th), dtype=np.int64)
+        # prefix of initial conditions (time_evolution.py:484-502)
+        pre = np.stack([np.asarray(self.cfgs[S - 1 - k]).reshape(Nj, W) for k in range(S)], axis=2)
+        Ic = self.problem.circuit._Ic()[:, None]
+        cp = self.problem.current_phase_relation
+        pre_I = np.stack([cp.eval(Ic, pre[:, :, k]) for k in range(S)], axis=2)
+        return (np.concatenate((pre, th), axis=2), np.concatenate((pre_I, I), axis=2), n)
+
+    def bytes_per_step(self):
+        """Algorithmic HBM bytes of one time step of the whole batch:
+        W (24 Nj + 48 Nf) + factor + 8 Nj k_par + sources + outputs, with the
+        factor counted once per step as values plus 2-byte indices."""
+        sp = self.host.solve_plan
+        factor = 10 * (len(sp.lo_val) + len(sp.up_val)) + 10 * (len(sp.inv_cm) + len(sp.inv_rm))
+        src = sum(s.bytes_per_step for k, s in self.srcs.items() if not s.src.zero)
+        out = 8 * self.Nj * self.W * (self.n_th + self.n_I) / max(self.Nt, 1)
+        return (self.W * (24 * self.Nj + 48 * self.Nf) + factor + 8 * self.Nj * 5 + src + out)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.jja_plan_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -23,6 +23,8 @@ import scipy.sparse
 from . import _native
 from .ordering import nested_dissection
 
+MAX_ITER = 8          # entries per thread and slice record (kernel register budget)
+
 STENCILS = {
     3: ((1.0, -1.0, 0.0), (1.0, -2.0, 1.0)),
     4: ((1.5, -2.0, 0.5, 0.0), (2.0, -5.0, 4.0, -1.0)),
@@ -208,6 +210,79 @@ class SolvePlan:
                    + 4 * 4 * self.n)
 
 
+def _pow2ceil(x):
+    g = 1
+    while g < x:
+        g *= 2
+    return g
+
+
+def sliced(row_ptr, idx, val, BS, row_ids, target_iter=4, max_group=64, fixed_group=None):
+    """Sliced layout of a set of rows (CSR) for workgroups of BS threads.
+
+    A row of length n gets G lanes, the smallest power of two (<= max_group) with
+    at most ``target_iter`` entries per lane; rows of equal G are taken longest
+    first, BS/G per slice; lane l of a row takes its entries l, l+G, ...; entry `it`
+    of thread tid sits at base + it*BS + tid. Row slots without a row hold -1.
+
+    Returns dict with per-slice base, iters, group, rowbase, and the flat arrays
+    rows (row id per slot), idx, val (padding: index 0, value 0).
+    """
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    nrows = len(row_ptr) - 1
+    lens = np.diff(row_ptr)
+    # lanes per row depend on the row's own length only, so the summation order
+    # of a row (and the bits of the result) does not depend on BS
+    if fixed_group:
+        gsel = np.full(nrows, fixed_group, dtype=np.int64)
+    else:
+        need = np.maximum(1, -(-lens // target_iter))
+        gsel = np.minimum(max_group, 2 ** np.ceil(np.log2(need)).astype(np.int64))
+    order = np.lexsort((-lens, -gsel))
+    bases, iters, groups, rowbases, rows = [], [], [], [], []
+    row_slice = np.zeros(nrows, dtype=np.int64)
+    row_slot = np.zeros(nrows, dtype=np.int64)
+    row_G = np.ones(nrows, dtype=np.int64)
+    base = 0
+    rb = 0
+    i = 0
+    while i < nrows:
+        G = int(gsel[order[i]])
+        R = BS // G
+        take = order[i:i + R]
+        take = take[gsel[take] == G]
+        L0 = int(lens[take[0]])
+        it = -(-L0 // G)
+        k = len(bases)
+        row_slice[take] = k
+        row_slot[take] = np.arange(len(take))
+        row_G[take] = G
+        slot_rows = -np.ones(R, dtype=np.int64)
+        slot_rows[:len(take)] = np.asarray(row_ids)[take]
+        bases.append(base)
+        iters.append(it)
+        groups.append(G)
+        rowbases.append(rb)
+        rows.append(slot_rows)
+        base += it * BS
+        rb += R
+        i += len(take)
+    idx_out = np.zeros(base, dtype=np.int64)
+    val_out = np.zeros(base, dtype=np.float64)
+    nnz = int(row_ptr[-1] - row_ptr[0]) if nrows else 0
+    if nnz:
+        r = np.repeat(np.arange(nrows), lens)
+        kk = np.arange(nnz) - np.repeat(row_ptr[:-1] - row_ptr[0], lens)
+        G = row_G[r]
+        pos = np.asarray(bases, dtype=np.int64)[row_slice[r]] + (kk // G) * BS + row_slot[r] * G + kk % G
+        idx_out[pos] = idx
+        val_out[pos] = val
+    return {"base": np.asarray(bases, dtype=np.int64), "iters": np.asarray(iters, dtype=np.int64),
+            "group": np.asarray(groups, dtype=np.int64), "rowbase": np.asarray(rowbases, dtype=np.int64),
+            "rows": np.concatenate(rows) if rows else np.zeros(0, np.int64),
+            "idx": idx_out, "val": val_out}
+
+
 def cycle_codes(A, Nf, Nj):
     """Face-wise and junction-wise index arrays of the cycle matrix.
 
@@ -235,14 +310,16 @@ def cycle_codes(A, Nf, Nj):
 
 
 class CircuitPlan:
-    """All host arrays of one circuit for ``jja_plan_create``."""
+    """All host arrays of one circuit for ``jja_plan_create``, laid out for
+    workgroups of ``block_size`` threads."""
 
-    def __init__(self, circuit, dt, stencil_width=3, leaf_size=16):
+    def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8):
         if stencil_width not in STENCILS:
             raise ValueError(f"stencil width must be 3, 4 or 5 (equals {stencil_width})")
         self.Nj, self.Nf = circuit._Nj(), circuit._Nf()
         self.stencil_width = stencil_width
         self.dt = dt
+        self.block_size = block_size
         A = circuit.get_cycle_matrix()
         self.face_ptr, self.face_code, self.junc_code = cycle_codes(A, self.Nf, self.Nj)
         Rv = 1 / (dt * circuit._R()[:, None])
@@ -259,6 +336,132 @@ class CircuitPlan:
         else:
             xy = np.zeros((0, 2))
         self.solve_plan = SolvePlan(self.K, xy, leaf_size=leaf_size)
+        self._build_layouts(target_iter)
+
+    def _build_layouts(self, target_iter=8):
+        BS, sp, Nf, Nj = self.block_size, self.solve_plan, self.Nf, self.Nj
+        if Nf and (Nf > 65536 or sp.max_stage_rows > 65536):
+            raise ValueError("face count exceeds the 16-bit LDS index range")
+        pos = sp.face_pos.astype(np.int64)
+        # junctions: face positions of the (up to two) faces, ascending face index
+        jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
+        jpos = np.where(jc >= 0, 2 * pos[np.maximum(jc, 0) >> 1] + (jc & 1), -1)
+        self.junc_pos_code = jpos.astype(np.int32).ravel()
+        self.face_of_pos = np.asarray(sp.perm, dtype=np.int32) if Nf else np.zeros(0, np.int32)
+        rows_all = []
+        n_rows = 0
+
+        def records(L, ent_offset, r0, kind_flags, barrier):
+            """Split slices into <= MAX_ITER-entry records (one row may span several)."""
+            nonlocal n_rows
+            out = []
+            for k in range(len(L["base"])):
+                it, G = int(L["iters"][k]), int(L["group"][k])
+                R = BS // G
+                live = int(np.count_nonzero(L["rows"][L["rowbase"][k]:L["rowbase"][k] + R] >= 0))
+                n_chunks = max(1, -(-it // MAX_ITER))
+                for c in range(n_chunks):
+                    flags = G | kind_flags | (live << 16)
+                    if c == n_chunks - 1:
+                        flags |= _native.SLICE_FINISH
+                    out.append((int(L["base"][k]) + ent_offset + c * MAX_ITER * BS,
+                                int(L["rowbase"][k]) + n_rows, r0,
+                                min(MAX_ITER, it - c * MAX_ITER), flags))
+            if barrier and out:
+                b, rb, rr, it, fl = out[-1]
+                out[-1] = (b, rb, rr, it, fl | _native.SLICE_BARRIER)
+            rows_all.append(L["rows"])
+            n_rows += len(L["rows"])
+            return out
+
+        # faces by position, one thread each; entries: junction codes (sign in bit 0)
+        fp = self.face_ptr.astype(np.int64)
+        lens = np.diff(fp)[self.face_of_pos] if Nf else np.zeros(0, np.int64)
+        ptr_by_pos = np.concatenate(([0], np.cumsum(lens)))
+        codes = np.concatenate([self.face_code[fp[f]:fp[f + 1]] for f in self.face_of_pos]) \
+            if Nf else np.zeros(0, np.int64)
+        F = sliced(ptr_by_pos, codes, np.ones(len(codes)), BS, np.arange(Nf), fixed_group=1,
+                   target_iter=MAX_ITER)
+        self.face_code_sliced = np.where(F["val"] != 0, F["idx"], -1).astype(np.int32)
+        self.face_slices = records(F, 0, 0, 0, False)
+        # solve program: forward stages ascending, backward stages descending
+        rb, bsz, boff = sp.row_block.astype(np.int64), sp.row_bsize.astype(np.int64), sp.row_boff
+        per_stage = []
+        for s in range(sp.n_stages):
+            r0, r1 = int(sp.stage_ptr[s]), int(sp.stage_ptr[s + 1])
+            rows = np.arange(r0, r1)
+            a = rows - rb[rows]
+            B = bsz[rows]
+            parts = [(sp.lo_ptr[r0:r1 + 1] - sp.lo_ptr[r0], sp.lo_col[sp.lo_ptr[r0]:sp.lo_ptr[r1]],
+                      sp.lo_val[sp.lo_ptr[r0]:sp.lo_ptr[r1]])]
+            cnt = a + 1
+            ptr = np.concatenate(([0], np.cumsum(cnt)))
+            rr = np.repeat(np.arange(r1 - r0), cnt)
+            c = np.arange(ptr[-1]) - np.repeat(ptr[:-1], cnt)
+            ar = a[rr]
+            parts.append((ptr, (rb[rows][rr] - r0) + c, sp.inv_rm[boff[rows][rr] + ar * (ar + 1) // 2 + c]))
+            parts.append((sp.up_ptr[r0:r1 + 1] - sp.up_ptr[r0], sp.up_col[sp.up_ptr[r0]:sp.up_ptr[r1]],
+                          sp.up_val[sp.up_ptr[r0]:sp.up_ptr[r1]]))
+            cnt = B - a
+            ptr = np.concatenate(([0], np.cumsum(cnt)))
+            rr = np.repeat(np.arange(r1 - r0), cnt)
+            c = np.arange(ptr[-1]) - np.repeat(ptr[:-1], cnt)
+            ar, Br = a[rr], B[rr]
+            parts.append((ptr, (rb[rows][rr] - r0) + ar + c,
+                          sp.inv_cm[boff[rows][rr] + ar * Br - ar * (ar - 1) // 2 + c]))
+            per_stage.append((r0, rows, parts))
+        order = [(s, 0) for s in range(sp.n_stages)]
+        prog = []
+        for s in range(sp.n_stages):
+            prog += [(s, 0), (s, 1)]
+        for s in range(sp.n_stages - 1, -1, -1):
+            prog += [(s, 2), (s, 3)]
+        del order
+        # solve records: values pair-interleaved (16-byte loads), indices lane-major
+        solve = []
+        vals, idxs = [], []
+        nv = ni = 0
+        tid = np.arange(BS)
+        for s, kind in prog:
+            r0, rows, parts = per_stage[s]
+            pptr, pidx, pval = parts[kind]
+            L = sliced(pptr, pidx, pval, BS, rows, target_iter=target_iter)
+            recs = records(L, 0, r0, _native.SLICE_BLOCK if kind % 2 else 0, True)
+            # LDS byte offsets: z at 8*position, stage scratch t at 8*(Nf + row)
+            lds_base = 8 * Nf if kind % 2 else 0
+            for (base, rowbase, rr0, n, flags) in recs:
+                i = np.arange(n)[:, None]
+                src = base + i * BS + tid[None, :]
+                NP, NQ = -(-n // 2), -(-n // 4)
+                v = np.zeros(2 * BS * NP)
+                v[((i // 2) * 2 * BS + 2 * tid[None, :] + i % 2).ravel()] = L["val"][src].ravel()
+                ix = np.full(4 * BS * NQ, lds_base, dtype=np.int64)
+                ix[(tid[None, :] * 4 * NQ + i).ravel()] = lds_base + 8 * L["idx"][src].ravel()
+                solve.append((nv, ni, rowbase, rr0, n, flags))
+                vals.append(v)
+                idxs.append(ix)
+                nv += len(v)
+                ni += len(ix)
+        cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)
+        # four records the kernel loads ahead but never executes
+        self.n_solve_exec = len(solve)
+        solve = solve + [(0, 0, 0, 0, 0, 1)] * 4
+        self.solve_slices = solve
+        self.face_slices = [(b, 0, rb, r0, n, fl) for (b, rb, r0, n, fl) in self.face_slices]
+        self.slice_rows = cat(rows_all, np.int32)
+        self.ent_off = cat(idxs, np.uint32)
+        self.ent_val = cat(vals, np.float64)
+        self._face_rec = self._slice_array(self.face_slices)
+        self._solve_rec = self._slice_array(self.solve_slices)
+        self.n_slices = len(solve)
+
+    @staticmethod
+    def _slice_array(recs):
+        arr = (_native.Slice * max(len(recs), 1))()
+        for i, (b, ib, rb, r0, it, fl) in enumerate(recs):
+            arr[i].base, arr[i].ibase, arr[i].rowbase = b, ib, rb
+            arr[i].r0, arr[i].n_iter, arr[i].flags = r0, it, fl
+        return arr
 
     def desc(self):
         """ctypes ProblemDesc pointing at this object's arrays (keep self alive)."""
@@ -266,28 +469,75 @@ class CircuitPlan:
         P = _native.ptr
         d = _native.ProblemDesc()
         d.n_junctions, d.n_faces = self.Nj, self.Nf
-        d.stencil_width, d.n_stages = self.stencil_width, sp.n_stages
-        d.face_ptr = P(self.face_ptr, C.c_int32)
-        d.face_code = P(self.face_code, C.c_int32)
-        d.junc_code = P(self.junc_code, C.c_int32)
+        d.stencil_width, d.block_size = self.stencil_width, self.block_size
         d.Ic = P(self.Ic, C.c_double)
         d.Rv = P(self.Rv, C.c_double)
         d.coef = P(self.coef, C.c_double)
-        d.face_pos = P(sp.face_pos, C.c_int32)
-        d.stage_ptr = P(sp.stage_ptr, C.c_int32)
-        d.lo_ptr = P(sp.lo_ptr, C.c_int64)
-        d.lo_col = P(sp.lo_col, C.c_int32)
-        d.lo_val = P(sp.lo_val, C.c_double)
-        d.up_ptr = P(sp.up_ptr, C.c_int64)
-        d.up_col = P(sp.up_col, C.c_int32)
-        d.up_val = P(sp.up_val, C.c_double)
-        d.row_block = P(sp.row_block, C.c_int32)
-        d.row_bsize = P(sp.row_bsize, C.c_int32)
-        d.row_boff = P(sp.row_boff, C.c_int64)
-        d.n_inv = len(sp.inv_cm)
-        d.inv_cm = P(sp.inv_cm, C.c_double)
-        d.inv_rm = P(sp.inv_rm, C.c_double)
+        d.junc_code = P(self.junc_pos_code, C.c_int32)
+        d.face_of_pos = P(self.face_of_pos, C.c_int32)
+        d.n_face_slices = len(self.face_slices)
+        d.n_solve_slices = self.n_solve_exec
+        d.face_slices = C.cast(self._face_rec, C.c_void_p)
+        d.solve_slices = C.cast(self._solve_rec, C.c_void_p)
+        d.n_face_entries = len(self.face_code_sliced)
+        d.face_code = P(self.face_code_sliced, C.c_int32)
+        d.n_slice_rows = len(self.slice_rows)
+        d.slice_rows = P(self.slice_rows, C.c_int32)
+        d.n_entries = len(self.ent_val)
+        d.ent_val = P(self.ent_val, C.c_double)
+        d.n_index = len(self.ent_off)
+        d.ent_off = P(self.ent_off, C.c_uint32)
+        d.max_stage_rows = sp.max_stage_rows
         return d
 
     def lds_bytes(self):
         return 8 * (self.Nf + self.solve_plan.max_stage_rows)
+
+    # ---------------------------------------------------- host emulation
+    def factor_bytes(self):
+        """Bytes of solve-program data one column loads per step (live row slots)."""
+        n = 0
+        for (_, _, _, _, it, fl) in self.solve_slices[:self.n_solve_exec]:
+            n += (fl >> 16) * (fl & 0xFF) * (16 * (-(-it // 2)) + 16 * (-(-it // 4)) + 4)
+        return int(n + 32 * self.n_solve_exec)
+
+    def emulate_face_and_solve(self, u):
+        """Exactly the kernel's face phase and slice program, on the host:
+        u (Nj,) = x/c0 - theta_s  ->  J (Nf,) = K^-1 (A u)  (f = 0)."""
+        BS, Nf = self.block_size, self.Nf
+        sp = self.solve_plan
+        zs = np.zeros(Nf)
+        ts = np.zeros(max(sp.max_stage_rows, 1))
+        tids = np.arange(BS)
+        acc = np.zeros(BS)
+        for (base, _, rowbase, r0, n_it, flags) in self.face_slices:
+            for it in range(n_it):
+                code = self.face_code_sliced[base + it * BS + tids]
+                v = np.where(code >= 0, u[np.maximum(code, 0) >> 1], 0.0)
+                acc = acc + np.where(code & 1, -v, v)
+            if flags & _native.SLICE_FINISH:
+                pos = self.slice_rows[rowbase + tids]
+                ok = pos >= 0
+                zs[pos[ok]] = acc[ok]
+                acc = np.zeros(BS)
+        acc = np.zeros(BS)
+        for (vbase, ibase, rowbase, r0, n_it, flags) in self.solve_slices[:self.n_solve_exec]:
+            G = flags & 0xFF
+            block = bool(flags & _native.SLICE_BLOCK)
+            NQ = -(-n_it // 4)
+            live = (tids // G) < (flags >> 16)
+            lds = np.concatenate((zs, ts))
+            for it in range(n_it):
+                v = self.ent_val[vbase + (it // 2) * 2 * BS + 2 * tids + it % 2]
+                off = self.ent_off[ibase + tids * 4 * NQ + it].astype(np.int64)
+                acc = acc + np.where(live, v * lds[off // 8], 0.0)
+            if flags & _native.SLICE_FINISH:
+                red = acc.reshape(BS // G, G).sum(axis=1)
+                rows = self.slice_rows[rowbase + np.arange(BS // G)]
+                ok = rows >= 0
+                if block:
+                    zs[rows[ok]] = red[ok]
+                else:
+                    ts[rows[ok] - r0] = zs[rows[ok]] - red[ok]
+                acc = np.zeros(BS)
+        return zs[sp.face_pos]

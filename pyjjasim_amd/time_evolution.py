@@ -26,13 +26,10 @@ There is no CPU fallback: without a HIP device ``compute()`` raises.
 """
 from __future__ import annotations
 
-import ctypes as C
-
 import numpy as np
 
 from .circuit import Circuit, DefaultCPR
-from . import _native
-from .solver_plan import CircuitPlan, STENCILS
+from .solver_plan import STENCILS
 
 __all__ = ["TimeEvolutionProblem", "TimeEvolutionResult", "ThetaNotStored", "CurrentNotStored",
            "VoltageNotStored", "DataAtTimepointNotStored"]
@@ -300,192 +297,17 @@ def time_evolution(problem: TimeEvolutionProblem):
                                vortex=n_out if problem.store_theta else None)
 
 
-# ---------------------------------------------------------------- GPU core
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError("pyjjasim_amd: TimeEvolutionProblem.compute() needs a HIP (ROCm) "
-                           "device; none is visible")
-    return torch
-
-
-class _DeviceSource:
-    """Device copy of one input plus its jja_source descriptor."""
-
-    def __init__(self, torch, value, n, W, Nt, zero, chunk=False):
-        self.callable = callable(value)
-        self.value = value
-        self.n, self.W = n, W
-        self.src = _native.Source()
-        self.src.zero = int(bool(zero))
-        self.tensor = None
-        if not self.callable:
-            arr = np.ascontiguousarray(np.array(value, dtype=np.float64))
-            view = np.broadcast_to(arr, (n, W, Nt))
-            strides = [s // 8 for s in view.strides]
-            self.tensor = torch.from_numpy(arr.reshape(-1) if arr.size else np.zeros(1)).cuda()
-            self.src.base = self.tensor.data_ptr()
-            self.src.s_i, self.src.s_w, self.src.s_t = strides
-            self.src.t0 = 0
-
-    def fill_chunk(self, torch, t0, n_steps):
-        """Evaluate a callable input for steps t0..t0+n_steps-1 into a device buffer
-        laid out [t][w][i]."""
-        if not self.callable:
-            return
-        buf = np.empty((n_steps, self.W, self.n))
-        for k in range(n_steps):
-            buf[k] = np.broadcast_to(self.value(t0 + k), (self.n, self.W)).T
-        self.tensor = torch.from_numpy(buf.reshape(-1) if buf.size else np.zeros(1)).cuda()
-        self.src.base = self.tensor.data_ptr()
-        self.src.s_i, self.src.s_w, self.src.s_t = 1, self.n, self.n * self.W
-        self.src.t0 = t0
-
-
-def _zero_flag(problem, name, sample, timedep):
-    return (not timedep) and bool(np.allclose(sample, 0))
-
-
-def _reference_noise(Nj, W):
-    """Replays the reference's legacy-RNG draw order (time_evolution.py:533-537)."""
-    state = {}
-
-    def draw(i):
-        if Nj > 500:
-            if i % 3 == 0:
-                state["r"] = np.random.randn(Nj, W)
-            else:
-                state["r"] = state["r"][np.random.permutation(Nj), :]
-        else:
-            state["r"] = np.random.randn(Nj, W)
-        return state["r"]
-    return draw
-
-
-def _default_block_size(Nj, Nf):
-    work = max(Nj, Nf)
-    for bs in (64, 128, 256, 512):
-        if work <= 2 * bs:
-            return bs
-    return 1024
-
-
 def _gpu_core(problem: TimeEvolutionProblem, th_mask, I_mask):
-    """Replacement of time_evolution_core: returns (th_out, I_out, n_out) with the
-    reference layout (Nj, W, prefix + stored) and vortex counts of stored theta."""
-    if not isinstance(problem.current_phase_relation, DefaultCPR):
-        raise ValueError("the GPU time evolution evaluates the default current-phase relation "
-                         "Ic*sin(theta) only; got a custom CurrentPhaseRelation")
-    torch = _torch()
-    circuit = problem.circuit
-    Nj, Nf, W, Nt = circuit._Nj(), circuit._Nf(), problem.problem_count, problem._Nt()
-    dt = problem._dt()
-    sw = problem.stencil_width
-    S = sw - 1
-    plan_host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size)
-    lib = _native.lib()
-    desc = plan_host.desc()
-    handle = C.c_void_p()
-    _native.check(lib.jja_plan_create(1, C.byref(desc), C.byref(handle)))
+    """Replacement of time_evolution_core (time_evolution.py:461-582): returns
+    (th_out, I_out, n_out) in the reference layout (Nj, W, prefix + stored) plus the
+    vortex counts of the stored theta."""
+    from .engine import Engine
+    eng = Engine(problem, th_mask, I_mask)
     try:
-        if lib.jja_plan_lds_bytes(handle, 0) == 0:
-            raise _native.NativeError(
-                f"circuit with {Nf} faces needs {plan_host.lds_bytes()} bytes of LDS per column; "
-                "larger than one workgroup's LDS")
-        dev = torch.device("cuda")
-        f64 = torch.float64
-        # history slots: theta at step m in slot m mod S; initial configs at m = -1..-S
-        cfgs = [problem.config_at_minus_1, problem.config_at_minus_2]
-        if sw >= 4:
-            cfgs.append(problem.config_at_minus_3)
-        if sw >= 5:
-            cfgs.append(problem.config_at_minus_4)
-        hist = np.zeros((S, W, Nj))
-        for k, cfg in enumerate(cfgs):
-            hist[(-(k + 1)) % S] = np.asarray(cfg, dtype=np.float64).reshape(Nj, W).T
-        theta = torch.from_numpy(hist).to(dev)
-        # inputs and the reference's zero flags (time_evolution.py:509-519)
-        srcs = {}
-        for name, val, n, td in (("Is", problem.current_sources, Nj, problem._Is_is_timedep),
-                                 ("Vs", problem.voltage_sources, Nj, problem._Vs_is_timedep),
-                                 ("f", problem.external_flux, Nf, problem._f_is_timedep),
-                                 ("T", problem.temperature, Nj, problem._T_is_timedep)):
-            sample = np.broadcast_to(val(0), (n, W)) if callable(val) else val[:, :, 0]
-            zero = _zero_flag(problem, name, sample, td)
-            srcs[name] = _DeviceSource(torch, problem._raw[name], n, W, Nt, zero)
-        theta_s = torch.zeros((W, Nj), dtype=f64, device=dev) if not srcs["Vs"].src.zero else None
-        # storage slots
-        th_slot = np.where(th_mask, np.cumsum(th_mask) - 1, -1).astype(np.int32)
-        I_slot = np.where(I_mask, np.cumsum(I_mask) - 1, -1).astype(np.int32)
-        n_th, n_I = int(th_mask.sum()), int(I_mask.sum())
-        th_slot_d = torch.from_numpy(th_slot).to(dev)
-        I_slot_d = torch.from_numpy(I_slot).to(dev)
-        th_out = torch.empty((max(n_th, 1), W, Nj), dtype=f64, device=dev)
-        I_out = torch.empty((max(n_I, 1), W, Nj), dtype=f64, device=dev)
-
-        run = _native.ProblemRun()
-        run.W = W
-        run.column_offset = 0
-        run.dt = dt
-        run.theta = theta.data_ptr()
-        run.theta_s = theta_s.data_ptr() if theta_s is not None else None
-        run.th_slot = th_slot_d.data_ptr()
-        run.I_slot = I_slot_d.data_ptr()
-        run.slot_t0 = 0
-        run.th_out = th_out.data_ptr()
-        run.I_out = I_out.data_ptr()
-        T_zero = srcs["T"].src.zero
-        ref_noise = (not T_zero) and problem.noise == "reference"
-        if not T_zero and problem.noise == "philox":
-            run.seed = int(problem.seed) if problem.seed is not None \
-                else int(np.random.randint(0, 2 ** 62, dtype=np.int64))
-        draw = _reference_noise(Nj, W) if ref_noise else None
-        any_callable = any(s.callable for s in srcs.values())
-        spl = problem.steps_per_launch
-        if spl is None:
-            spl = 2000 if not (any_callable or ref_noise) else 500
-        bs = problem.block_size or _default_block_size(Nj, Nf)
-        stream = torch.cuda.current_stream().cuda_stream
-        t0 = 0
-        noise_buf = None
-        while t0 < Nt:
-            n_steps = min(spl, Nt - t0)
-            for s in srcs.values():
-                s.fill_chunk(torch, t0, n_steps)
-            if ref_noise:
-                z = np.empty((n_steps, W, Nj))
-                for k in range(n_steps):
-                    z[k] = draw(t0 + k).T
-                noise_buf = torch.from_numpy(z.reshape(-1)).to(dev)
-                run.noise = noise_buf.data_ptr()
-                run.noise_t0 = t0
-            run.Is, run.Vs, run.f, run.T = (srcs[k].src for k in ("Is", "Vs", "f", "T"))
-            _native.check(lib.jja_evolve(handle, C.byref(run), n_steps, t0, bs, stream))
-            if any_callable or ref_noise:
-                torch.cuda.current_stream().synchronize()   # chunk buffers are reused
-            t0 += n_steps
-        n_dev = None
-        if n_th and Nf:
-            n_dev = torch.empty((n_th, W, Nf), dtype=torch.int32, device=dev)
-            _native.check(lib.jja_vortex(handle, 0, th_out.data_ptr(), n_th * W,
-                                         n_dev.data_ptr(), stream))
-        torch.cuda.synchronize()
-        th_host = th_out[:n_th].cpu().numpy().transpose(2, 1, 0) if n_th else np.zeros((Nj, W, 0))
-        I_host = I_out[:n_I].cpu().numpy().transpose(2, 1, 0) if n_I else np.zeros((Nj, W, 0))
-        n_host = None
-        if n_dev is not None:
-            n_host = n_dev.cpu().numpy().astype(np.int64).transpose(2, 1, 0)
-        elif n_th:
-            n_host = np.zeros((0, W, n_th), dtype=np.int64)
+        eng.advance(problem._Nt())
+        return eng.results()
     finally:
-        lib.jja_plan_destroy(handle)
-    # prefix of initial conditions (time_evolution.py:484-502)
-    pre = np.stack([np.asarray(cfgs[S - 1 - k]).reshape(Nj, W) for k in range(S)], axis=2)
-    Ic = circuit._Ic()[:, None]
-    pre_I = np.stack([problem.current_phase_relation.eval(Ic, pre[:, :, k]) for k in range(S)], axis=2)
-    th_full = np.concatenate((pre, th_host), axis=2)
-    I_full = np.concatenate((pre_I, I_host), axis=2)
-    return th_full, I_full, n_host
+        eng.close()
 
 
 class TimeEvolutionResult:

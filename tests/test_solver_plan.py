@@ -83,3 +83,27 @@ def test_cycle_codes():
             if code >= 0:
                 dense2[code >> 1, j] = -1 if code & 1 else 1
     assert np.array_equal(dense2, A.toarray())
+
+
+@pytest.mark.parametrize("BS", [64, 256, 1024])
+@pytest.mark.parametrize("name", ["sq10", "hc7", "tri8", "sqL", "het", "squid"])
+def test_sliced_layout_emulation(name, BS):
+    """The exact face phase and staged solve the kernel runs, executed on the host
+    from the sliced layouts, equals a direct solve of K J = A u."""
+    c = _circuits()[name]
+    plan = CircuitPlan(c, 0.05, leaf_size=8, block_size=BS)
+    u = np.random.default_rng(2).standard_normal(c.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
+    got = plan.emulate_face_and_solve(u)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * abs(want).max())
+
+
+def test_sliced_layout_holed_array():
+    full = pa.SquareArray(9, 9)
+    x, y = full.get_node_coordinates()
+    c = full.remove_nodes((x > 2) & (x < 6) & (y > 2) & (y < 6))
+    plan = CircuitPlan(c, 0.05, block_size=64)
+    u = np.random.default_rng(3).standard_normal(c.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
+    np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12,
+                               atol=1e-12 * abs(want).max())
