@@ -16,7 +16,8 @@ after ``--warmup`` untimed steps. Reported:
   cpu_baseline  the FP64 numpy/scipy restatement of the reference scheme (same
              library calls), timed on this host on a bounded sample of steps
 
-Usage: python bench.py [--gpus 1] [--steps 20000] [--warmup 1000]
+Usage: python bench.py [--gpus 1] [--steps 20000] [--warmup 1000] [--plain-steps 0]
+       [--no-cpu] [--copy-calibration MIB]
 """
 import argparse
 import json
@@ -104,6 +105,39 @@ def cpu_baseline(pa, n, W, budget_s):
                        f"{t:.1f} s total")
 
 
+def measured_traffic(steps_per_launch):
+    """HBM bytes per evolve_kernel launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor), or
+    None when the summary was taken at another launch length."""
+    path = os.path.join(ROOT, "profiles", "evolve_kernel_square100_W256_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if ":" in line and not line.startswith(("#", " ", "\t")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        if int(vals["steps_per_launch"]) != steps_per_launch:
+            return None
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def copy_calibration(torch, mib):
+    """One plain device copy of `mib` MiB: the reference point for the HBM
+    counters (FETCH_SIZE and WRITE_SIZE of this kernel should read `mib` MiB)."""
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +150,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +162,9 @@ def main():
     import pyjjasim_amd as pa
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration(torch, args.copy_calibration)
+        return
 
     wall, launch_ms, th, nv, c, info = run_gpu(pa, torch, args.n, args.W, args.steps, args.warmup,
                                                args.steps_per_launch, args.block_size)
@@ -137,21 +176,20 @@ def main():
     mean_launch_ms = tot_ms / len(launch_ms)
     bytes_per_launch = info["bytes_per_step"] * tot_steps / len(launch_ms)
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "evolve_kernel_square100_W256_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            tj = json.load(fh)
-        if tj.get("steps_per_launch") == info["steps_per_launch"]:
-            traffic = tj.get("hbm_bytes_per_launch")
+    traffic = measured_traffic(info["steps_per_launch"]) if (args.n, W) == (100, 256) else None
     # integrity check at full size: device vortex counts equal the host count
     A = c.get_cycle_matrix()
     n_host = (-A @ np.round(th[:, :, -1] / (2.0 * np.pi))).astype(int)
     assert np.array_equal(n_host, nv[:, :, -1]), "device vortex counts disagree with theta"
     # plain: one problem at a time (W = 1) on the same circuit
-    pwall, pl, _, _, _, _ = run_gpu(pa, torch, args.n, 1, args.plain_steps, 100,
-                                    args.steps_per_launch, args.block_size, events=False)
-    plain_rate = Nj * 1 * args.plain_steps / pwall
+    plain = None
+    if args.plain_steps > 0:
+        pwall, pl, _, _, _, _ = run_gpu(pa, torch, args.n, 1, args.plain_steps, 100,
+                                        args.steps_per_launch, args.block_size, events=False)
+        plain_rate = Nj * 1 * args.plain_steps / pwall
+        plain = {"value": plain_rate, "unit": "junction-steps/s",
+                 "note": "one problem (W=1) per run, same circuit and kernel",
+                 "batched_over_plain": value / plain_rate}
     out = {
         "metric": "junction-steps/s (batched IV sweep, FP64)",
         "value": value,
@@ -174,9 +212,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "evolve_kernel", "bytes_per_step": info["bytes_per_step"],
                      "mean_launch_ms": mean_launch_ms},
-        "plain": {"value": plain_rate, "unit": "junction-steps/s",
-                  "note": "one problem (W=1) per run, same circuit and kernel",
-                  "batched_over_plain": value / plain_rate},
+        "plain": plain,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pa, args.n, W, args.cpu_seconds)

@@ -235,10 +235,15 @@ class Engine:
 
     def bytes_per_step(self):
         """Algorithmic HBM bytes of one time step of the whole batch:
-        W (24 Nj + 48 Nf) + factor + 8 Nj k_par + sources + outputs, with the
-        factor counted once per step as values plus 2-byte indices."""
+        W (24 Nj + 48 Nf) + 12 nnz + 8 Nj k_par + sources + outputs.
+        24 Nj: theta_{t-1}, theta_{t-2} read and theta_t written; 48 Nf: face
+        right-hand side written, read and written by each triangular pass and
+        read by the A^T gather; nnz: factor entries both passes stream (off-block
+        rows of L and L^T plus the dense diagonal-block inverses), counted once per
+        step as an 8-byte value and a 4-byte index; k_par = 5 coefficients per
+        junction (Ic, c0, c1, c2, noise scale)."""
         sp = self.host.solve_plan
-        factor = 10 * (len(sp.lo_val) + len(sp.up_val)) + 10 * (len(sp.inv_cm) + len(sp.inv_rm))
+        factor = 12 * (len(sp.lo_val) + len(sp.up_val) + len(sp.inv_cm) + len(sp.inv_rm))
         src = sum(s.bytes_per_step for k, s in self.srcs.items() if not s.src.zero)
         out = 8 * self.Nj * self.W * (self.n_th + self.n_I) / max(self.Nt, 1)
         return (self.W * (24 * self.Nj + 48 * self.Nf) + factor + 8 * self.Nj * 5 + src + out)
