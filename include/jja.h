@@ -67,16 +67,19 @@ const char* jja_factor_last_error(void);
  *   - G = flags & 0xff consecutive threads share one row (G a power of two
  *     <= 64); thread tid works on row slice_rows[rowbase + tid/G] (-1: none) and
  *     takes that row's entries tid%G, tid%G + G, ...;
- *   - every address is known without reading an index first. Face slices:
- *     code `it` (it < n_iter <= 8) of thread tid at base + it*block_size + tid.
- *     Solve slices hold n_iter <= 8 entries per thread: values in pairs, value
- *     `it` of thread tid at base + (it/2)*2*block_size + 2*tid + it%2 (16-byte
- *     loads, ceil(n_iter/2) pairs); LDS byte offsets lane-major in groups of
- *     four, offset `it` at ibase + tid*NQ*4 + it with NQ = ceil(n_iter/4).
- *     An offset addresses the workgroup's LDS: z at 8*position, the stage
- *     scratch t at 8*(n_faces + stage row). Face codes pad with -1.
- *     The solve program ends with 4 records that are never executed (flags 0):
- *     the kernel loads up to four records ahead.
+ *   - every address is known without reading an index first.
+ * Face slices (workgroup-wide): code `it` (it < n_iter <= 8) of thread tid at
+ *     base + it*block_size + tid; the row of tid at slice_rows[rowbase + tid].
+ *     Face codes pad with -1.
+ * Solve records (one wavefront of 64 lanes each): every record holds 8 entries
+ *     per lane, zero-padded. Value `it` of lane l at base + (it/2)*128 + 2*l +
+ *     it%2 (16-byte loads); LDS index `it` of lane l at ibase + 8*l + it (one
+ *     16-byte load per lane); the row of lane l at slice_rows[rowbase + l/G].
+ *     An index addresses the workgroup's LDS in doubles: z at its position,
+ *     the stage scratch t at n_faces + stage row. Wave w of the workgroup runs records solve_wave_ptr[w] ..
+ *     solve_wave_ptr[w+1]-5: an even count (a record with G 1 and no rows is a
+ *     no-op), followed by 4 records that it loads ahead but never executes.
+ *     Every wave's program has the same number of BARRIER records.
  * A row may span several consecutive slices; only the last one (flag FINISH)
  * reduces over the G lanes and writes the row. Row slots are filled from the
  * front: flags >> 16 is the number of slots holding a row, and threads of the
@@ -118,9 +121,9 @@ typedef struct jja_problem_desc {
   const int32_t* junc_code; /* [2*Nj] */
   const int32_t* face_of_pos; /* [Nf] */
   int32_t n_face_slices;
-  int32_t n_solve_slices;
+  int32_t n_solve_slices;        /* all records of all waves, lookahead records included */
   const jja_slice* face_slices;  /* [n_face_slices] */
-  const jja_slice* solve_slices; /* [n_solve_slices], execution order */
+  const jja_slice* solve_slices; /* [n_solve_slices], per-wave programs in execution order */
   int64_t n_face_entries;
   const int32_t* face_code;      /* [n_face_entries] */
   int64_t n_slice_rows;
@@ -128,11 +131,20 @@ typedef struct jja_problem_desc {
   int64_t n_entries;
   const double* ent_val;         /* [n_entries] */
   int64_t n_index;
-  const uint32_t* ent_off;       /* [n_index] LDS byte offsets */
+  const uint16_t* ent_off;       /* [n_index] LDS indices (doubles) */
   int32_t max_stage_rows;        /* size of the stage scratch t */
-  int32_t pad_;
+  /* the first stage's rows have no off-block entries, so its scratch equals b:
+   * the face phase writes b[p] also to t[p] for positions p < face_copy_rows,
+   * and the program has no records for that phase */
+  int32_t face_copy_rows;
+  const int32_t* solve_wave_ptr; /* [block_size/64 + 1] first record of each wave's program */
 } jja_problem_desc;
 
+/*
+ * Check every size, index and offset of n_problems descriptors without touching
+ * a device (jja_plan_create runs the same checks before it uploads anything).
+ */
+int jja_plan_validate(int32_t n_problems, const jja_problem_desc* problems);
 /* Create a plan for n_problems circuits on the current HIP device. */
 int jja_plan_create(int32_t n_problems, const jja_problem_desc* problems, jja_plan** out);
 int jja_plan_destroy(jja_plan* plan);

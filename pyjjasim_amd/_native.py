@@ -45,8 +45,9 @@ class ProblemDesc(C.Structure):
         ("n_face_entries", C.c_int64), ("face_code", i32p),
         ("n_slice_rows", C.c_int64), ("slice_rows", i32p),
         ("n_entries", C.c_int64), ("ent_val", f64p), ("n_index", C.c_int64),
-        ("ent_off", C.POINTER(C.c_uint32)),
-        ("max_stage_rows", C.c_int32), ("pad_", C.c_int32),
+        ("ent_off", C.POINTER(C.c_uint16)),
+        ("max_stage_rows", C.c_int32), ("face_copy_rows", C.c_int32),
+        ("solve_wave_ptr", i32p),
     ]
 
 
@@ -76,7 +77,7 @@ class NativeError(RuntimeError):
 _lib = None
 
 EXPORTS = ["jja_cholesky", "jja_factor_get", "jja_factor_free", "jja_factor_last_error",
-           "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes", "jja_evolve",
+           "jja_plan_validate", "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes", "jja_evolve",
            "jja_vortex", "jja_last_error"]
 
 
@@ -93,6 +94,7 @@ def lib():
     L.jja_factor_free.argtypes = [C.c_void_p]
     L.jja_factor_free.restype = None
     L.jja_factor_last_error.restype = C.c_char_p
+    L.jja_plan_validate.argtypes = [C.c_int32, C.POINTER(ProblemDesc)]
     L.jja_plan_create.argtypes = [C.c_int32, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
     L.jja_plan_destroy.argtypes = [C.c_void_p]
     L.jja_plan_lds_bytes.argtypes = [C.c_void_p, C.c_int32]

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/jja.h"
+#include "jja_math.h"
 
 namespace {
 
@@ -50,13 +51,15 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr double kTwoPi = 6.283185307179586;  // 2 * np.pi
-constexpr int kMaxIter = 8;                   // entries per thread and slice
+constexpr int kMaxIter = 8;                   // entries per lane and record
+constexpr int kWave = 64;                     // lanes of a wavefront (solve records are per wave)
 
 // Pointers read out of the descriptor structs are generic; casting them to the
 // global (1) or constant (4) address space lets the compiler use global and
 // scalar loads instead of flat ones, which would also stall on LDS traffic.
 #define GAS __attribute__((address_space(1)))
 #define CAS __attribute__((address_space(4)))
+#define LAS __attribute__((address_space(3)))
 template <typename T>
 __device__ __forceinline__ const GAS T* gbl(const T* p) {
   return (const GAS T*)p;
@@ -88,7 +91,7 @@ __device__ __forceinline__ jja_slice load_rec(const jja_slice* prog, int32_t k) 
 // Device-resident description of one circuit (all pointers device memory).
 struct DevProblem {
   int32_t Nj, Nf, sw, BS;
-  int32_t n_face_slices, n_solve_slices, max_stage_rows, pad_;
+  int32_t n_face_slices, n_solve_slices, max_stage_rows, copy_rows;
   const double* Ic;
   const double* Rv;
   const double* coef;
@@ -99,8 +102,9 @@ struct DevProblem {
   const int32_t* face_code;
   const int32_t* slice_rows;
   const double* ent_val;
-  const uint32_t* ent_off;
-  const double* zero;  // 64 zero bytes
+  const uint16_t* ent_off;
+  const int32_t* wave_ptr;  // [BS/64 + 1]
+  const double* zero;       // 8 zero doubles: absent input terms read here
 };
 
 struct WorkItem {
@@ -176,60 +180,56 @@ __device__ __forceinline__ double group_sum_last(double v, int G) {
   return v;
 }
 
-// Entries of one solve record for one thread, held in registers.
+// Entries of one solve record for one lane, held in registers.
 struct SliceRegs {
   double v[kMaxIter];
-  uint32_t off[kMaxIter];  // LDS byte offsets
+  uint32_t idx[kMaxIter / 2];  // LDS indices (doubles), two 16-bit halves each
   int32_t row;
 };
 
-// Issue the loads of one record: always the same 7 vector loads, without
-// branches, so that the compiler can keep two records in flight while it
-// processes a third. Lanes without a row and entries past n_iter read the
-// plan's zero page (value 0, LDS offset 0).
-template <int BS>
-__device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice& m, int tid,
+// Issue the loads of one solve record: always the same 6 vector loads, each a
+// scalar base (from the record) plus a per-lane offset that does not change
+// between records, so no address arithmetic runs per record.
+__device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice& m, int lane,
                                            SliceRegs& r) {
-  const int G = m.flags & 0xff;
-  const bool live = tid / G < (m.flags >> 16);
-  r.row = gbl(P.slice_rows)[m.rowbase + tid / G];
-  const int np = (m.n_iter + 1) >> 1, nq = (m.n_iter + 3) >> 2;
-  const GAS vdouble2* __restrict__ vp = gbl(reinterpret_cast<const vdouble2*>(P.ent_val + m.base)) + tid;
-  const GAS vuint4* __restrict__ op = gbl(reinterpret_cast<const vuint4*>(P.ent_off + m.ibase)) + tid * nq;
-  const GAS vdouble2* __restrict__ zd = gbl(reinterpret_cast<const vdouble2*>(P.zero));
-  const GAS vuint4* __restrict__ zo = gbl(reinterpret_cast<const vuint4*>(P.zero));
+  const int lg = __builtin_ctz(static_cast<unsigned>(m.flags & 0xff));
+  const GAS vdouble2* __restrict__ vp = gbl(reinterpret_cast<const vdouble2*>(P.ent_val + m.base)) + lane;
+  const GAS vuint4* __restrict__ op = gbl(reinterpret_cast<const vuint4*>(P.ent_off + m.ibase)) + lane;
+  const int slot = lane >> lg;
+  const int32_t row = gbl(P.slice_rows)[m.rowbase + slot];
+  r.row = slot < (m.flags >> 16) ? row : -1;
 #pragma unroll
   for (int q = 0; q < kMaxIter / 2; ++q) {
-    const vdouble2 d = *((live && q < np) ? vp + q * BS : zd);
+    const vdouble2 d = vp[q * kWave];
     r.v[2 * q] = d.x;
     r.v[2 * q + 1] = d.y;
   }
-#pragma unroll
-  for (int q = 0; q < kMaxIter / 4; ++q) {
-    const vuint4 o = *((live && q < nq) ? op + q : zo);
-    r.off[4 * q] = o.x;
-    r.off[4 * q + 1] = o.y;
-    r.off[4 * q + 2] = o.z;
-    r.off[4 * q + 3] = o.w;
-  }
+  const vuint4 o = *op;
+  r.idx[0] = o.x;
+  r.idx[1] = o.y;
+  r.idx[2] = o.z;
+  r.idx[3] = o.w;
 }
 
-__device__ __forceinline__ double lds_at(const double* lds, uint32_t byte_off) {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(lds) + byte_off);
+// LDS read of double number `i`. The evolve kernel has no static LDS
+// (checked on the host before every launch), so its dynamic LDS array starts
+// at address 0 and the plan's indices address it directly.
+__device__ __forceinline__ double lds_abs(uint32_t i) {
+  return *reinterpret_cast<const LAS double*>(static_cast<uintptr_t>(i) << 3);
 }
 
 // Gather, accumulate, and (flag FINISH) reduce and write one record's rows.
-__device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t Nf,
+__device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t Nf, int lane,
                                               const jja_slice& m, const SliceRegs& r, double& acc) {
   double zv[kMaxIter];
 #pragma unroll
-  for (int i = 0; i < kMaxIter; ++i) zv[i] = lds_at(lds, r.off[i]);
+  for (int i = 0; i < kMaxIter; ++i) zv[i] = lds_abs((r.idx[i / 2] >> (16 * (i % 2))) & 0xffffu);
 #pragma unroll
   for (int i = 0; i < kMaxIter; ++i) acc = __builtin_fma(r.v[i], zv[i], acc);
   if (m.flags & JJA_SLICE_FINISH) {
     const int G = m.flags & 0xff;
     acc = group_sum_last(acc, G);
-    if (r.row >= 0 && (static_cast<int>(threadIdx.x) & (G - 1)) == G - 1) {
+    if (r.row >= 0 && (lane & (G - 1)) == G - 1) {
       if (m.flags & JJA_SLICE_BLOCK)
         lds[r.row] = acc;
       else
@@ -240,29 +240,37 @@ __device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t 
   if (m.flags & JJA_SLICE_BARRIER) lds_barrier();
 }
 
-// The whole forward + backward solve: a linear walk over the record program,
-// software-pipelined in two register buffers used in turn (the next record is
-// in flight while one is processed; no register copies, so the wait before a
-// record covers only its own loads). The program carries four never-executed
-// records at its end.
-template <int BS>
+// The whole forward + backward solve. Each wave walks its own record program,
+// software-pipelined in two register buffers used in turn: the loads of record
+// k+1 are issued before record k is processed (the scheduling barriers keep
+// the compiler from hoisting record k's LDS address use above them, which
+// would make it wait for record k's loads first), and descriptors are read one
+// pair ahead. A wave's executed record count is even and four never-executed
+// records follow it, so the loop has a single exit and every lookahead load
+// stays inside the program. Waves meet only at BARRIER records (one per phase,
+// the same number in every wave).
 __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ lds) {
-  const int tid = threadIdx.x;
-  const int32_t nk = P.n_solve_slices;
-  const jja_slice* __restrict__ prog = P.solve_slices;
-  jja_slice mA = load_rec(prog, 0), mB;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+  const CAS int32_t* wp = (const CAS int32_t*)(P.wave_ptr) + wave;
+  const int32_t k0 = __builtin_amdgcn_readfirstlane(wp[0]);
+  const int32_t nk = __builtin_amdgcn_readfirstlane(wp[1]) - k0 - 4;
+  const jja_slice* __restrict__ prog = P.solve_slices + k0;
+  jja_slice m0 = load_rec(prog, 0), m1 = load_rec(prog, 1);
   SliceRegs A, B;
-  load_slice<BS>(P, mA, tid, A);
+  load_slice(P, m0, lane, A);
   double acc = 0.0;
-  for (int32_t k = 0;; k += 2) {
-    mB = load_rec(prog, k + 1);
-    load_slice<BS>(P, mB, tid, B);
-    if (k >= nk) break;
-    process_slice(lds, P.Nf, mA, A, acc);
-    mA = load_rec(prog, k + 2);
-    load_slice<BS>(P, mA, tid, A);
-    if (k + 1 >= nk) break;
-    process_slice(lds, P.Nf, mB, B, acc);
+  for (int32_t k = 0; k < nk; k += 2) {
+    const jja_slice m2 = load_rec(prog, k + 2);
+    load_slice(P, m1, lane, B);
+    __builtin_amdgcn_sched_barrier(0);
+    process_slice(lds, P.Nf, lane, m0, A, acc);
+    const jja_slice m3 = load_rec(prog, k + 3);
+    load_slice(P, m2, lane, A);
+    __builtin_amdgcn_sched_barrier(0);
+    process_slice(lds, P.Nf, lane, m1, B, acc);
+    m0 = m2;
+    m1 = m3;
   }
 }
 
@@ -293,107 +301,286 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_proble
       if (pos >= 0) {
         if (!R.f.zero) acc = acc - kTwoPi * src_at(R.f, gbl(P.face_of_pos)[pos], w, t);
         zs[pos] = acc;
+        if (pos < P.copy_rows) zs[P.Nf + pos] = acc;  // first stage: scratch = b
       }
       acc = 0.0;
     }
   }
 }
 
-// x of junction j for step t from the phases of the previous steps
-// (time_evolution.py:533-558), in the reference's operation order.
-__device__ __forceinline__ double junction_x(const DevProblem& P, const jja_problem_run& R, int32_t j,
-                                             int64_t w, int64_t t, double t1, double t2, double t3,
-                                             double t4, double z) {
-  const size_t Nj = static_cast<size_t>(P.Nj);
-  const GAS double* __restrict__ c = gbl(P.coef);
-  const double Ic = gbl(P.Ic)[j];
-  double X;
-  if (P.sw == 3) {
-    X = Ic * sin(2.0 * t1 - t2) + c[Nj + j] * t1 + c[2 * Nj + j] * t2;
-  } else if (P.sw == 4) {
-    X = Ic * sin(3.0 * t1 - 3.0 * t2 + t3) + c[Nj + j] * t1 + c[2 * Nj + j] * t2 +
-        c[3 * Nj + j] * t3;
-  } else {
-    X = Ic * sin(4.0 * t1 - 6.0 * t2 + 4.0 * t3 - t4) + c[Nj + j] * t1 + c[2 * Nj + j] * t2 +
-        c[3 * Nj + j] * t3 + c[4 * Nj + j] * t4;
+// One input term of one column at one step: element i is p[i * si] * amp.
+// amp is exactly 1 for a term without amplitude; a term that is absent reads
+// an exact +0.0 from the plan's zero page (si = 0), which leaves every sum it
+// enters bit-identical to skipping it.
+struct TermView {
+  const GAS double* p;
+  int64_t si;
+  double amp;
+};
+
+__device__ __forceinline__ TermView term_view(const jja_source& s, bool present, const double* zero,
+                                              int64_t w, int64_t t) {
+  TermView v{gbl(zero), 0, 1.0};
+  if (present && s.base) {
+    const int64_t tt = t - s.t0;
+    v.p = gbl(s.base) + w * s.s_w + tt * s.s_t;
+    v.si = s.s_i;
+    if (s.amp) v.amp = gbl(s.amp)[w * s.a_w + tt * s.a_t];
   }
-  double eta = 0.0;
-  if (!R.T.zero) {
-    const double sig = R.sigma.base ? src_at(R.sigma, j, w, t)
-                                    : sqrt(2.0 * src_at(R.T, j, w, t) * gbl(P.Rv)[j]);
-    eta = sig * z;
-  }
-  return R.Is.zero ? eta + X : (eta - src_at(R.Is, j, w, t)) + X;
+  return v;
 }
 
-// Junction pass. With `update`: finish step t (time_evolution.py:560-580):
-// y = A^T J from the face vector, theta_t = (y - x)/c0, outputs, voltage-source
-// phase. With `prep`: write x (history slot) and u (scratch) of step tx.
-// Each thread takes the junction pair (2p, 2p+1): one Philox draw gives both
-// deviates.
+__device__ __forceinline__ double term_at(const TermView& v, int32_t i) {
+  return v.p[static_cast<int64_t>(i) * v.si] * v.amp;
+}
+
+// Everything one junction pass needs that is the same for all its junctions.
+struct JunctionStep {
+  GAS double* th_n;            // slot of step t: holds x_t, receives theta_t
+  const GAS double* th_k[4];   // theta_{tx-1-k}, k = 0..S-1 (k = 0 unused with UPDATE)
+  GAS double* th_x;            // slot of step tx: receives x_tx
+  GAS double* ths;             // voltage-source phase (NULL without Vs)
+  GAS double* u;
+  GAS double* th_out;          // stored theta_t row or NULL
+  GAS double* I_out;           // stored current row or NULL
+  TermView Is_t, Vs_t;         // step t (update)
+  TermView Is_x, sig_x;        // step tx (prep); sig_x is sigma, or T when from_T
+  const GAS double* noise;     // reference normals of step tx, or NULL
+  bool is_zero, t_zero, from_T;
+  double dt;
+  uint64_t seed;
+  uint32_t noise_col;
+  uint64_t tx;
+};
+
+// ---------------------------------------------------------------- general
+// One junction, all options: with UPDATE finish step t (time_evolution.py:
+// 560-580): y = A^T J from the face vector, theta_t = (y - x)/c0, stored
+// outputs, theta_s += Vs dt; with PREP form x of step tx (time_evolution.py:
+// 533-558) and write x to the history slot and u = x/c0 - theta_s to the
+// scratch. All loads are issued before any arithmetic.
+template <int SW, bool UPDATE, bool PREP>
+__device__ __forceinline__ void junction_one(const DevProblem& P, const JunctionStep& J,
+                                             const double* __restrict__ zs, int32_t j, double zj) {
+  const int32_t Nj = P.Nj;
+  const GAS double* __restrict__ coef = gbl(P.coef);
+  const double c0 = coef[j];
+  vint2 code{-1, -1};
+  double xo = 0.0, isn = 0.0, thsv = 0.0, vsv = 0.0;
+  if (UPDATE) {
+    code = gbl(reinterpret_cast<const vint2*>(P.junc_code))[j];
+    xo = J.th_n[j];
+    if (J.I_out) isn = term_at(J.Is_t, j);
+    if (J.ths) vsv = term_at(J.Vs_t, j);
+  }
+  if (J.ths) thsv = J.ths[j];
+  double t[4] = {0.0, 0.0, 0.0, 0.0}, c[SW], Ic = 0.0, isx = 0.0, sg = 0.0, rv = 0.0;
+  if (PREP) {
+#pragma unroll
+    for (int k = UPDATE ? 1 : 0; k < SW - 1; ++k) t[k] = J.th_k[k][j];
+    Ic = gbl(P.Ic)[j];
+#pragma unroll
+    for (int k = 1; k < SW; ++k) c[k] = coef[static_cast<int64_t>(k) * Nj + j];
+    isx = term_at(J.Is_x, j);
+    sg = term_at(J.sig_x, j);
+    if (J.from_T) rv = gbl(P.Rv)[j];
+    if (J.noise) zj = J.noise[j];
+  }
+  if (UPDATE) {
+    double y = 0.0;
+    if (code.x >= 0) y = (code.x & 1) ? y - zs[code.x >> 1] : y + zs[code.x >> 1];
+    if (code.y >= 0) y = (code.y & 1) ? y - zs[code.y >> 1] : y + zs[code.y >> 1];
+    t[0] = (y - xo) / c0;
+    J.th_n[j] = t[0];
+    if (J.th_out) J.th_out[j] = t[0];
+    if (J.I_out) J.I_out[j] = y + isn;
+    if (J.ths) {
+      thsv += vsv * J.dt;
+      J.ths[j] = thsv;
+    }
+  }
+  if (PREP) {
+    double X;
+    if constexpr (SW == 3) {
+      X = Ic * jja_sin(2.0 * t[0] - t[1]) + c[1] * t[0] + c[2] * t[1];
+    } else if constexpr (SW == 4) {
+      X = Ic * jja_sin(3.0 * t[0] - 3.0 * t[1] + t[2]) + c[1] * t[0] + c[2] * t[1] + c[3] * t[2];
+    } else {
+      X = Ic * jja_sin(4.0 * t[0] - 6.0 * t[1] + 4.0 * t[2] - t[3]) + c[1] * t[0] + c[2] * t[1] +
+          c[3] * t[2] + c[4] * t[3];
+    }
+    double eta = 0.0;
+    if (!J.t_zero) eta = (J.from_T ? sqrt(2.0 * sg * rv) : sg) * zj;
+    const double x = J.is_zero ? eta + X : (eta - isx) + X;
+    J.th_x[j] = x;
+    double uj = x / c0;
+    if (J.ths) uj = uj - thsv;
+    J.u[j] = uj;
+  }
+}
+
+// Deviate of this lane's junction from a Philox pair computed by another lane
+// of the wave: lane `src` holds (z0, z1) for junctions (2i, 2i+1); `second`
+// picks z1.
+__device__ __forceinline__ double take_normal(double z0, double z1, int src, bool second) {
+  const long long b0 = __builtin_bit_cast(long long, z0), b1 = __builtin_bit_cast(long long, z1);
+  const int a = src << 2;
+  const int lo0 = __builtin_amdgcn_ds_bpermute(a, static_cast<int>(b0));
+  const int hi0 = __builtin_amdgcn_ds_bpermute(a, static_cast<int>(b0 >> 32));
+  const int lo1 = __builtin_amdgcn_ds_bpermute(a, static_cast<int>(b1));
+  const int hi1 = __builtin_amdgcn_ds_bpermute(a, static_cast<int>(b1 >> 32));
+  const int lo = second ? lo1 : lo0, hi = second ? hi1 : hi0;
+  return __builtin_bit_cast(double, static_cast<long long>(static_cast<unsigned>(lo)) |
+                                        (static_cast<long long>(hi) << 32));
+}
+
+// Normals for the junctions k*BS + tid and (k+1)*BS + tid of this lane. Each
+// Philox pair (junctions 2i, 2i+1) is drawn once: lanes 0..31 of a wave draw
+// the 32 pairs its 64 junctions of iteration k form, lanes 32..63 those of
+// iteration k+1, and every lane then fetches its own deviate. All lanes of the
+// wave take part, whether or not their junctions exist.
 template <int BS>
+__device__ __forceinline__ void wave_normals(const JunctionStep& J, int32_t k, double& zA, double& zB) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wave_base = static_cast<int32_t>(threadIdx.x) - lane;
+  const int32_t it = lane < 32 ? k : k + 1;
+  const int32_t j_even = it * BS + wave_base + 2 * (lane & 31);
+  double z0, z1;
+  normal_pair(J.seed, static_cast<uint32_t>(j_even), J.noise_col, J.tx, &z0, &z1);
+  zA = take_normal(z0, z1, lane >> 1, lane & 1);
+  zB = take_normal(z0, z1, 32 + (lane >> 1), lane & 1);
+}
+
+// The junction pass, general form: thread tid takes junctions k*BS + tid
+// (consecutive lanes, consecutive junctions), two iterations at a time.
+template <int BS, int SW, bool UPDATE, bool PREP>
+__device__ __forceinline__ void junction_loop(const DevProblem& P, const JunctionStep& J,
+                                              const double* __restrict__ zs) {
+  const int32_t Nj = P.Nj;
+  const int32_t tid = threadIdx.x;
+  const bool gauss = PREP && !J.t_zero && !J.noise;
+  for (int32_t k = 0; k * BS < Nj; k += 2) {
+    double zA = 0.0, zB = 0.0;
+    if (gauss) wave_normals<BS>(J, k, zA, zB);
+    const int32_t jA = k * BS + tid, jB = jA + BS;
+    if (jA < Nj) junction_one<SW, UPDATE, PREP>(P, J, zs, jA, zA);
+    // keep the two junctions apart: interleaving them would double the live
+    // registers and spill
+    __builtin_amdgcn_sched_barrier(0);
+    if (jB < Nj) junction_one<SW, UPDATE, PREP>(P, J, zs, jB, zB);
+  }
+}
+
+// ------------------------------------------------------------ common step
+// The usual step (stencil 3, no voltage sources, nothing stored at step t,
+// noise amplitude given per junction, Philox or no noise): the same
+// arithmetic as junction_one over the same junction order, with the loads of
+// both junctions of an iteration issued first and the Philox draws computed
+// while they are in flight.
+struct JFast {
+  vint2 code;
+  double c0, xo, t2, c1, c2, Ic, isx, sg;
+};
+
+__device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionStep& J, int32_t j,
+                                           JFast& L) {
+  const int32_t Nj = P.Nj;
+  const GAS double* __restrict__ coef = gbl(P.coef);
+  L.code = gbl(reinterpret_cast<const vint2*>(P.junc_code))[j];
+  L.c0 = coef[j];
+  L.xo = J.th_n[j];
+  L.t2 = J.th_k[1][j];
+  L.c1 = coef[static_cast<int64_t>(Nj) + j];
+  L.c2 = coef[2 * static_cast<int64_t>(Nj) + j];
+  L.Ic = gbl(P.Ic)[j];
+  L.isx = term_at(J.Is_x, j);
+  L.sg = term_at(J.sig_x, j);
+}
+
+__device__ __forceinline__ void jfast_compute(const JunctionStep& J, const double* __restrict__ zs,
+                                              int32_t j, const JFast& L, double z) {
+  double y = 0.0;
+  if (L.code.x >= 0) y = (L.code.x & 1) ? y - zs[L.code.x >> 1] : y + zs[L.code.x >> 1];
+  if (L.code.y >= 0) y = (L.code.y & 1) ? y - zs[L.code.y >> 1] : y + zs[L.code.y >> 1];
+  const double t1 = (y - L.xo) / L.c0;
+  J.th_n[j] = t1;
+  const double X = L.Ic * jja_sin(2.0 * t1 - L.t2) + L.c1 * t1 + L.c2 * L.t2;
+  double eta = 0.0;
+  if (!J.t_zero) eta = L.sg * z;
+  const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
+  J.th_x[j] = x;
+  J.u[j] = x / L.c0;
+}
+
+template <int BS>
+__device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J,
+                                                   const double* __restrict__ zs) {
+  const int32_t Nj = P.Nj;
+  const int32_t tid = threadIdx.x;
+  const int32_t last = Nj - 1;
+  const bool gauss = !J.t_zero;
+  for (int32_t k = 0; k * BS < Nj; k += 2) {
+    const int32_t jA = k * BS + tid, jB = jA + BS;
+    // loads of junctions past the end read the last junction and are not used
+    JFast A, B;
+    jfast_load(P, J, min(jA, last), A);
+    jfast_load(P, J, min(jB, last), B);
+    __builtin_amdgcn_sched_barrier(0);
+    // the Philox draws run while the loads are in flight
+    double zA = 0.0, zB = 0.0;
+    if (gauss) wave_normals<BS>(J, k, zA, zB);
+    if (jA < Nj) jfast_compute(J, zs, jA, A, zA);
+    if (jB < Nj) jfast_compute(J, zs, jB, B, zB);
+  }
+}
+
+// Set up and run one junction pass (finish step t if `update`, prepare step tx
+// if `prep`) for column w.
+template <int BS, int SW>
 __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_problem_run& R,
                                               GAS double* __restrict__ th, GAS double* __restrict__ ths,
                                               GAS double* __restrict__ u, const double* __restrict__ zs,
                                               int64_t w, size_t slot_stride, int64_t t, bool update,
                                               int64_t tx, bool prep, uint32_t noise_col) {
-  const int32_t Nj = P.Nj;
-  const int32_t S = P.sw - 1;
+  constexpr int32_t S = SW - 1;
   auto slot = [&](int64_t m) -> size_t {
     int64_t r = m % S;
     return static_cast<size_t>(r < 0 ? r + S : r) * slot_stride;
   };
-  const size_t sn = slot(t);
-  const size_t sx = slot(tx), sx1 = slot(tx - 1), sx2 = slot(tx - 2);
-  const size_t sx3 = P.sw >= 4 ? slot(tx - 3) : 0, sx4 = P.sw >= 5 ? slot(tx - 4) : 0;
+  JunctionStep J;
+  J.th_n = th + slot(t);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) J.th_k[k] = th + (k < S ? slot(tx - 1 - k) : 0);
+  J.th_x = th + slot(tx);
+  J.ths = ths;
+  J.u = u;
+  const size_t col = static_cast<size_t>(w) * P.Nj;
   const int32_t tho = (update && R.th_slot) ? gbl(R.th_slot)[t - R.slot_t0] : -1;
   const int32_t Io = (update && R.I_slot) ? gbl(R.I_slot)[t - R.slot_t0] : -1;
-  const GAS double* __restrict__ c0 = gbl(P.coef);
-  const GAS vint2* __restrict__ jcode = gbl(reinterpret_cast<const vint2*>(P.junc_code));
-  GAS double* __restrict__ th_out = gblw(R.th_out);
-  GAS double* __restrict__ I_out = gblw(R.I_out);
-  for (int32_t p = threadIdx.x; 2 * p < Nj; p += BS) {
-    double z[2] = {0.0, 0.0};
-    if (prep && !R.T.zero && !R.noise)
-      normal_pair(R.seed, static_cast<uint32_t>(2 * p), noise_col, static_cast<uint64_t>(tx), &z[0], &z[1]);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int32_t j = 2 * p + h;
-      if (j >= Nj) break;
-      double tn = 0.0;
-      if (update) {
-        const vint2 code = jcode[j];
-        double y = 0.0;
-        if (code.x >= 0) y = (code.x & 1) ? y - zs[code.x >> 1] : y + zs[code.x >> 1];
-        if (code.y >= 0) y = (code.y & 1) ? y - zs[code.y >> 1] : y + zs[code.y >> 1];
-        const double x = th[sn + j];
-        tn = (y - x) / c0[j];
-        th[sn + j] = tn;
-        if (tho >= 0) th_out[static_cast<size_t>(tho) * slot_stride + static_cast<size_t>(w) * Nj + j] = tn;
-        if (Io >= 0)
-          I_out[static_cast<size_t>(Io) * slot_stride + static_cast<size_t>(w) * Nj + j] =
-              y + src_at(R.Is, j, w, t);
-        if (ths) ths[j] += src_at(R.Vs, j, w, t) * R.dt;
-      }
-      if (prep) {
-        const double t1 = update ? tn : th[sx1 + j];
-        const double t2 = th[sx2 + j];
-        const double t3 = P.sw >= 4 ? th[sx3 + j] : 0.0;
-        const double t4 = P.sw >= 5 ? th[sx4 + j] : 0.0;
-        double zz = z[h];
-        if (!R.T.zero && R.noise)
-          zz = gbl(R.noise)[static_cast<size_t>(tx - R.noise_t0) * slot_stride + static_cast<size_t>(w) * Nj + j];
-        const double x = junction_x(P, R, j, w, tx, t1, t2, t3, t4, zz);
-        th[sx + j] = x;
-        double uj = x / c0[j];
-        if (ths) uj = uj - ths[j];
-        u[j] = uj;
-      }
-    }
-  }
+  J.th_out = tho >= 0 ? gblw(R.th_out) + static_cast<size_t>(tho) * slot_stride + col : nullptr;
+  J.I_out = Io >= 0 ? gblw(R.I_out) + static_cast<size_t>(Io) * slot_stride + col : nullptr;
+  J.Is_t = term_view(R.Is, update, P.zero, w, t);
+  J.Vs_t = term_view(R.Vs, update && ths, P.zero, w, t);
+  J.Is_x = term_view(R.Is, prep, P.zero, w, tx);
+  J.t_zero = R.T.zero;
+  J.from_T = !R.sigma.base;
+  J.sig_x = term_view(J.from_T ? R.T : R.sigma, prep && !R.T.zero, P.zero, w, tx);
+  J.noise = (prep && !R.T.zero && R.noise)
+                ? gbl(R.noise) + static_cast<size_t>(tx - R.noise_t0) * slot_stride + col
+                : nullptr;
+  J.is_zero = R.Is.zero;
+  J.dt = R.dt;
+  J.seed = R.seed;
+  J.noise_col = noise_col;
+  J.tx = static_cast<uint64_t>(tx);
+  const bool fast = SW == 3 && update && prep && !ths && !J.th_out && !J.I_out && !J.noise && !J.from_T;
+  if (fast) junction_loop_fast<BS>(P, J, zs);
+  else if (update && prep) junction_loop<BS, SW, true, true>(P, J, zs);
+  else if (update) junction_loop<BS, SW, true, false>(P, J, zs);
+  else if (prep) junction_loop<BS, SW, false, true>(P, J, zs);
 }
 
-template <int BS>
+template <int BS, int SW>
 __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict__ probs,
                                                     const jja_problem_run* __restrict__ runs,
                                                     const WorkItem* __restrict__ work,
@@ -403,23 +590,22 @@ __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict
   const DevProblem& P = probs[wi.problem];
   const jja_problem_run& R = runs[wi.problem];
   const int64_t w = wi.column;
-  double* __restrict__ zs = lds;          // face vector, by solve position
-  double* __restrict__ ts = lds + P.Nf;   // stage scratch
+  double* __restrict__ zs = lds;  // face vector by solve position; stage scratch after it
   const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
   GAS double* __restrict__ th = gblw(R.theta) + static_cast<size_t>(w) * P.Nj;
   GAS double* __restrict__ ths = R.theta_s ? gblw(R.theta_s) + static_cast<size_t>(w) * P.Nj : nullptr;
   GAS double* __restrict__ u = gblw(R.u) + static_cast<size_t>(w) * P.Nj;
   const uint32_t noise_col = static_cast<uint32_t>(R.column_offset + w);
 
-  junction_pass<BS>(P, R, th, ths, u, zs, w, slot_stride, step0, false, step0, true, noise_col);
+  junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, step0, false, step0, true, noise_col);
   __syncthreads();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
     face_phase<BS>(P, R, u, zs, w, t);
     lds_barrier();
-    solve<BS>(P, lds);
+    solve(P, lds);
     // finish step t and prepare step t+1 in one pass
-    junction_pass<BS>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
+    junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     __syncthreads();
   }
 }
@@ -463,6 +649,7 @@ struct Plan {
   size_t work_cap = 0;
   int64_t lds_limit = 0;
   int32_t block_size = 1024;
+  int32_t stencil_width = 3;
 };
 
 template <typename T>
@@ -502,7 +689,7 @@ int validate(const jja_problem_desc& d) {
   if (!d.face_of_pos || !d.face_code || !d.face_slices || !d.slice_rows ||
       (d.n_solve_slices && (!d.solve_slices || !d.ent_val || !d.ent_off)))
     return fail(JJA_EINVAL, "jja_plan_create: missing face or solve arrays");
-  if (Nf > 65536 || ms > 65536) return fail(JJA_EINVAL, "face vector exceeds 16-bit indices");
+  if (static_cast<int64_t>(Nf) + ms > 65536) return fail(JJA_EINVAL, "face vector exceeds 16-bit indices");
   std::vector<char> seen(Nf, 0);
   for (int32_t q = 0; q < Nf; ++q) {
     const int32_t f = d.face_of_pos[q];
@@ -533,37 +720,59 @@ int validate(const jja_problem_desc& d) {
     for (int64_t e = m.base; e < m.base + static_cast<int64_t>(m.n_iter) * BS; ++e)
       if (d.face_code[e] < -1 || (d.face_code[e] >> 1) >= Nj) return fail(JJA_EINVAL, "face_code out of range");
   }
-  // the four records after the program are loaded ahead, never executed
-  for (int32_t k = d.n_solve_slices; k < d.n_solve_slices + 4; ++k)
-    if (int rc = check_slice(d.solve_slices[k], d.n_entries, true)) return rc;
-  for (int32_t k = 0; k < d.n_solve_slices; ++k) {
-    const jja_slice& m = d.solve_slices[k];
-    if (int rc = check_slice(m, d.n_entries, true)) return rc;
-    const bool block = m.flags & JJA_SLICE_BLOCK;
-    const int32_t G = m.flags & 0xff;
-    for (int32_t i = 0; i < BS / G; ++i) {
-      const int32_t r = d.slice_rows[m.rowbase + i];
-      if (r == -1) continue;
-      if (r < 0 || r >= Nf || (!block && (r - m.r0 < 0 || r - m.r0 >= ms)))
-        return fail(JJA_EINVAL, "solve slice row out of range");
+  // solve programs: one per wave of 64 lanes; each runs an even number of
+  // records and loads four more ahead, all of which must be in range
+  const int32_t NW = BS / kWave;
+  const int32_t* wp = d.solve_wave_ptr;
+  if (!wp || wp[0] != 0 || wp[NW] != d.n_solve_slices)
+    return fail(JJA_EINVAL, "solve_wave_ptr must cover the solve records");
+  if (d.n_entries < kMaxIter * kWave || d.n_index < kMaxIter * kWave)
+    return fail(JJA_EINVAL, "solve arrays too short");
+  if (d.face_copy_rows < 0 || d.face_copy_rows > Nf || d.face_copy_rows > ms)
+    return fail(JJA_EINVAL, "face_copy_rows out of range");
+  const int64_t lds_end = static_cast<int64_t>(Nf) + ms;
+  int32_t barriers = -1;
+  for (int32_t w = 0; w < NW; ++w) {
+    if (wp[w + 1] - wp[w] < 4 || (wp[w + 1] - wp[w] - 4) % 2)
+      return fail(JJA_EINVAL, "a wave program needs an even record count and 4 lookahead records");
+    int32_t nb = 0;
+    for (int32_t k = wp[w]; k < wp[w + 1]; ++k) {
+      const jja_slice& m = d.solve_slices[k];
+      const int32_t G = m.flags & 0xff, rows = m.flags >> 16;
+      if (!pow2_le(G, kWave)) return fail(JJA_EINVAL, "solve record lanes per row must be a power of two <= 64");
+      if (m.n_iter < 0 || m.n_iter > kMaxIter || rows < 0 || rows > kWave / G || (m.flags & 0xf800))
+        return fail(JJA_EINVAL, "solve record fields out of range");
+      if (m.base < 0 || (m.base % 2) || m.base + kMaxIter * kWave > d.n_entries || m.ibase < 0 ||
+          (m.ibase % 8) || m.ibase + kMaxIter * kWave > d.n_index || m.rowbase < 0 ||
+          m.rowbase + kWave / G > d.n_slice_rows)
+        return fail(JJA_EINVAL, "solve record data out of range or misaligned");
+      if (k >= wp[w + 1] - 4) continue;  // lookahead only: loaded, never executed
+      if (m.flags & JJA_SLICE_BARRIER) ++nb;
+      const bool block = m.flags & JJA_SLICE_BLOCK;
+      for (int32_t i = 0; i < rows; ++i) {
+        const int32_t r = d.slice_rows[m.rowbase + i];
+        if (r == -1) continue;
+        if (r < 0 || r >= Nf || (!block && (r - m.r0 < 0 || r - m.r0 >= ms)))
+          return fail(JJA_EINVAL, "solve slice row out of range");
+      }
+      for (int64_t e = m.ibase; e < m.ibase + kMaxIter * kWave; ++e)
+        if (d.ent_off[e] >= lds_end) return fail(JJA_EINVAL, "solve LDS offset out of range");
     }
-    const int64_t NQ = (m.n_iter + 3) / 4;
-    if (m.ibase < 0 || (m.ibase % 4) || m.ibase + 4 * NQ * BS > d.n_index || (m.base % 2))
-      return fail(JJA_EINVAL, "solve record offsets out of range or misaligned");
-    // off-block records read z (bytes [0, 8 Nf)), block records read t
-    const int64_t lo = block ? 8LL * Nf : 0, hi = block ? 8LL * (Nf + ms) : 8LL * Nf;
-    for (int64_t e = m.ibase; e < m.ibase + 4 * NQ * BS; ++e) {
-      const int64_t o = d.ent_off[e];
-      if (o < lo || o + 8 > hi || (o & 7)) return fail(JJA_EINVAL, "solve LDS offset out of range");
-    }
+    if (barriers < 0) barriers = nb;
+    if (nb != barriers) return fail(JJA_EINVAL, "wave programs disagree on their barrier count");
   }
   return JJA_OK;
 }
 
-template <int BS>
+template <int BS, int SW>
 int launch_evolve(Plan* plan, int32_t n_work, int64_t lds, int32_t n_steps, int64_t step0,
                   hipStream_t stream) {
-  auto* kern = evolve_kernel<BS>;
+  auto* kern = evolve_kernel<BS, SW>;
+  // the solve reads LDS at the plan's absolute byte offsets: the dynamic LDS
+  // array must start at address 0, i.e. the kernel must have no static LDS
+  hipFuncAttributes attr;
+  JJA_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kern)));
+  if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "evolve kernel has static LDS");
   JJA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
   hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
@@ -587,14 +796,30 @@ extern "C" {
 
 const char* jja_last_error(void) { return g_error.c_str(); }
 
+int jja_plan_validate(int32_t n_problems, const jja_problem_desc* descs) {
+  if (n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_validate: invalid arguments");
+  for (int32_t p = 0; p < n_problems; ++p) {
+    if (descs[p].block_size != descs[0].block_size)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size");
+    if (descs[p].stencil_width != descs[0].stencil_width)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
+    if (int rc = validate(descs[p])) return rc;
+  }
+  return JJA_OK;
+}
+
 int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan** out) {
   if (!out || n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_create: invalid arguments");
   *out = nullptr;
   for (int32_t p = 1; p < n_problems; ++p)
     if (descs[p].block_size != descs[0].block_size)
       return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size");
+  for (int32_t p = 1; p < n_problems; ++p)
+    if (descs[p].stencil_width != descs[0].stencil_width)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
   Plan* plan = new Plan();
   plan->block_size = descs[0].block_size;
+  plan->stencil_width = descs[0].stencil_width;
   int rc = JJA_OK;
   hipDeviceProp_t prop;
   if (hipGetDevice(&plan->device) != hipSuccess ||
@@ -615,6 +840,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.n_face_slices = d.n_face_slices;
     dp.n_solve_slices = d.n_solve_slices;
     dp.max_stage_rows = d.max_stage_rows;
+    dp.copy_rows = d.face_copy_rows;
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
@@ -623,7 +849,8 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     if (Nf) {
       if ((rc = upload(plan, d.face_of_pos, Nf, &dp.face_of_pos))) break;
       if ((rc = upload(plan, d.face_slices, static_cast<size_t>(d.n_face_slices), &dp.face_slices))) break;
-      if ((rc = upload(plan, d.solve_slices, static_cast<size_t>(d.n_solve_slices) + 4, &dp.solve_slices))) break;
+      if ((rc = upload(plan, d.solve_slices, static_cast<size_t>(d.n_solve_slices), &dp.solve_slices))) break;
+      if ((rc = upload(plan, d.solve_wave_ptr, static_cast<size_t>(d.block_size / kWave + 1), &dp.wave_ptr))) break;
       if ((rc = upload(plan, d.face_code, static_cast<size_t>(d.n_face_entries), &dp.face_code))) break;
       if ((rc = upload(plan, d.slice_rows, static_cast<size_t>(d.n_slice_rows), &dp.slice_rows))) break;
       if ((rc = upload(plan, d.ent_val, static_cast<size_t>(d.n_entries), &dp.ent_val))) break;
@@ -714,13 +941,19 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
                          hipMemcpyHostToDevice, stream));
   const int32_t n_work = static_cast<int32_t>(work.size());
   if (lds < 8) lds = 8;
+  const int sw = plan->stencil_width;
+#define JJA_LAUNCH(BSZ)                                                                     \
+  return sw == 3 ? launch_evolve<BSZ, 3>(plan, n_work, lds, n_steps, step0, stream)         \
+       : sw == 4 ? launch_evolve<BSZ, 4>(plan, n_work, lds, n_steps, step0, stream)         \
+                 : launch_evolve<BSZ, 5>(plan, n_work, lds, n_steps, step0, stream)
   switch (plan->block_size) {
-    case 64: return launch_evolve<64>(plan, n_work, lds, n_steps, step0, stream);
-    case 128: return launch_evolve<128>(plan, n_work, lds, n_steps, step0, stream);
-    case 256: return launch_evolve<256>(plan, n_work, lds, n_steps, step0, stream);
-    case 512: return launch_evolve<512>(plan, n_work, lds, n_steps, step0, stream);
-    default: return launch_evolve<1024>(plan, n_work, lds, n_steps, step0, stream);
+    case 64: JJA_LAUNCH(64);
+    case 128: JJA_LAUNCH(128);
+    case 256: JJA_LAUNCH(256);
+    case 512: JJA_LAUNCH(512);
+    default: JJA_LAUNCH(1024);
   }
+#undef JJA_LAUNCH
 }
 
 int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_t* n_out,

@@ -26,12 +26,27 @@ def _torch():
     return torch
 
 
+def source_layout(value, n, W, Nt):
+    """Device layout of an (n, W, Nt)-broadcastable input: the un-broadcast
+    elements in [t][w][i] order (element index fastest) and the element strides
+    (s_i, s_w, s_t), 0 on broadcast axes: element (i, w, t) is
+    flat[i*s_i + w*s_w + t*s_t]."""
+    view = np.broadcast_to(np.asarray(value, dtype=np.float64), (n, W, Nt))
+    keep = [st != 0 and d > 1 for d, st in zip(view.shape, view.strides)]
+    sub = view[tuple(slice(None) if k else slice(0, 1) for k in keep)]
+    dev = np.ascontiguousarray(sub.transpose(2, 1, 0))
+    ni, nw = dev.shape[2], dev.shape[1]
+    strides = (1 if keep[0] else 0, ni if keep[1] else 0, ni * nw if keep[2] else 0)
+    return dev.reshape(dev.shape[0], nw, ni).reshape(-1).copy(), strides
+
+
 class DeviceSource:
     """Device copy of one input (f, Is, Vs or T) plus its jja_source descriptor.
 
-    Arrays are uploaded un-broadcast; numpy's broadcast strides (0 on broadcast
-    axes) become the descriptor's element strides. Callables are evaluated on the
-    host for each launch's chunk of steps.
+    Arrays are uploaded un-broadcast and reordered to [t][w][i] (element index
+    fastest), so that consecutive threads read consecutive elements; a broadcast
+    axis keeps stride 0 and takes no memory. Callables are evaluated on the host
+    for each launch's chunk of steps.
     """
 
     def __init__(self, torch, value, n, W, Nt, zero):
@@ -42,13 +57,12 @@ class DeviceSource:
         self.src.zero = int(bool(zero))
         self.tensor = None
         if not self.callable:
-            arr = np.ascontiguousarray(np.array(value, dtype=np.float64))
-            view = np.broadcast_to(arr, (n, W, Nt))
-            self.tensor = torch.from_numpy(arr.reshape(-1).copy()).cuda()
+            dev, (s_i, s_w, s_t) = source_layout(value, n, W, Nt)
+            self.tensor = torch.from_numpy(dev).cuda()
             self.src.base = self.tensor.data_ptr()
-            self.src.s_i, self.src.s_w, self.src.s_t = (s // 8 for s in view.strides)
+            self.src.s_i, self.src.s_w, self.src.s_t = s_i, s_w, s_t
             self.src.t0 = 0
-            self.bytes_per_step = 8 * int(np.prod([d for d, st in zip((n, W), view.strides[:2]) if st]))
+            self.bytes_per_step = 8 * (n if s_i else 1) * (W if s_w else 1)
         else:
             self.bytes_per_step = 8 * n * W
 

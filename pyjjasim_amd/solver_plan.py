@@ -23,7 +23,8 @@ import scipy.sparse
 from . import _native
 from .ordering import nested_dissection
 
-MAX_ITER = 8          # entries per thread and slice record (kernel register budget)
+MAX_ITER = 8          # entries per lane and solve record (kernel register budget)
+WAVE = 64             # lanes of one wavefront: solve records are per wave
 
 STENCILS = {
     3: ((1.0, -1.0, 0.0), (1.0, -2.0, 1.0)),
@@ -340,8 +341,9 @@ class CircuitPlan:
 
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.block_size, self.solve_plan, self.Nf, self.Nj
-        if Nf and (Nf > 65536 or sp.max_stage_rows > 65536):
-            raise ValueError("face count exceeds the 16-bit LDS index range")
+        if Nf + sp.max_stage_rows > 65536:
+            raise ValueError(f"circuit with {Nf} faces is too large for the LDS-resident solve "
+                             "(16-bit LDS indices)")
         pos = sp.face_pos.astype(np.int64)
         # junctions: face positions of the (up to two) faces, ascending face index
         jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
@@ -410,50 +412,139 @@ class CircuitPlan:
             parts.append((ptr, (rb[rows][rr] - r0) + ar + c,
                           sp.inv_cm[boff[rows][rr] + ar * Br - ar * (ar - 1) // 2 + c]))
             per_stage.append((r0, rows, parts))
-        order = [(s, 0) for s in range(sp.n_stages)]
         prog = []
         for s in range(sp.n_stages):
             prog += [(s, 0), (s, 1)]
         for s in range(sp.n_stages - 1, -1, -1):
             prog += [(s, 2), (s, 3)]
-        del order
-        # solve records: values pair-interleaved (16-byte loads), indices lane-major
-        solve = []
-        vals, idxs = [], []
-        nv = ni = 0
-        tid = np.arange(BS)
+        phases = []
+        self.face_copy_rows = 0
         for s, kind in prog:
             r0, rows, parts = per_stage[s]
             pptr, pidx, pval = parts[kind]
-            L = sliced(pptr, pidx, pval, BS, rows, target_iter=target_iter)
-            recs = records(L, 0, r0, _native.SLICE_BLOCK if kind % 2 else 0, True)
-            # LDS byte offsets: z at 8*position, stage scratch t at 8*(Nf + row)
-            lds_base = 8 * Nf if kind % 2 else 0
-            for (base, rowbase, rr0, n, flags) in recs:
-                i = np.arange(n)[:, None]
-                src = base + i * BS + tid[None, :]
-                NP, NQ = -(-n // 2), -(-n // 4)
-                v = np.zeros(2 * BS * NP)
-                v[((i // 2) * 2 * BS + 2 * tid[None, :] + i % 2).ravel()] = L["val"][src].ravel()
-                ix = np.full(4 * BS * NQ, lds_base, dtype=np.int64)
-                ix[(tid[None, :] * 4 * NQ + i).ravel()] = lds_base + 8 * L["idx"][src].ravel()
-                solve.append((nv, ni, rowbase, rr0, n, flags))
-                vals.append(v)
-                idxs.append(ix)
-                nv += len(v)
-                ni += len(ix)
-        cat = lambda xs, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt)
-        # four records the kernel loads ahead but never executes
-        self.n_solve_exec = len(solve)
-        solve = solve + [(0, 0, 0, 0, 0, 1)] * 4
-        self.solve_slices = solve
+            if (s, kind) == (0, 0) and r0 == 0 and pptr[-1] == pptr[0]:
+                # leaves have no off-block entries: their scratch is b itself,
+                # written by the face phase
+                self.face_copy_rows = len(rows)
+                continue
+            # LDS indices in doubles: z at its position, stage scratch t at Nf + row
+            lds_base = Nf if kind % 2 else 0
+            phases.append((r0, rows, pptr, lds_base + np.asarray(pidx, dtype=np.int64), pval,
+                           _native.SLICE_BLOCK if kind % 2 else 0))
+        self._build_wave_program(phases, target_iter)
         self.face_slices = [(b, 0, rb, r0, n, fl) for (b, rb, r0, n, fl) in self.face_slices]
-        self.slice_rows = cat(rows_all, np.int32)
-        self.ent_off = cat(idxs, np.uint32)
-        self.ent_val = cat(vals, np.float64)
+        # face rows follow the solve rows (record row bases were taken before)
+        face_rows = rows_all[0]
+        self.face_slices = [(b, ib, rb + len(self.slice_rows), r0, n, fl)
+                            for (b, ib, rb, r0, n, fl) in self.face_slices]
+        self.slice_rows = np.concatenate((self.slice_rows, face_rows)).astype(np.int32)
         self._face_rec = self._slice_array(self.face_slices)
         self._solve_rec = self._slice_array(self.solve_slices)
+        self._wave_ptr = np.ascontiguousarray(self.wave_ptr, dtype=np.int32)
+
+    def _build_wave_program(self, phases, target_iter):
+        """Per-wave solve programs.
+
+        Each phase (rows of one stage and kind) is cut into chains of 64-lane
+        records: rows of equal G (the smallest power of two <= 64 with at most
+        ``target_iter`` entries per lane, from the row's own length), longest
+        first, 64/G rows per chain; lane j of a row takes entries j, j+G, ...
+        and a record holds 8 of them per lane (a longer row continues in the next
+        record of its chain). Chains go to the least-loaded wave (longest first);
+        the last record of every wave in a phase carries BARRIER (a wave without
+        work gets a no-op record), so all waves meet at every phase boundary.
+        Record data: value `it` of lane l at base + (it/2)*128 + 2l + it%2, LDS
+        index `it` (in doubles) of lane l at ibase + 8l + it, row of lane l at
+        rowbase + l/G. Block 0 of every array is zero (the no-op record).
+        """
+        NW = self.block_size // WAVE
+        vals = [np.zeros(512)]
+        offs = [np.zeros(512, dtype=np.int64)]
+        rows_out = [-np.ones(WAVE, dtype=np.int64)]
+        nv, ni, nr = 512, 512, WAVE
+        noop = (0, 0, 0, 0, 0, 1)
+        progs = [[] for _ in range(NW)]
+        lane = np.arange(WAVE)
+        for (r0, rows, ptr, idx, val, kflag) in phases:
+            ptr = np.asarray(ptr, dtype=np.int64)
+            lens = np.diff(ptr)
+            need = np.maximum(1, -(-lens // target_iter))
+            Gsel = np.minimum(WAVE, 2 ** np.ceil(np.log2(need)).astype(np.int64))
+            chains = []
+            for g in sorted(set(Gsel.tolist())):
+                sel = np.nonzero(Gsel == g)[0]
+                sel = sel[np.argsort(-lens[sel], kind="stable")]
+                R = WAVE // g
+                for i in range(0, len(sel), R):
+                    rr = sel[i:i + R]
+                    n_it = -(-int(lens[rr[0]]) // g)
+                    nch = max(1, -(-n_it // MAX_ITER))
+                    v = np.zeros((nch, MAX_ITER, WAVE))
+                    o = np.zeros((nch, MAX_ITER, WAVE), dtype=np.int64)
+                    L = lens[rr]
+                    slot = np.repeat(np.arange(len(rr)), L)
+                    e = np.arange(int(L.sum())) - np.repeat(np.cumsum(L) - L, L)
+                    src = np.repeat(ptr[rr], L) + e
+                    k = e // g
+                    ln = slot * g + e % g
+                    v[k // MAX_ITER, k % MAX_ITER, ln] = val[src]
+                    o[k // MAX_ITER, k % MAX_ITER, ln] = idx[src]
+                    rws = -np.ones(WAVE, dtype=np.int64)
+                    rws[:len(rr)] = rows[rr]
+                    chains.append((g, len(rr), nch, v, o, rws, n_it))
+            order = sorted(range(len(chains)), key=lambda c: -chains[c][2])
+            load = np.zeros(NW, dtype=np.int64)
+            assign = [[] for _ in range(NW)]
+            for c in order:
+                w = int(np.argmin(load))
+                load[w] += chains[c][2]
+                assign[w].append(c)
+            for w in range(NW):
+                recs = []
+                for c in assign[w]:
+                    g, nrow, nch, v, o, rws, n_it = chains[c]
+                    rowbase = nr
+                    rows_out.append(rws[:WAVE // g])
+                    nr += WAVE // g
+                    for ch in range(nch):
+                        vb = np.zeros(512)
+                        ob = np.zeros(512, dtype=np.int64)
+                        for it in range(MAX_ITER):
+                            vb[(it // 2) * 128 + 2 * lane + it % 2] = v[ch, it]
+                            ob[8 * lane + it] = o[ch, it]
+                        vals.append(vb)
+                        offs.append(ob)
+                        flags = g | kflag | (nrow << 16)
+                        if ch == nch - 1:
+                            flags |= _native.SLICE_FINISH
+                        recs.append((nv, ni, rowbase, r0, min(MAX_ITER, n_it - ch * MAX_ITER), flags))
+                        nv += 512
+                        ni += 512
+                if not recs:
+                    recs.append(noop)
+                b_, ib_, rb_, r0_, it_, fl_ = recs[-1]
+                recs[-1] = (b_, ib_, rb_, r0_, it_, fl_ | _native.SLICE_BARRIER)
+                progs[w] += recs
+        solve, wave_ptr, n_exec = [], [], 0
+        for w in range(NW):
+            recs = progs[w]
+            if len(recs) % 2:
+                recs = recs + [noop]
+            wave_ptr.append(len(solve))
+            n_exec += len(recs)
+            solve += recs + [noop] * 4
+        wave_ptr.append(len(solve))
+        self.solve_slices = solve
+        self.wave_ptr = np.asarray(wave_ptr, dtype=np.int64)
+        self.n_solve_exec = n_exec
+        self.ent_val = np.concatenate(vals).astype(np.float64)
+        self.ent_off = np.concatenate(offs).astype(np.uint16)
+        self.slice_rows = np.concatenate(rows_out).astype(np.int32)
         self.n_slices = len(solve)
+
+    def wave_program(self, w):
+        """Executed records of wave w (the four lookahead records stripped)."""
+        return self.solve_slices[int(self.wave_ptr[w]):int(self.wave_ptr[w + 1]) - 4]
 
     @staticmethod
     def _slice_array(recs):
@@ -476,9 +567,10 @@ class CircuitPlan:
         d.junc_code = P(self.junc_pos_code, C.c_int32)
         d.face_of_pos = P(self.face_of_pos, C.c_int32)
         d.n_face_slices = len(self.face_slices)
-        d.n_solve_slices = self.n_solve_exec
+        d.n_solve_slices = len(self.solve_slices)
         d.face_slices = C.cast(self._face_rec, C.c_void_p)
         d.solve_slices = C.cast(self._solve_rec, C.c_void_p)
+        d.solve_wave_ptr = P(self._wave_ptr, C.c_int32)
         d.n_face_entries = len(self.face_code_sliced)
         d.face_code = P(self.face_code_sliced, C.c_int32)
         d.n_slice_rows = len(self.slice_rows)
@@ -486,8 +578,9 @@ class CircuitPlan:
         d.n_entries = len(self.ent_val)
         d.ent_val = P(self.ent_val, C.c_double)
         d.n_index = len(self.ent_off)
-        d.ent_off = P(self.ent_off, C.c_uint32)
+        d.ent_off = P(self.ent_off, C.c_uint16)
         d.max_stage_rows = sp.max_stage_rows
+        d.face_copy_rows = self.face_copy_rows
         return d
 
     def lds_bytes(self):
@@ -495,15 +588,16 @@ class CircuitPlan:
 
     # ---------------------------------------------------- host emulation
     def factor_bytes(self):
-        """Bytes of solve-program data one column loads per step (live row slots)."""
-        n = 0
-        for (_, _, _, _, it, fl) in self.solve_slices[:self.n_solve_exec]:
-            n += (fl >> 16) * (fl & 0xFF) * (16 * (-(-it // 2)) + 16 * (-(-it // 4)) + 4)
-        return int(n + 32 * self.n_solve_exec)
+        """Bytes of solve-program data one column loads per step: every executed
+        record moves 8 values and 8 LDS indices per lane, a row per lane and its
+        32-byte descriptor."""
+        return int(self.n_solve_exec * (512 * 8 + 512 * 2 + WAVE * 4 + 32))
 
     def emulate_face_and_solve(self, u):
-        """Exactly the kernel's face phase and slice program, on the host:
-        u (Nj,) = x/c0 - theta_s  ->  J (Nf,) = K^-1 (A u)  (f = 0)."""
+        """Exactly the kernel's face phase and per-wave solve programs, on the host:
+        u (Nj,) = x/c0 - theta_s  ->  J (Nf,) = K^-1 (A u)  (f = 0). Lanes
+        accumulate with FMA-free numpy adds of products, so sums agree with the
+        kernel to rounding, not bit for bit."""
         BS, Nf = self.block_size, self.Nf
         sp = self.solve_plan
         zs = np.zeros(Nf)
@@ -519,25 +613,46 @@ class CircuitPlan:
                 pos = self.slice_rows[rowbase + tids]
                 ok = pos >= 0
                 zs[pos[ok]] = acc[ok]
+                cp = ok & (pos < self.face_copy_rows)
+                ts[pos[cp]] = acc[cp]
                 acc = np.zeros(BS)
-        acc = np.zeros(BS)
-        for (vbase, ibase, rowbase, r0, n_it, flags) in self.solve_slices[:self.n_solve_exec]:
-            G = flags & 0xFF
-            block = bool(flags & _native.SLICE_BLOCK)
-            NQ = -(-n_it // 4)
-            live = (tids // G) < (flags >> 16)
+        # split every wave's program at its barriers; phase p of all waves runs
+        # before phase p+1 (waves meet at each barrier)
+        NW = BS // WAVE
+        phases = []
+        for w in range(NW):
+            cur, mine = [], []
+            for rec in self.wave_program(w):
+                cur.append(rec)
+                if rec[5] & _native.SLICE_BARRIER:
+                    mine.append(cur)
+                    cur = []
+            assert not cur or all(r[5] & 0xFFFF0000 == 0 for r in cur), "work after the last barrier"
+            phases.append(mine)
+        n_ph = len(phases[0])
+        assert all(len(p) == n_ph for p in phases), "waves disagree on the barrier count"
+        lane = np.arange(WAVE)
+        for ph in range(n_ph):
             lds = np.concatenate((zs, ts))
-            for it in range(n_it):
-                v = self.ent_val[vbase + (it // 2) * 2 * BS + 2 * tids + it % 2]
-                off = self.ent_off[ibase + tids * 4 * NQ + it].astype(np.int64)
-                acc = acc + np.where(live, v * lds[off // 8], 0.0)
-            if flags & _native.SLICE_FINISH:
-                red = acc.reshape(BS // G, G).sum(axis=1)
-                rows = self.slice_rows[rowbase + np.arange(BS // G)]
-                ok = rows >= 0
+            writes = []
+            for w in range(NW):
+                acc = np.zeros(WAVE)
+                for (vbase, ibase, rowbase, r0, n_it, flags) in phases[w][ph]:
+                    G = flags & 0xFF
+                    for it in range(MAX_ITER):
+                        v = self.ent_val[vbase + (it // 2) * 128 + 2 * lane + it % 2]
+                        off = self.ent_off[ibase + 8 * lane + it].astype(np.int64)
+                        acc = acc + v * lds[off]
+                    if flags & _native.SLICE_FINISH:
+                        red = acc.reshape(WAVE // G, G).sum(axis=1)
+                        rows = self.slice_rows[rowbase + np.arange(WAVE // G)]
+                        live = np.arange(WAVE // G) < (flags >> 16)
+                        ok = live & (rows >= 0)
+                        writes.append((bool(flags & _native.SLICE_BLOCK), rows[ok], r0, red[ok]))
+                        acc = np.zeros(WAVE)
+            for block, rows, r0, red in writes:
                 if block:
-                    zs[rows[ok]] = red[ok]
+                    zs[rows] = red
                 else:
-                    ts[rows[ok] - r0] = zs[rows[ok]] - red[ok]
-                acc = np.zeros(BS)
+                    ts[rows - r0] = zs[rows] - red
         return zs[sp.face_pos]

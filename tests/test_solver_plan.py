@@ -107,3 +107,41 @@ def test_sliced_layout_holed_array():
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
     np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12,
                                atol=1e-12 * abs(want).max())
+
+
+@pytest.mark.parametrize("BS", [64, 1024])
+@pytest.mark.parametrize("name", ["sq10", "hc7", "tri8", "sqL", "het", "squid"])
+def test_plan_descriptors_pass_native_validation(name, BS):
+    """Every plan the host builds passes the library's own index and offset checks
+    (the ones jja_plan_create runs before any upload); no device is touched."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    plan = CircuitPlan(_circuits()[name], 0.05, leaf_size=8, block_size=BS)
+    d = plan.desc()
+    assert plan.n_solve_exec % 2 == 0
+    assert _native.lib().jja_plan_validate(1, C.byref(d)) == 0, _native.lib().jja_last_error()
+
+
+def test_native_validation_rejects_bad_programs():
+    """A corrupted LDS offset, a wave program of odd length or waves that disagree
+    on their barriers are refused (EINVAL) before anything runs."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    lib = _native.lib()
+
+    def fresh():
+        return CircuitPlan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128)
+    plan = fresh()
+    k = int(plan.wave_ptr[0])
+    plan.ent_off[plan.solve_slices[k][1] + 3] = plan.Nf + plan.solve_plan.max_stage_rows
+    assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
+    assert b"offset" in lib.jja_last_error()
+    plan = fresh()
+    plan._wave_ptr[1] -= 1
+    assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
+    plan = fresh()
+    w1 = [i for i in range(int(plan.wave_ptr[1]), int(plan.wave_ptr[2]) - 4)
+          if plan._solve_rec[i].flags & _native.SLICE_BARRIER]
+    plan._solve_rec[w1[0]].flags &= ~_native.SLICE_BARRIER
+    assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
+    assert b"barrier" in lib.jja_last_error()
