@@ -77,8 +77,8 @@ const char* jja_factor_last_error(void);
  *     16-byte load per lane); the row of lane l at slice_rows[rowbase + l/G].
  *     An index addresses the workgroup's LDS in doubles: z at its position,
  *     the stage scratch t at n_faces + stage row. Wave w of the workgroup runs records solve_wave_ptr[w] ..
- *     solve_wave_ptr[w+1]-5: an even count (a record with G 1 and no rows is a
- *     no-op), followed by 4 records that it loads ahead but never executes.
+ *     solve_wave_ptr[w+1]-5: a multiple of three (a record with G 1 and no rows
+ *     is a no-op), followed by 4 records that it loads ahead but never executes.
  *     Every wave's program has the same number of BARRIER records.
  * A row may span several consecutive slices; only the last one (flag FINISH)
  * reduces over the G lanes and writes the row. Row slots are filled from the

@@ -195,9 +195,9 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice&
   const int lg = __builtin_ctz(static_cast<unsigned>(m.flags & 0xff));
   const GAS vdouble2* __restrict__ vp = gbl(reinterpret_cast<const vdouble2*>(P.ent_val + m.base)) + lane;
   const GAS vuint4* __restrict__ op = gbl(reinterpret_cast<const vuint4*>(P.ent_off + m.ibase)) + lane;
-  const int slot = lane >> lg;
-  const int32_t row = gbl(P.slice_rows)[m.rowbase + slot];
-  r.row = slot < (m.flags >> 16) ? row : -1;
+  // the raw row; whether this lane's slot holds one is decided when the record
+  // is processed (a select here would wait for the load)
+  r.row = gbl(P.slice_rows)[m.rowbase + (lane >> lg)];
 #pragma unroll
   for (int q = 0; q < kMaxIter / 2; ++q) {
     const vdouble2 d = vp[q * kWave];
@@ -229,7 +229,8 @@ __device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t 
   if (m.flags & JJA_SLICE_FINISH) {
     const int G = m.flags & 0xff;
     acc = group_sum_last(acc, G);
-    if (r.row >= 0 && (lane & (G - 1)) == G - 1) {
+    const int slot = lane >> __builtin_ctz(static_cast<unsigned>(G));
+    if (slot < (m.flags >> 16) && r.row >= 0 && (lane & (G - 1)) == G - 1) {
       if (m.flags & JJA_SLICE_BLOCK)
         lds[r.row] = acc;
       else
@@ -241,14 +242,14 @@ __device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t 
 }
 
 // The whole forward + backward solve. Each wave walks its own record program,
-// software-pipelined in two register buffers used in turn: the loads of record
-// k+1 are issued before record k is processed (the scheduling barriers keep
-// the compiler from hoisting record k's LDS address use above them, which
-// would make it wait for record k's loads first), and descriptors are read one
-// pair ahead. A wave's executed record count is even and four never-executed
-// records follow it, so the loop has a single exit and every lookahead load
-// stays inside the program. Waves meet only at BARRIER records (one per phase,
-// the same number in every wave).
+// software-pipelined in three register buffers used in turn: the loads of
+// records k+1 and k+2 are in flight while record k is processed (the
+// scheduling barriers keep the compiler from hoisting a record's LDS address
+// use above the next record's loads, which would make it wait for all loads),
+// and descriptors are read ahead. A wave's executed record count is a multiple
+// of three and four never-executed records follow it, so the loop has a single
+// exit and every lookahead load stays inside the program. Waves meet only at
+// BARRIER records (one per phase, the same number in every wave).
 __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ lds) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
@@ -256,21 +257,27 @@ __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ 
   const int32_t k0 = __builtin_amdgcn_readfirstlane(wp[0]);
   const int32_t nk = __builtin_amdgcn_readfirstlane(wp[1]) - k0 - 4;
   const jja_slice* __restrict__ prog = P.solve_slices + k0;
-  jja_slice m0 = load_rec(prog, 0), m1 = load_rec(prog, 1);
-  SliceRegs A, B;
+  jja_slice m0 = load_rec(prog, 0), m1 = load_rec(prog, 1), m2 = load_rec(prog, 2);
+  SliceRegs A, B, C;
   load_slice(P, m0, lane, A);
+  load_slice(P, m1, lane, B);
   double acc = 0.0;
-  for (int32_t k = 0; k < nk; k += 2) {
-    const jja_slice m2 = load_rec(prog, k + 2);
-    load_slice(P, m1, lane, B);
+  for (int32_t k = 0; k < nk; k += 3) {
+    const jja_slice m3 = load_rec(prog, k + 3);
+    load_slice(P, m2, lane, C);
     __builtin_amdgcn_sched_barrier(0);
     process_slice(lds, P.Nf, lane, m0, A, acc);
-    const jja_slice m3 = load_rec(prog, k + 3);
-    load_slice(P, m2, lane, A);
+    const jja_slice m4 = load_rec(prog, k + 4);
+    load_slice(P, m3, lane, A);
     __builtin_amdgcn_sched_barrier(0);
     process_slice(lds, P.Nf, lane, m1, B, acc);
-    m0 = m2;
-    m1 = m3;
+    const jja_slice m5 = load_rec(prog, k + 5);
+    load_slice(P, m4, lane, B);
+    __builtin_amdgcn_sched_barrier(0);
+    process_slice(lds, P.Nf, lane, m2, C, acc);
+    m0 = m3;
+    m1 = m4;
+    m2 = m5;
   }
 }
 
@@ -720,7 +727,7 @@ int validate(const jja_problem_desc& d) {
     for (int64_t e = m.base; e < m.base + static_cast<int64_t>(m.n_iter) * BS; ++e)
       if (d.face_code[e] < -1 || (d.face_code[e] >> 1) >= Nj) return fail(JJA_EINVAL, "face_code out of range");
   }
-  // solve programs: one per wave of 64 lanes; each runs an even number of
+  // solve programs: one per wave of 64 lanes; each runs a multiple of three
   // records and loads four more ahead, all of which must be in range
   const int32_t NW = BS / kWave;
   const int32_t* wp = d.solve_wave_ptr;
@@ -733,8 +740,8 @@ int validate(const jja_problem_desc& d) {
   const int64_t lds_end = static_cast<int64_t>(Nf) + ms;
   int32_t barriers = -1;
   for (int32_t w = 0; w < NW; ++w) {
-    if (wp[w + 1] - wp[w] < 4 || (wp[w + 1] - wp[w] - 4) % 2)
-      return fail(JJA_EINVAL, "a wave program needs an even record count and 4 lookahead records");
+    if (wp[w + 1] - wp[w] < 4 || (wp[w + 1] - wp[w] - 4) % 3)
+      return fail(JJA_EINVAL, "a wave program needs a record count divisible by 3 and 4 lookahead records");
     int32_t nb = 0;
     for (int32_t k = wp[w]; k < wp[w + 1]; ++k) {
       const jja_slice& m = d.solve_slices[k];

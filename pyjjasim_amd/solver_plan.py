@@ -528,8 +528,7 @@ class CircuitPlan:
         solve, wave_ptr, n_exec = [], [], 0
         for w in range(NW):
             recs = progs[w]
-            if len(recs) % 2:
-                recs = recs + [noop]
+            recs = recs + [noop] * (-len(recs) % 3)   # the kernel runs records in threes
             wave_ptr.append(len(solve))
             n_exec += len(recs)
             solve += recs + [noop] * 4
