@@ -105,6 +105,74 @@ def cpu_baseline(pa, n, W, budget_s):
                        f"{t:.1f} s total")
 
 
+def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=5, fd_steps=200):
+    """Differentiation workload: SquareArray(n, n), trainable per-junction Ic
+    (log-normal, sigma 0.2, seed A), loss = mean over W bias points (0.5..2.0 Ic)
+    of (V_w - V*_w)^2 with V_w the time-averaged voltage of the biased junctions
+    and V* from a hidden Ic field (seed B); beta_C = 1, f = 0.1, T = 0, dt = 0.05.
+
+    Timed: the forward pass and the backward pass (segment recompute plus the
+    adjoint sweep), each after a warm-up. Checked: the gradient against central
+    differences (h = 1e-6) on a few junctions over the first ``fd_steps`` steps;
+    over the full horizon the dynamics are chaotic and central differences do not
+    converge as h shrinks, so no difference quotient can check it there."""
+    c = pa.SquareArray(n, n)
+    c.set_capacitance(1.0)
+    Nj, Nf = c.junction_count(), c.face_count()
+    dt = 0.05
+    base = c.current_base(0.0)
+    biased = torch.tensor(np.abs(base) > 0, device="cuda")
+    dev = torch.device("cuda")
+    Is = torch.tensor(base[:, None] * np.linspace(0.5, 2.0, W)[None, :], dtype=torch.float64, device=dev)
+    f = torch.full((Nf,), 0.1, dtype=torch.float64, device=dev)
+    Ic0 = torch.tensor(np.exp(0.2 * np.random.default_rng(101).standard_normal(Nj)), dtype=torch.float64, device=dev)
+    Ic_star = torch.tensor(np.exp(0.2 * np.random.default_rng(202).standard_normal(Nj)), dtype=torch.float64, device=dev)
+
+    def make(steps):
+        store = np.zeros(steps, dtype=bool)
+        store[-1] = True
+        evo = pa.DifferentiableEvolution(c, time_step=dt, time_step_count=steps, problem_count=W,
+                                         store_time_steps=store, checkpoint_every=checkpoint)
+
+        def volt(Ic):
+            th = evo(Ic, f, Is)[:, :, -1]
+            return th[biased].mean(0) / (steps * dt)
+        with torch.no_grad():
+            target = volt(Ic_star)
+        return lambda Ic: ((volt(Ic) - target) ** 2).mean()
+
+    loss = make(Nt)
+    times = []
+    for rep in range(2):                          # the first pass warms up
+        Ic = Ic0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = loss(Ic)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        times.append((t1 - t0, time.perf_counter() - t1))
+    t_fwd, t_bwd = times[-1]
+    loss_fd = make(fd_steps)
+    Ic = Ic0.clone().requires_grad_(True)
+    loss_fd(Ic).backward()
+    g = Ic.grad.detach().cpu().numpy()
+    idx = np.random.default_rng(303).choice(Nj, size=n_fd, replace=False)
+    h, errs = 1e-6, []
+    for j in idx:
+        p, m = Ic0.clone(), Ic0.clone()
+        p[j] += h
+        m[j] -= h
+        with torch.no_grad():
+            fd = (loss_fd(p) - loss_fd(m)).item() / (2 * h)
+        errs.append(abs(g[j] - fd) / max(abs(fd), 1e-3 * np.abs(g).max()))
+    return {"workload": f"SquareArray({n},{n}) W={W} Nt={Nt} checkpoint={checkpoint}, d loss / d Ic",
+            "forward_s": t_fwd, "backward_s": t_bwd, "grad_over_fwd": t_bwd / t_fwd,
+            "fd_check": {"steps": fd_steps, "junctions": int(n_fd), "h": h,
+                         "max_rel_err": float(max(errs))}}
+
+
 def measured_traffic(steps_per_launch):
     """HBM bytes per evolve_kernel launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor), or
@@ -150,6 +218,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -214,6 +283,8 @@ def main():
                      "mean_launch_ms": mean_launch_ms},
         "plain": plain,
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(pa, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pa, args.n, W, args.cpu_seconds)
     print(json.dumps(out), flush=True)

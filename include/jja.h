@@ -191,6 +191,7 @@ typedef struct jja_problem_run {
   int64_t slot_t0;
   double* th_out;        /* [slots][W][Nj] */
   double* I_out;         /* [slots][W][Nj] */
+  const double* Ic;      /* [Nj] critical currents of this run, or NULL: the plan's */
 } jja_problem_run;
 
 /*
@@ -202,6 +203,49 @@ typedef struct jja_problem_run {
  */
 int jja_evolve(jja_plan* plan, const jja_problem_run* runs, int32_t n_steps, int64_t step0,
                void* stream);
+
+/* --------------------------------------------------------------- adjoint */
+
+/*
+ * Reverse-mode sweep of the stencil-3 step (DefaultCPR) for the gradients of a
+ * loss of the stored phases. The step is theta_k = M x_k + (terms in f, theta_s)
+ * with M = D A^T K^-1 A D - D, D = diag(1/c0), symmetric; x_k depends on
+ * theta_{k-1}, theta_{k-2} through a_k = 2 Ic cos(arg_k) + c1 and
+ * b_k = c2 - Ic cos(arg_k), arg_k = 2 theta_{k-1} - theta_{k-2}. With lambda_k
+ * the adjoint of theta_k and mu_k = M lambda_k (the same face phase, solve and
+ * junction gather as the forward step, applied to lambda):
+ *     lambda_{k-1} += a_k mu_k,   lambda_{k-2} += b_k mu_k,
+ *     dL/dIc += sin(arg_k) mu_k,  dL/dIs_k = -mu_k,
+ *     dL/df_k = -2 pi K^-1 A D lambda_k  (the solve's face vector).
+ * All pointers are DEVICE pointers; arrays are [W][N] per problem.
+ */
+typedef struct jja_adjoint_run {
+  int32_t W;
+  int32_t prime;          /* 1: first call of a sweep (see jja_adjoint) */
+  const double* Ic;       /* [Nj] or NULL: the plan's */
+  const double* hist;     /* theta_m at hist[(m - hist_m0) * W * Nj + w * Nj + j] for
+                             m in [k_lo - 2, k_hi - 1] */
+  int64_t hist_m0;
+  double* lam;            /* [2][W][Nj] adjoint of step m in slot m mod 2 */
+  double* u;              /* [W][Nj] scratch: lambda_k / c0 of the step being solved */
+  const double* g;        /* [slots][W][Nj] upstream gradient of stored theta */
+  const int32_t* g_slot;  /* slot of step m - slot_t0 in g, or -1 */
+  int64_t slot_t0;
+  int64_t g_lo;           /* g_slot covers steps [g_lo, g_hi) */
+  int64_t g_hi;
+  double* grad_Ic;        /* [W][Nj] += sum_k sin(arg_k) mu_k */
+  double* grad_Is;        /* [W][Nj] += sum_k -mu_k */
+  double* grad_f;         /* [W][Nf] by solve position: += sum_k -2 pi K^-1 A D lambda_k */
+} jja_adjoint_run;
+
+/*
+ * Process steps k_hi down to k_lo (k_hi >= k_lo >= 0). On entry lam[k_hi mod 2]
+ * holds lambda_{k_hi} complete and u = lambda_{k_hi}/c0, and lam[(k_hi - 1) mod 2]
+ * the contributions to lambda_{k_hi - 1} from later steps; with prime = 1 the
+ * kernel first adds g_{k_hi} to lam[k_hi mod 2] and forms u itself. On exit the
+ * same holds for k_lo - 1. The stencil width must be 3.
+ */
+int jja_adjoint(jja_plan* plan, const jja_adjoint_run* runs, int64_t k_hi, int64_t k_lo, void* stream);
 
 /*
  * Vortex configuration n[f] = -sum_j A[f,j] rint(theta_j / (2 pi))

@@ -2,3 +2,11 @@
 from .graph import *  # noqa: F401,F403
 from .circuit import *  # noqa: F401,F403
 from .time_evolution import *  # noqa: F401,F403
+
+
+def __getattr__(name):
+    # the differentiable path needs torch; import it only when asked for
+    if name == "DifferentiableEvolution":
+        from .adjoint import DifferentiableEvolution
+        return DifferentiableEvolution
+    raise AttributeError(name)

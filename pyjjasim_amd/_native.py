@@ -66,7 +66,16 @@ class ProblemRun(C.Structure):
         ("Is", Source), ("Vs", Source), ("f", Source), ("T", Source), ("sigma", Source),
         ("noise", C.c_void_p), ("noise_t0", C.c_int64), ("seed", C.c_uint64),
         ("th_slot", C.c_void_p), ("I_slot", C.c_void_p), ("slot_t0", C.c_int64),
-        ("th_out", C.c_void_p), ("I_out", C.c_void_p),
+        ("th_out", C.c_void_p), ("I_out", C.c_void_p), ("Ic", C.c_void_p),
+    ]
+
+
+class AdjointRun(C.Structure):
+    _fields_ = [
+        ("W", C.c_int32), ("prime", C.c_int32), ("Ic", C.c_void_p), ("hist", C.c_void_p),
+        ("hist_m0", C.c_int64), ("lam", C.c_void_p), ("u", C.c_void_p), ("g", C.c_void_p),
+        ("g_slot", C.c_void_p), ("slot_t0", C.c_int64), ("g_lo", C.c_int64), ("g_hi", C.c_int64),
+        ("grad_Ic", C.c_void_p), ("grad_Is", C.c_void_p), ("grad_f", C.c_void_p),
     ]
 
 
@@ -78,7 +87,7 @@ _lib = None
 
 EXPORTS = ["jja_cholesky", "jja_factor_get", "jja_factor_free", "jja_factor_last_error",
            "jja_plan_validate", "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes", "jja_evolve",
-           "jja_vortex", "jja_last_error"]
+           "jja_vortex", "jja_adjoint", "jja_last_error"]
 
 
 def lib():
@@ -101,6 +110,7 @@ def lib():
     L.jja_plan_lds_bytes.restype = C.c_int64
     L.jja_evolve.argtypes = [C.c_void_p, C.POINTER(ProblemRun), C.c_int32, C.c_int64, C.c_void_p]
     L.jja_vortex.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    L.jja_adjoint.argtypes = [C.c_void_p, C.POINTER(AdjointRun), C.c_int64, C.c_int64, C.c_void_p]
     L.jja_last_error.restype = C.c_char_p
     _lib = L
     return L

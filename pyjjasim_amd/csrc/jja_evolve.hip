@@ -285,7 +285,7 @@ __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ 
 // A face longer than one record continues in the next record (flag FINISH on
 // its last one).
 template <int BS>
-__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_problem_run& R,
+__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f,
                                            const GAS double* __restrict__ u, double* __restrict__ zs,
                                            int64_t w, int64_t t) {
   const int tid = threadIdx.x;
@@ -306,7 +306,7 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_proble
     if ((m.flags & JJA_SLICE_FINISH) && live) {
       const int32_t pos = gbl(P.slice_rows)[m.rowbase + tid];
       if (pos >= 0) {
-        if (!R.f.zero) acc = acc - kTwoPi * src_at(R.f, gbl(P.face_of_pos)[pos], w, t);
+        if (f && !f->zero) acc = acc - kTwoPi * src_at(*f, gbl(P.face_of_pos)[pos], w, t);
         zs[pos] = acc;
         if (pos < P.copy_rows) zs[P.Nf + pos] = acc;  // first stage: scratch = b
       }
@@ -353,6 +353,7 @@ struct JunctionStep {
   TermView Is_t, Vs_t;         // step t (update)
   TermView Is_x, sig_x;        // step tx (prep); sig_x is sigma, or T when from_T
   const GAS double* noise;     // reference normals of step tx, or NULL
+  const GAS double* Ic;        // critical currents (the run's, else the plan's)
   bool is_zero, t_zero, from_T;
   double dt;
   uint64_t seed;
@@ -385,7 +386,7 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
   if (PREP) {
 #pragma unroll
     for (int k = UPDATE ? 1 : 0; k < SW - 1; ++k) t[k] = J.th_k[k][j];
-    Ic = gbl(P.Ic)[j];
+    Ic = J.Ic[j];
 #pragma unroll
     for (int k = 1; k < SW; ++k) c[k] = coef[static_cast<int64_t>(k) * Nj + j];
     isx = term_at(J.Is_x, j);
@@ -499,7 +500,7 @@ __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionSt
   L.t2 = J.th_k[1][j];
   L.c1 = coef[static_cast<int64_t>(Nj) + j];
   L.c2 = coef[2 * static_cast<int64_t>(Nj) + j];
-  L.Ic = gbl(P.Ic)[j];
+  L.Ic = J.Ic[j];
   L.isx = term_at(J.Is_x, j);
   L.sg = term_at(J.sig_x, j);
 }
@@ -575,6 +576,7 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   J.noise = (prep && !R.T.zero && R.noise)
                 ? gbl(R.noise) + static_cast<size_t>(tx - R.noise_t0) * slot_stride + col
                 : nullptr;
+  J.Ic = gbl(R.Ic ? R.Ic : P.Ic);
   J.is_zero = R.Is.zero;
   J.dt = R.dt;
   J.seed = R.seed;
@@ -608,11 +610,114 @@ __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict
   __syncthreads();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
-    face_phase<BS>(P, R, u, zs, w, t);
+    face_phase<BS>(P, &R.f, u, zs, w, t);
     lds_barrier();
     solve(P, lds);
     // finish step t and prepare step t+1 in one pass
     junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ adjoint
+// Adjoint of theta_k wherever a run stores it: lam[m mod 2].
+struct AdjCol {
+  GAS double* lam;  // slot m mod 2 at lam + (m & 1) * stride
+  const GAS double* g;  // this column of the upstream gradient (slot s at g + s * stride)
+  GAS double* u;
+  const GAS double* hist;
+  int64_t hist_m0;
+  size_t stride;
+  const GAS double* Ic;
+};
+
+__device__ __forceinline__ double adj_g(const jja_adjoint_run& A, const AdjCol& C, int64_t m, int32_t j) {
+  if (!A.g || m < A.g_lo || m >= A.g_hi) return 0.0;
+  const int32_t sl = gbl(A.g_slot)[m - A.slot_t0];
+  return sl >= 0 ? C.g[static_cast<size_t>(sl) * C.stride + j] : 0.0;
+}
+
+// lambda_k += g_k (complete) and u = lambda_k / c0: the first step of a sweep.
+template <int BS>
+__device__ __forceinline__ void adjoint_prime(const DevProblem& P, const jja_adjoint_run& A,
+                                              const AdjCol& C, int64_t k) {
+  const GAS double* __restrict__ c0 = gbl(P.coef);
+  GAS double* __restrict__ lk = C.lam + (k & 1) * C.stride;
+  for (int32_t j = threadIdx.x; j < P.Nj; j += BS) {
+    const double l = lk[j] + adj_g(A, C, k, j);
+    lk[j] = l;
+    C.u[j] = l / c0[j];
+  }
+}
+
+// After the solve of step k (face vector J' = K^-1 A D lambda_k in LDS):
+// mu_k = (A^T J' - lambda_k)/c0 and the updates of lambda_{k-1}, lambda_{k-2}
+// and of the gradients; then lambda_{k-1} gets g_{k-1} and u = lambda_{k-1}/c0
+// prepares step k-1.
+template <int BS>
+__device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja_adjoint_run& A,
+                                                  const AdjCol& C, const double* __restrict__ zs,
+                                                  int64_t k, int64_t w) {
+  const int32_t Nj = P.Nj;
+  const GAS double* __restrict__ coef = gbl(P.coef);
+  const GAS vint2* __restrict__ jcode = gbl(reinterpret_cast<const vint2*>(P.junc_code));
+  GAS double* __restrict__ lk = C.lam + (k & 1) * C.stride;
+  GAS double* __restrict__ lk1 = C.lam + ((k - 1) & 1) * C.stride;
+  const GAS double* __restrict__ h1 = C.hist + static_cast<size_t>(k - 1 - C.hist_m0) * C.stride;
+  const GAS double* __restrict__ h2 = C.hist + static_cast<size_t>(k - 2 - C.hist_m0) * C.stride;
+  GAS double* __restrict__ gIc = gblw(A.grad_Ic) + static_cast<size_t>(w) * Nj;
+  GAS double* __restrict__ gIs = gblw(A.grad_Is) + static_cast<size_t>(w) * Nj;
+  for (int32_t j = threadIdx.x; j < Nj; j += BS) {
+    const vint2 code = jcode[j];
+    const double c0 = coef[j], c1 = coef[static_cast<int64_t>(Nj) + j], c2 = coef[2 * static_cast<int64_t>(Nj) + j];
+    const double lam = lk[j], lprev = lk1[j], t1 = h1[j], t2 = h2[j], Ic = C.Ic[j];
+    const double gj = adj_g(A, C, k - 1, j);
+    const double gi = gIc[j], gs = gIs[j];
+    double y = 0.0;
+    if (code.x >= 0) y = (code.x & 1) ? y - zs[code.x >> 1] : y + zs[code.x >> 1];
+    if (code.y >= 0) y = (code.y & 1) ? y - zs[code.y >> 1] : y + zs[code.y >> 1];
+    const double mu = (y - lam) / c0;
+    double sn, cs;
+    jja_sincos(2.0 * t1 - t2, &sn, &cs);
+    const double a = 2.0 * Ic * cs + c1, b = c2 - Ic * cs;
+    gIc[j] = gi + sn * mu;
+    gIs[j] = gs - mu;
+    lk[j] = b * mu;  // slot k mod 2 now holds lambda_{k-2}'s partial sum
+    const double l1 = (lprev + a * mu) + gj;
+    lk1[j] = l1;
+    C.u[j] = l1 / c0;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restrict__ probs,
+                                                     const jja_adjoint_run* __restrict__ runs,
+                                                     const WorkItem* __restrict__ work, int64_t k_hi,
+                                                     int64_t k_lo) {
+  extern __shared__ double lds[];
+  const WorkItem wi = work[blockIdx.x];
+  const DevProblem& P = probs[wi.problem];
+  const jja_adjoint_run& A = runs[wi.problem];
+  const int64_t w = wi.column;
+  const size_t stride = static_cast<size_t>(A.W) * P.Nj;
+  const size_t col = static_cast<size_t>(w) * P.Nj;
+  AdjCol C;
+  C.lam = gblw(A.lam) + col;
+  C.g = A.g ? gbl(A.g) + col : nullptr;
+  C.u = gblw(A.u) + col;
+  C.hist = gbl(A.hist) + col;
+  C.hist_m0 = A.hist_m0;
+  C.stride = stride;
+  C.Ic = gbl(A.Ic ? A.Ic : P.Ic);
+  GAS double* __restrict__ gf = gblw(A.grad_f) + static_cast<size_t>(w) * P.Nf;
+  if (A.prime) adjoint_prime<BS>(P, A, C, k_hi);
+  __syncthreads();
+  for (int64_t k = k_hi; k >= k_lo; --k) {
+    face_phase<BS>(P, nullptr, C.u, lds, w, k);
+    lds_barrier();
+    solve(P, lds);
+    for (int32_t p = threadIdx.x; p < P.Nf; p += BS) gf[p] = gf[p] - kTwoPi * lds[p];
+    adjoint_junctions<BS>(P, A, C, lds, k, w);
     __syncthreads();
   }
 }
@@ -657,6 +762,7 @@ struct Plan {
   int64_t lds_limit = 0;
   int32_t block_size = 1024;
   int32_t stencil_width = 3;
+  jja_adjoint_run* d_adj = nullptr;  // allocated at the first jja_adjoint call
 };
 
 template <typename T>
@@ -784,6 +890,21 @@ int launch_evolve(Plan* plan, int32_t n_work, int64_t lds, int32_t n_steps, int6
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
   hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
                      plan->d_problems, plan->d_runs, plan->d_work, n_steps, step0);
+  JJA_HIP(hipGetLastError());
+  return JJA_OK;
+}
+
+template <int BS>
+int launch_adjoint(Plan* plan, int32_t n_work, int64_t lds, int64_t k_hi, int64_t k_lo,
+                   hipStream_t stream) {
+  auto* kern = adjoint_kernel<BS>;
+  hipFuncAttributes attr;
+  JJA_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kern)));
+  if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "adjoint kernel has static LDS");
+  JJA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
+                     plan->d_problems, plan->d_adj, plan->d_work, k_hi, k_lo);
   JJA_HIP(hipGetLastError());
   return JJA_OK;
 }
@@ -961,6 +1082,59 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
     default: JJA_LAUNCH(1024);
   }
 #undef JJA_LAUNCH
+}
+
+int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t k_lo, void* stream_ptr) {
+  Plan* plan = reinterpret_cast<Plan*>(p);
+  if (!plan || !runs || k_lo < 0 || k_hi < k_lo) return fail(JJA_EINVAL, "jja_adjoint: invalid arguments");
+  if (plan->stencil_width != 3) return fail(JJA_EUNSUPPORTED, "jja_adjoint: stencil width 3 only");
+  const int32_t np = static_cast<int32_t>(plan->problems.size());
+  std::vector<WorkItem> work;
+  int64_t lds = 0;
+  for (int32_t i = 0; i < np; ++i) {
+    const jja_adjoint_run& r = runs[i];
+    if (r.W <= 0 || !r.hist || !r.lam || !r.u || !r.grad_Ic || !r.grad_Is || !r.grad_f)
+      return fail(JJA_EINVAL, "jja_adjoint: run needs W > 0, hist, lam, u and the gradient arrays");
+    if (r.hist_m0 > k_lo - 2) return fail(JJA_EINVAL, "jja_adjoint: hist must start at k_lo - 2 or earlier");
+    if (r.g && (!r.g_slot || r.g_lo < r.slot_t0 || r.g_hi < r.g_lo))
+      return fail(JJA_EINVAL, "jja_adjoint: g needs g_slot covering [g_lo, g_hi)");
+    if (plan->lds_bytes[i] > plan->lds_limit)
+      return fail(JJA_EUNSUPPORTED, "jja_adjoint: face vector of problem " + std::to_string(i) + " exceeds LDS");
+    lds = std::max(lds, plan->lds_bytes[i]);
+    for (int32_t w = 0; w < r.W; ++w) work.push_back({i, w});
+  }
+  if (!plan->d_adj) {
+    void* d = nullptr;
+    JJA_HIP(hipMalloc(&d, sizeof(jja_adjoint_run) * np));
+    plan->allocations.push_back(d);
+    plan->d_adj = static_cast<jja_adjoint_run*>(d);
+  }
+  if (work.size() > plan->work_cap) {
+    if (plan->d_work) {
+      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_work));
+      if (it != plan->allocations.end()) plan->allocations.erase(it);
+      JJA_HIP(hipFree(plan->d_work));
+      plan->d_work = nullptr;
+    }
+    void* d = nullptr;
+    JJA_HIP(hipMalloc(&d, work.size() * sizeof(WorkItem)));
+    plan->allocations.push_back(d);
+    plan->d_work = static_cast<WorkItem*>(d);
+    plan->work_cap = work.size();
+  }
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  JJA_HIP(hipMemcpyAsync(plan->d_adj, runs, sizeof(jja_adjoint_run) * np, hipMemcpyHostToDevice, stream));
+  JJA_HIP(hipMemcpyAsync(plan->d_work, work.data(), sizeof(WorkItem) * work.size(),
+                         hipMemcpyHostToDevice, stream));
+  const int32_t n_work = static_cast<int32_t>(work.size());
+  if (lds < 8) lds = 8;
+  switch (plan->block_size) {
+    case 64: return launch_adjoint<64>(plan, n_work, lds, k_hi, k_lo, stream);
+    case 128: return launch_adjoint<128>(plan, n_work, lds, k_hi, k_lo, stream);
+    case 256: return launch_adjoint<256>(plan, n_work, lds, k_hi, k_lo, stream);
+    case 512: return launch_adjoint<512>(plan, n_work, lds, k_hi, k_lo, stream);
+    default: return launch_adjoint<1024>(plan, n_work, lds, k_hi, k_lo, stream);
+  }
 }
 
 int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_t* n_out,

@@ -68,4 +68,25 @@ JJA_HD double jja_sin(double x) {
   return q >= 2.0 ? -v : v;
 }
 
+// sin(x) and cos(x) with one shared argument reduction (same method and
+// accuracy as jja_sin; cos(x) = sin(x + pi/2) is quadrant n + 1).
+JJA_HD void jja_sincos(double x, double* sn, double* cs) {
+  const double kTwoOverPi = 6.36619772367581382433e-01;
+  const double P1 = 1.57079632679489655800e+00;
+  const double P2 = 6.12323399573676603587e-17;
+  const double n = rint(x * kTwoOverPi);
+  const double hi = n * P1;
+  const double lo = fma(n, P1, -hi);
+  const double rh = x - hi;
+  const double s = fma(-n, P2, -lo);
+  const double r = rh + s;
+  const double rl = (rh - r) + s;
+  const double q = n - 4.0 * floor(n * 0.25);
+  const double ks = jja_ksin(r, rl), kc = jja_kcos(r, rl);
+  const double vs = (q == 0.0 || q == 2.0) ? ks : kc;
+  const double vc = (q == 0.0 || q == 2.0) ? kc : ks;
+  *sn = q >= 2.0 ? -vs : vs;
+  *cs = (q == 1.0 || q == 2.0) ? -vc : vc;
+}
+
 #endif  // JJA_MATH_H

@@ -47,7 +47,17 @@ int main(void) {
     }
   }
   int specials = jja_sin(0.0) == 0.0 && jja_sin(NAN) != jja_sin(NAN) && jja_sin(INFINITY) != jja_sin(INFINITY);
-  printf("%%lld %%.6g %%d\n", (long long)worst, abserr, specials);
+  int64_t worst_sc = 0;
+  for (long i = 0; i < 400000; ++i) {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    double x = (2.0 * ((double)(s >> 11) * 0x1p-53) - 1.0) * 1e4, sn, cs;
+    jja_sincos(x, &sn, &cs);
+    int64_t d1 = ulps(sn, sin(x)), d2 = fabs(cos(x)) > 1e-3 ? ulps(cs, cos(x)) : 0;
+    if (fabs(cs - cos(x)) > 1e-15) d2 = 1000;
+    if (d1 > worst_sc) worst_sc = d1;
+    if (d2 > worst_sc) worst_sc = d2;
+  }
+  printf("%%lld %%.6g %%d %%lld\n", (long long)worst, abserr, specials, (long long)worst_sc);
   return 0;
 }
 """
@@ -60,7 +70,8 @@ def test_kernel_sine_matches_libm(tmp_path):
     exe = tmp_path / "t"
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    worst_ulp, abserr, specials = int(out[0]), float(out[1]), int(out[2])
+    worst_ulp, abserr, specials, worst_sc = int(out[0]), float(out[1]), int(out[2]), int(out[3])
     assert worst_ulp <= 1, f"max error {worst_ulp} ulp"
+    assert worst_sc <= 1, f"sincos: max error {worst_sc} ulp"
     assert abserr < 1e-24, f"absolute error near k*pi/2: {abserr}"
     assert specials == 1
