@@ -73,10 +73,10 @@ const char* jja_factor_last_error(void);
  *     Face codes pad with -1.
  * Solve records (one wavefront of 64 lanes each): every record holds 8 entries
  *     per lane, zero-padded. Value `it` of lane l at base + (it/2)*128 + 2*l +
- *     it%2 (16-byte loads); LDS index `it` of lane l at ibase + 8*l + it (one
- *     16-byte load per lane); the row of lane l at slice_rows[rowbase + l/G].
- *     An index addresses the workgroup's LDS in doubles: z at its position,
- *     the stage scratch t at n_faces + stage row. Wave w of the workgroup runs records solve_wave_ptr[w] ..
+ *     it%2 (16-byte loads); index `it` of lane l at ibase + 8*l + it; the row
+ *     of lane l at slice_rows[rowbase + l/G]. An index addresses the column's
+ *     face store in doubles: z at its position, the stage scratch t at
+ *     n_faces + stage row. Wave w of the workgroup runs records solve_wave_ptr[w] ..
  *     solve_wave_ptr[w+1]-5: a multiple of three (a record with G 1 and no rows
  *     is a no-op), followed by 4 records that it loads ahead but never executes.
  *     Every wave's program has the same number of BARRIER records.
@@ -131,13 +131,20 @@ typedef struct jja_problem_desc {
   int64_t n_entries;
   const double* ent_val;         /* [n_entries] */
   int64_t n_index;
-  const uint16_t* ent_off;       /* [n_index] LDS indices (doubles) */
+  const void* ent_off;           /* [n_index] face-store indices (doubles), index_bytes wide */
   int32_t max_stage_rows;        /* size of the stage scratch t */
   /* the first stage's rows have no off-block entries, so its scratch equals b:
    * the face phase writes b[p] also to t[p] for positions p < face_copy_rows,
    * and the program has no records for that phase */
   int32_t face_copy_rows;
   const int32_t* solve_wave_ptr; /* [block_size/64 + 1] first record of each wave's program */
+  /* 2: the face vector and stage scratch live in the workgroup's LDS (indices are
+   *    uint16 absolute LDS addresses in doubles);
+   * 4: they live in a per-column global array (jja_problem_run.zbuf, uint32
+   *    indices; each lane's 8 indices are two 16-byte loads), for circuits whose
+   *    face vector does not fit in LDS. */
+  int32_t index_bytes;
+  int32_t pad_;
 } jja_problem_desc;
 
 /*
@@ -150,6 +157,8 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* problems, jja_pl
 int jja_plan_destroy(jja_plan* plan);
 /* Dynamic LDS bytes one column of problem p needs (0 if it cannot run). */
 int64_t jja_plan_lds_bytes(const jja_plan* plan, int32_t p);
+/* Doubles per column of the global face store problem p needs (0: it uses LDS). */
+int64_t jja_plan_zbuf_doubles(const jja_plan* plan, int32_t p);
 
 /* ---------------------------------------------------------------- evolve */
 
@@ -192,6 +201,8 @@ typedef struct jja_problem_run {
   double* th_out;        /* [slots][W][Nj] */
   double* I_out;         /* [slots][W][Nj] */
   const double* Ic;      /* [Nj] critical currents of this run, or NULL: the plan's */
+  double* zbuf;          /* [W][n_faces + max_stage_rows] face store when the problem was
+                            planned with index_bytes 4 (see jja_plan_zbuf_doubles), else NULL */
 } jja_problem_run;
 
 /*
@@ -236,6 +247,7 @@ typedef struct jja_adjoint_run {
   double* grad_Ic;        /* [W][Nj] += sum_k sin(arg_k) mu_k */
   double* grad_Is;        /* [W][Nj] += sum_k -mu_k */
   double* grad_f;         /* [W][Nf] by solve position: += sum_k -2 pi K^-1 A D lambda_k */
+  double* zbuf;           /* face store as in jja_problem_run */
 } jja_adjoint_run;
 
 /*

@@ -45,9 +45,9 @@ class ProblemDesc(C.Structure):
         ("n_face_entries", C.c_int64), ("face_code", i32p),
         ("n_slice_rows", C.c_int64), ("slice_rows", i32p),
         ("n_entries", C.c_int64), ("ent_val", f64p), ("n_index", C.c_int64),
-        ("ent_off", C.POINTER(C.c_uint16)),
+        ("ent_off", C.c_void_p),
         ("max_stage_rows", C.c_int32), ("face_copy_rows", C.c_int32),
-        ("solve_wave_ptr", i32p),
+        ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -66,7 +66,7 @@ class ProblemRun(C.Structure):
         ("Is", Source), ("Vs", Source), ("f", Source), ("T", Source), ("sigma", Source),
         ("noise", C.c_void_p), ("noise_t0", C.c_int64), ("seed", C.c_uint64),
         ("th_slot", C.c_void_p), ("I_slot", C.c_void_p), ("slot_t0", C.c_int64),
-        ("th_out", C.c_void_p), ("I_out", C.c_void_p), ("Ic", C.c_void_p),
+        ("th_out", C.c_void_p), ("I_out", C.c_void_p), ("Ic", C.c_void_p), ("zbuf", C.c_void_p),
     ]
 
 
@@ -76,6 +76,7 @@ class AdjointRun(C.Structure):
         ("hist_m0", C.c_int64), ("lam", C.c_void_p), ("u", C.c_void_p), ("g", C.c_void_p),
         ("g_slot", C.c_void_p), ("slot_t0", C.c_int64), ("g_lo", C.c_int64), ("g_hi", C.c_int64),
         ("grad_Ic", C.c_void_p), ("grad_Is", C.c_void_p), ("grad_f", C.c_void_p),
+        ("zbuf", C.c_void_p),
     ]
 
 
@@ -86,7 +87,8 @@ class NativeError(RuntimeError):
 _lib = None
 
 EXPORTS = ["jja_cholesky", "jja_factor_get", "jja_factor_free", "jja_factor_last_error",
-           "jja_plan_validate", "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes", "jja_evolve",
+           "jja_plan_validate", "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes",
+           "jja_plan_zbuf_doubles", "jja_evolve",
            "jja_vortex", "jja_adjoint", "jja_last_error"]
 
 
@@ -108,6 +110,8 @@ def lib():
     L.jja_plan_destroy.argtypes = [C.c_void_p]
     L.jja_plan_lds_bytes.argtypes = [C.c_void_p, C.c_int32]
     L.jja_plan_lds_bytes.restype = C.c_int64
+    L.jja_plan_zbuf_doubles.argtypes = [C.c_void_p, C.c_int32]
+    L.jja_plan_zbuf_doubles.restype = C.c_int64
     L.jja_evolve.argtypes = [C.c_void_p, C.POINTER(ProblemRun), C.c_int32, C.c_int64, C.c_void_p]
     L.jja_vortex.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     L.jja_adjoint.argtypes = [C.c_void_p, C.POINTER(AdjointRun), C.c_int64, C.c_int64, C.c_void_p]

@@ -64,7 +64,8 @@ class DifferentiableEvolution:
 
     def __init__(self, circuit, time_step=0.05, time_step_count=1000, problem_count=1,
                  temperature=0.0, store_time_steps=None, config_at_minus_1=None,
-                 config_at_minus_2=None, seed=None, checkpoint_every=200, block_size=None):
+                 config_at_minus_2=None, seed=None, checkpoint_every=200, block_size=None,
+                 face_store="auto"):
         torch = self.torch = _torch()
         W, Nj, Nf = int(problem_count), circuit._Nj(), circuit._Nf()
         self.W, self.Nj, self.Nf, self.Nt = W, Nj, Nf, int(time_step_count)
@@ -78,7 +79,7 @@ class DifferentiableEvolution:
             temperature=temperature, store_time_steps=store_time_steps, store_theta=True,
             store_voltage=False, store_current=False, config_at_minus_1=config_at_minus_1,
             config_at_minus_2=config_at_minus_2, seed=seed if seed is not None else 0,
-            steps_per_launch=self.C, block_size=block_size)
+            steps_per_launch=self.C, block_size=block_size, face_store=face_store)
         self.problem = prob
         self.th_mask = prob.store_time_steps.copy()
         self.eng = Engine(prob, self.th_mask, np.zeros(self.Nt, dtype=bool))
@@ -148,6 +149,7 @@ class DifferentiableEvolution:
             adj.g_slot = eng.th_slot.data_ptr()
             adj.slot_t0, adj.g_lo, adj.g_hi = 0, 0, Nt
         adj.grad_Ic, adj.grad_Is, adj.grad_f = gIc.data_ptr(), gIs.data_ptr(), gf.data_ptr()
+        adj.zbuf = eng.zbuf.data_ptr() if eng.zbuf is not None else None
         try:
             for seg in reversed(range(self.n_seg)):
                 s0, s1 = seg * C_, min((seg + 1) * C_, Nt)

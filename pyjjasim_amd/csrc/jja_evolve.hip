@@ -92,6 +92,7 @@ __device__ __forceinline__ jja_slice load_rec(const jja_slice* prog, int32_t k) 
 struct DevProblem {
   int32_t Nj, Nf, sw, BS;
   int32_t n_face_slices, n_solve_slices, max_stage_rows, copy_rows;
+  int32_t index_bytes, pad_;
   const double* Ic;
   const double* Rv;
   const double* coef;
@@ -102,7 +103,7 @@ struct DevProblem {
   const int32_t* face_code;
   const int32_t* slice_rows;
   const double* ent_val;
-  const uint16_t* ent_off;
+  const void* ent_off;      // LDS indices: uint16 (face store in LDS) or uint32 (in global memory)
   const int32_t* wave_ptr;  // [BS/64 + 1]
   const double* zero;       // 8 zero doubles: absent input terms read here
 };
@@ -180,21 +181,62 @@ __device__ __forceinline__ double group_sum_last(double v, int G) {
   return v;
 }
 
-// Entries of one solve record for one lane, held in registers.
+// Where a column keeps its face vector z (by solve position) and the stage
+// scratch t after it. FaceStore<false>: the workgroup's LDS, addressed by the
+// plan's 16-bit indices as absolute LDS addresses (the evolve and adjoint
+// kernels have no static LDS, checked on the host before every launch).
+// FaceStore<true>: a per-column array in global memory for circuits whose face
+// vector does not fit in LDS, addressed by 32-bit indices. The arithmetic on
+// the values is the same in both.
+template <bool GLOBAL>
+struct FaceStore;
+
+template <>
+struct FaceStore<false> {
+  double* p;
+  __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
+  __device__ __forceinline__ double gather(uint32_t i) const {
+    return *reinterpret_cast<const LAS double*>(static_cast<uintptr_t>(i) << 3);
+  }
+  __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
+  // orders LDS traffic only: global loads in flight stay in flight
+  __device__ __forceinline__ static void barrier() { lds_barrier(); }
+};
+
+template <>
+struct FaceStore<true> {
+  GAS double* p;
+  __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
+  __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
+  __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
+  __device__ __forceinline__ static void barrier() { __syncthreads(); }
+};
+
+// Entries of one solve record for one lane, held in registers: 8 values and
+// their indices (two 16-bit halves per word, or one 32-bit index per word).
+template <bool GLOBAL>
 struct SliceRegs {
   double v[kMaxIter];
-  uint32_t idx[kMaxIter / 2];  // LDS indices (doubles), two 16-bit halves each
+  uint32_t idx[GLOBAL ? kMaxIter : kMaxIter / 2];
   int32_t row;
 };
 
-// Issue the loads of one solve record: always the same 6 vector loads, each a
+template <bool GLOBAL>
+__device__ __forceinline__ uint32_t slice_index(const SliceRegs<GLOBAL>& r, int i) {
+  if constexpr (GLOBAL)
+    return r.idx[i];
+  else
+    return (r.idx[i / 2] >> (16 * (i % 2))) & 0xffffu;
+}
+
+// Issue the loads of one solve record: always the same vector loads, each a
 // scalar base (from the record) plus a per-lane offset that does not change
 // between records, so no address arithmetic runs per record.
+template <bool GLOBAL>
 __device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice& m, int lane,
-                                           SliceRegs& r) {
+                                           SliceRegs<GLOBAL>& r) {
   const int lg = __builtin_ctz(static_cast<unsigned>(m.flags & 0xff));
   const GAS vdouble2* __restrict__ vp = gbl(reinterpret_cast<const vdouble2*>(P.ent_val + m.base)) + lane;
-  const GAS vuint4* __restrict__ op = gbl(reinterpret_cast<const vuint4*>(P.ent_off + m.ibase)) + lane;
   // the raw row; whether this lane's slot holds one is decided when the record
   // is processed (a select here would wait for the load)
   r.row = gbl(P.slice_rows)[m.rowbase + (lane >> lg)];
@@ -204,26 +246,33 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice&
     r.v[2 * q] = d.x;
     r.v[2 * q + 1] = d.y;
   }
-  const vuint4 o = *op;
-  r.idx[0] = o.x;
-  r.idx[1] = o.y;
-  r.idx[2] = o.z;
-  r.idx[3] = o.w;
-}
-
-// LDS read of double number `i`. The evolve kernel has no static LDS
-// (checked on the host before every launch), so its dynamic LDS array starts
-// at address 0 and the plan's indices address it directly.
-__device__ __forceinline__ double lds_abs(uint32_t i) {
-  return *reinterpret_cast<const LAS double*>(static_cast<uintptr_t>(i) << 3);
+  if constexpr (GLOBAL) {
+    const GAS vuint4* __restrict__ op =
+        gbl(reinterpret_cast<const vuint4*>(static_cast<const uint32_t*>(P.ent_off) + m.ibase)) + 2 * lane;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const vuint4 o = op[q];
+      r.idx[4 * q] = o.x;
+      r.idx[4 * q + 1] = o.y;
+      r.idx[4 * q + 2] = o.z;
+      r.idx[4 * q + 3] = o.w;
+    }
+  } else {
+    const vuint4 o = *(gbl(reinterpret_cast<const vuint4*>(static_cast<const uint16_t*>(P.ent_off) + m.ibase)) + lane);
+    r.idx[0] = o.x;
+    r.idx[1] = o.y;
+    r.idx[2] = o.z;
+    r.idx[3] = o.w;
+  }
 }
 
 // Gather, accumulate, and (flag FINISH) reduce and write one record's rows.
-__device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t Nf, int lane,
-                                              const jja_slice& m, const SliceRegs& r, double& acc) {
+template <bool GLOBAL>
+__device__ __forceinline__ void process_slice(const FaceStore<GLOBAL>& zs, int32_t Nf, int lane,
+                                              const jja_slice& m, const SliceRegs<GLOBAL>& r, double& acc) {
   double zv[kMaxIter];
 #pragma unroll
-  for (int i = 0; i < kMaxIter; ++i) zv[i] = lds_abs((r.idx[i / 2] >> (16 * (i % 2))) & 0xffffu);
+  for (int i = 0; i < kMaxIter; ++i) zv[i] = zs.gather(slice_index(r, i));
 #pragma unroll
   for (int i = 0; i < kMaxIter; ++i) acc = __builtin_fma(r.v[i], zv[i], acc);
   if (m.flags & JJA_SLICE_FINISH) {
@@ -232,13 +281,13 @@ __device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t 
     const int slot = lane >> __builtin_ctz(static_cast<unsigned>(G));
     if (slot < (m.flags >> 16) && r.row >= 0 && (lane & (G - 1)) == G - 1) {
       if (m.flags & JJA_SLICE_BLOCK)
-        lds[r.row] = acc;
+        zs.wr(r.row, acc);
       else
-        lds[Nf + r.row - m.r0] = lds[r.row] - acc;
+        zs.wr(Nf + r.row - m.r0, zs.rd(r.row) - acc);
     }
     acc = 0.0;
   }
-  if (m.flags & JJA_SLICE_BARRIER) lds_barrier();
+  if (m.flags & JJA_SLICE_BARRIER) FaceStore<GLOBAL>::barrier();
 }
 
 // The whole forward + backward solve. Each wave walks its own record program,
@@ -250,7 +299,8 @@ __device__ __forceinline__ void process_slice(double* __restrict__ lds, int32_t 
 // of three and four never-executed records follow it, so the loop has a single
 // exit and every lookahead load stays inside the program. Waves meet only at
 // BARRIER records (one per phase, the same number in every wave).
-__device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ lds) {
+template <bool GLOBAL>
+__device__ __forceinline__ void solve(const DevProblem& P, const FaceStore<GLOBAL>& zs) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
   const CAS int32_t* wp = (const CAS int32_t*)(P.wave_ptr) + wave;
@@ -258,7 +308,7 @@ __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ 
   const int32_t nk = __builtin_amdgcn_readfirstlane(wp[1]) - k0 - 4;
   const jja_slice* __restrict__ prog = P.solve_slices + k0;
   jja_slice m0 = load_rec(prog, 0), m1 = load_rec(prog, 1), m2 = load_rec(prog, 2);
-  SliceRegs A, B, C;
+  SliceRegs<GLOBAL> A, B, C;
   load_slice(P, m0, lane, A);
   load_slice(P, m1, lane, B);
   double acc = 0.0;
@@ -266,15 +316,15 @@ __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ 
     const jja_slice m3 = load_rec(prog, k + 3);
     load_slice(P, m2, lane, C);
     __builtin_amdgcn_sched_barrier(0);
-    process_slice(lds, P.Nf, lane, m0, A, acc);
+    process_slice(zs, P.Nf, lane, m0, A, acc);
     const jja_slice m4 = load_rec(prog, k + 4);
     load_slice(P, m3, lane, A);
     __builtin_amdgcn_sched_barrier(0);
-    process_slice(lds, P.Nf, lane, m1, B, acc);
+    process_slice(zs, P.Nf, lane, m1, B, acc);
     const jja_slice m5 = load_rec(prog, k + 5);
     load_slice(P, m4, lane, B);
     __builtin_amdgcn_sched_barrier(0);
-    process_slice(lds, P.Nf, lane, m2, C, acc);
+    process_slice(zs, P.Nf, lane, m2, C, acc);
     m0 = m3;
     m1 = m4;
     m2 = m5;
@@ -284,9 +334,9 @@ __device__ __forceinline__ void solve(const DevProblem& P, double* __restrict__ 
 // Face phase: b_p = sum +-u_j in ascending j, minus 2 pi f (time_evolution.py:560-569).
 // A face longer than one record continues in the next record (flag FINISH on
 // its last one).
-template <int BS>
+template <int BS, class Z>
 __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f,
-                                           const GAS double* __restrict__ u, double* __restrict__ zs,
+                                           const GAS double* __restrict__ u, const Z& zs,
                                            int64_t w, int64_t t) {
   const int tid = threadIdx.x;
   double acc = 0.0;
@@ -307,8 +357,8 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source
       const int32_t pos = gbl(P.slice_rows)[m.rowbase + tid];
       if (pos >= 0) {
         if (f && !f->zero) acc = acc - kTwoPi * src_at(*f, gbl(P.face_of_pos)[pos], w, t);
-        zs[pos] = acc;
-        if (pos < P.copy_rows) zs[P.Nf + pos] = acc;  // first stage: scratch = b
+        zs.wr(pos, acc);
+        if (pos < P.copy_rows) zs.wr(P.Nf + pos, acc);  // first stage: scratch = b
       }
       acc = 0.0;
     }
@@ -367,9 +417,9 @@ struct JunctionStep {
 // outputs, theta_s += Vs dt; with PREP form x of step tx (time_evolution.py:
 // 533-558) and write x to the history slot and u = x/c0 - theta_s to the
 // scratch. All loads are issued before any arithmetic.
-template <int SW, bool UPDATE, bool PREP>
+template <int SW, bool UPDATE, bool PREP, class Z>
 __device__ __forceinline__ void junction_one(const DevProblem& P, const JunctionStep& J,
-                                             const double* __restrict__ zs, int32_t j, double zj) {
+                                             const Z& zs, int32_t j, double zj) {
   const int32_t Nj = P.Nj;
   const GAS double* __restrict__ coef = gbl(P.coef);
   const double c0 = coef[j];
@@ -396,8 +446,8 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
   }
   if (UPDATE) {
     double y = 0.0;
-    if (code.x >= 0) y = (code.x & 1) ? y - zs[code.x >> 1] : y + zs[code.x >> 1];
-    if (code.y >= 0) y = (code.y & 1) ? y - zs[code.y >> 1] : y + zs[code.y >> 1];
+    if (code.x >= 0) y = (code.x & 1) ? y - zs.rd(code.x >> 1) : y + zs.rd(code.x >> 1);
+    if (code.y >= 0) y = (code.y & 1) ? y - zs.rd(code.y >> 1) : y + zs.rd(code.y >> 1);
     t[0] = (y - xo) / c0;
     J.th_n[j] = t[0];
     if (J.th_out) J.th_out[j] = t[0];
@@ -461,9 +511,8 @@ __device__ __forceinline__ void wave_normals(const JunctionStep& J, int32_t k, d
 
 // The junction pass, general form: thread tid takes junctions k*BS + tid
 // (consecutive lanes, consecutive junctions), two iterations at a time.
-template <int BS, int SW, bool UPDATE, bool PREP>
-__device__ __forceinline__ void junction_loop(const DevProblem& P, const JunctionStep& J,
-                                              const double* __restrict__ zs) {
+template <int BS, int SW, bool UPDATE, bool PREP, class Z>
+__device__ __forceinline__ void junction_loop(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
   const int32_t tid = threadIdx.x;
   const bool gauss = PREP && !J.t_zero && !J.noise;
@@ -471,11 +520,11 @@ __device__ __forceinline__ void junction_loop(const DevProblem& P, const Junctio
     double zA = 0.0, zB = 0.0;
     if (gauss) wave_normals<BS>(J, k, zA, zB);
     const int32_t jA = k * BS + tid, jB = jA + BS;
-    if (jA < Nj) junction_one<SW, UPDATE, PREP>(P, J, zs, jA, zA);
+    if (jA < Nj) junction_one<SW, UPDATE, PREP, Z>(P, J, zs, jA, zA);
     // keep the two junctions apart: interleaving them would double the live
     // registers and spill
     __builtin_amdgcn_sched_barrier(0);
-    if (jB < Nj) junction_one<SW, UPDATE, PREP>(P, J, zs, jB, zB);
+    if (jB < Nj) junction_one<SW, UPDATE, PREP, Z>(P, J, zs, jB, zB);
   }
 }
 
@@ -505,11 +554,12 @@ __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionSt
   L.sg = term_at(J.sig_x, j);
 }
 
-__device__ __forceinline__ void jfast_compute(const JunctionStep& J, const double* __restrict__ zs,
-                                              int32_t j, const JFast& L, double z) {
+template <class Z>
+__device__ __forceinline__ void jfast_compute(const JunctionStep& J, const Z& zs, int32_t j,
+                                              const JFast& L, double z) {
   double y = 0.0;
-  if (L.code.x >= 0) y = (L.code.x & 1) ? y - zs[L.code.x >> 1] : y + zs[L.code.x >> 1];
-  if (L.code.y >= 0) y = (L.code.y & 1) ? y - zs[L.code.y >> 1] : y + zs[L.code.y >> 1];
+  if (L.code.x >= 0) y = (L.code.x & 1) ? y - zs.rd(L.code.x >> 1) : y + zs.rd(L.code.x >> 1);
+  if (L.code.y >= 0) y = (L.code.y & 1) ? y - zs.rd(L.code.y >> 1) : y + zs.rd(L.code.y >> 1);
   const double t1 = (y - L.xo) / L.c0;
   J.th_n[j] = t1;
   const double X = L.Ic * jja_sin(2.0 * t1 - L.t2) + L.c1 * t1 + L.c2 * L.t2;
@@ -520,9 +570,8 @@ __device__ __forceinline__ void jfast_compute(const JunctionStep& J, const doubl
   J.u[j] = x / L.c0;
 }
 
-template <int BS>
-__device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J,
-                                                   const double* __restrict__ zs) {
+template <int BS, class Z>
+__device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
   const int32_t tid = threadIdx.x;
   const int32_t last = Nj - 1;
@@ -544,10 +593,10 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
 
 // Set up and run one junction pass (finish step t if `update`, prepare step tx
 // if `prep`) for column w.
-template <int BS, int SW>
+template <int BS, int SW, class Z>
 __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_problem_run& R,
                                               GAS double* __restrict__ th, GAS double* __restrict__ ths,
-                                              GAS double* __restrict__ u, const double* __restrict__ zs,
+                                              GAS double* __restrict__ u, const Z& zs,
                                               int64_t w, size_t slot_stride, int64_t t, bool update,
                                               int64_t tx, bool prep, uint32_t noise_col) {
   constexpr int32_t S = SW - 1;
@@ -583,13 +632,13 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   J.noise_col = noise_col;
   J.tx = static_cast<uint64_t>(tx);
   const bool fast = SW == 3 && update && prep && !ths && !J.th_out && !J.I_out && !J.noise && !J.from_T;
-  if (fast) junction_loop_fast<BS>(P, J, zs);
-  else if (update && prep) junction_loop<BS, SW, true, true>(P, J, zs);
-  else if (update) junction_loop<BS, SW, true, false>(P, J, zs);
-  else if (prep) junction_loop<BS, SW, false, true>(P, J, zs);
+  if (fast) junction_loop_fast<BS, Z>(P, J, zs);
+  else if (update && prep) junction_loop<BS, SW, true, true, Z>(P, J, zs);
+  else if (update) junction_loop<BS, SW, true, false, Z>(P, J, zs);
+  else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
 }
 
-template <int BS, int SW>
+template <int BS, int SW, bool GLOBAL>
 __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict__ probs,
                                                     const jja_problem_run* __restrict__ runs,
                                                     const WorkItem* __restrict__ work,
@@ -599,7 +648,12 @@ __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict
   const DevProblem& P = probs[wi.problem];
   const jja_problem_run& R = runs[wi.problem];
   const int64_t w = wi.column;
-  double* __restrict__ zs = lds;  // face vector by solve position; stage scratch after it
+  // face vector by solve position, stage scratch after it
+  FaceStore<GLOBAL> zs;
+  if constexpr (GLOBAL)
+    zs.p = gblw(R.zbuf) + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
+  else
+    zs.p = lds;
   const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
   GAS double* __restrict__ th = gblw(R.theta) + static_cast<size_t>(w) * P.Nj;
   GAS double* __restrict__ ths = R.theta_s ? gblw(R.theta_s) + static_cast<size_t>(w) * P.Nj : nullptr;
@@ -611,8 +665,8 @@ __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
     face_phase<BS>(P, &R.f, u, zs, w, t);
-    lds_barrier();
-    solve(P, lds);
+    FaceStore<GLOBAL>::barrier();
+    solve(P, zs);
     // finish step t and prepare step t+1 in one pass
     junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     __syncthreads();
@@ -654,10 +708,9 @@ __device__ __forceinline__ void adjoint_prime(const DevProblem& P, const jja_adj
 // mu_k = (A^T J' - lambda_k)/c0 and the updates of lambda_{k-1}, lambda_{k-2}
 // and of the gradients; then lambda_{k-1} gets g_{k-1} and u = lambda_{k-1}/c0
 // prepares step k-1.
-template <int BS>
+template <int BS, class Z>
 __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja_adjoint_run& A,
-                                                  const AdjCol& C, const double* __restrict__ zs,
-                                                  int64_t k, int64_t w) {
+                                                  const AdjCol& C, const Z& zs, int64_t k, int64_t w) {
   const int32_t Nj = P.Nj;
   const GAS double* __restrict__ coef = gbl(P.coef);
   const GAS vint2* __restrict__ jcode = gbl(reinterpret_cast<const vint2*>(P.junc_code));
@@ -674,8 +727,8 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
     const double gj = adj_g(A, C, k - 1, j);
     const double gi = gIc[j], gs = gIs[j];
     double y = 0.0;
-    if (code.x >= 0) y = (code.x & 1) ? y - zs[code.x >> 1] : y + zs[code.x >> 1];
-    if (code.y >= 0) y = (code.y & 1) ? y - zs[code.y >> 1] : y + zs[code.y >> 1];
+    if (code.x >= 0) y = (code.x & 1) ? y - zs.rd(code.x >> 1) : y + zs.rd(code.x >> 1);
+    if (code.y >= 0) y = (code.y & 1) ? y - zs.rd(code.y >> 1) : y + zs.rd(code.y >> 1);
     const double mu = (y - lam) / c0;
     double sn, cs;
     jja_sincos(2.0 * t1 - t2, &sn, &cs);
@@ -689,7 +742,7 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
   }
 }
 
-template <int BS>
+template <int BS, bool GLOBAL>
 __global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restrict__ probs,
                                                      const jja_adjoint_run* __restrict__ runs,
                                                      const WorkItem* __restrict__ work, int64_t k_hi,
@@ -710,14 +763,19 @@ __global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restric
   C.stride = stride;
   C.Ic = gbl(A.Ic ? A.Ic : P.Ic);
   GAS double* __restrict__ gf = gblw(A.grad_f) + static_cast<size_t>(w) * P.Nf;
+  FaceStore<GLOBAL> zs;
+  if constexpr (GLOBAL)
+    zs.p = gblw(A.zbuf) + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
+  else
+    zs.p = lds;
   if (A.prime) adjoint_prime<BS>(P, A, C, k_hi);
   __syncthreads();
   for (int64_t k = k_hi; k >= k_lo; --k) {
-    face_phase<BS>(P, nullptr, C.u, lds, w, k);
-    lds_barrier();
-    solve(P, lds);
-    for (int32_t p = threadIdx.x; p < P.Nf; p += BS) gf[p] = gf[p] - kTwoPi * lds[p];
-    adjoint_junctions<BS>(P, A, C, lds, k, w);
+    face_phase<BS>(P, nullptr, C.u, zs, w, k);
+    FaceStore<GLOBAL>::barrier();
+    solve(P, zs);
+    for (int32_t p = threadIdx.x; p < P.Nf; p += BS) gf[p] = gf[p] - kTwoPi * zs.rd(p);
+    adjoint_junctions<BS>(P, A, C, zs, k, w);
     __syncthreads();
   }
 }
@@ -754,7 +812,8 @@ struct Plan {
   int device = 0;
   std::vector<void*> allocations;
   std::vector<DevProblem> problems;
-  std::vector<int64_t> lds_bytes;
+  std::vector<int64_t> lds_bytes;     // per problem; 0 for a global face store
+  std::vector<int64_t> zbuf_doubles;  // per problem; 0 for an LDS face store
   DevProblem* d_problems = nullptr;
   jja_problem_run* d_runs = nullptr;
   WorkItem* d_work = nullptr;
@@ -802,7 +861,10 @@ int validate(const jja_problem_desc& d) {
   if (!d.face_of_pos || !d.face_code || !d.face_slices || !d.slice_rows ||
       (d.n_solve_slices && (!d.solve_slices || !d.ent_val || !d.ent_off)))
     return fail(JJA_EINVAL, "jja_plan_create: missing face or solve arrays");
-  if (static_cast<int64_t>(Nf) + ms > 65536) return fail(JJA_EINVAL, "face vector exceeds 16-bit indices");
+  if (d.index_bytes != 2 && d.index_bytes != 4) return fail(JJA_EINVAL, "index_bytes must be 2 or 4");
+  if (d.index_bytes == 2 && static_cast<int64_t>(Nf) + ms > 65536)
+    return fail(JJA_EINVAL, "face vector exceeds 16-bit indices");
+  if (static_cast<int64_t>(Nf) + ms > INT32_MAX) return fail(JJA_EINVAL, "face vector too large");
   std::vector<char> seen(Nf, 0);
   for (int32_t q = 0; q < Nf; ++q) {
     const int32_t f = d.face_of_pos[q];
@@ -856,7 +918,7 @@ int validate(const jja_problem_desc& d) {
       if (m.n_iter < 0 || m.n_iter > kMaxIter || rows < 0 || rows > kWave / G || (m.flags & 0xf800))
         return fail(JJA_EINVAL, "solve record fields out of range");
       if (m.base < 0 || (m.base % 2) || m.base + kMaxIter * kWave > d.n_entries || m.ibase < 0 ||
-          (m.ibase % 8) || m.ibase + kMaxIter * kWave > d.n_index || m.rowbase < 0 ||
+          (m.ibase % (16 / d.index_bytes)) || m.ibase + kMaxIter * kWave > d.n_index || m.rowbase < 0 ||
           m.rowbase + kWave / G > d.n_slice_rows)
         return fail(JJA_EINVAL, "solve record data out of range or misaligned");
       if (k >= wp[w + 1] - 4) continue;  // lookahead only: loaded, never executed
@@ -868,8 +930,11 @@ int validate(const jja_problem_desc& d) {
         if (r < 0 || r >= Nf || (!block && (r - m.r0 < 0 || r - m.r0 >= ms)))
           return fail(JJA_EINVAL, "solve slice row out of range");
       }
-      for (int64_t e = m.ibase; e < m.ibase + kMaxIter * kWave; ++e)
-        if (d.ent_off[e] >= lds_end) return fail(JJA_EINVAL, "solve LDS offset out of range");
+      for (int64_t e = m.ibase; e < m.ibase + kMaxIter * kWave; ++e) {
+        const int64_t ix = d.index_bytes == 2 ? static_cast<const uint16_t*>(d.ent_off)[e]
+                                              : static_cast<const uint32_t*>(d.ent_off)[e];
+        if (ix >= lds_end) return fail(JJA_EINVAL, "solve LDS offset out of range");
+      }
     }
     if (barriers < 0) barriers = nb;
     if (nb != barriers) return fail(JJA_EINVAL, "wave programs disagree on their barrier count");
@@ -877,10 +942,10 @@ int validate(const jja_problem_desc& d) {
   return JJA_OK;
 }
 
-template <int BS, int SW>
-int launch_evolve(Plan* plan, int32_t n_work, int64_t lds, int32_t n_steps, int64_t step0,
-                  hipStream_t stream) {
-  auto* kern = evolve_kernel<BS, SW>;
+template <int BS, int SW, bool GLOBAL>
+int launch_evolve(Plan* plan, const WorkItem* work, int32_t n_work, int64_t lds, int32_t n_steps,
+                  int64_t step0, hipStream_t stream) {
+  auto* kern = evolve_kernel<BS, SW, GLOBAL>;
   // the solve reads LDS at the plan's absolute byte offsets: the dynamic LDS
   // array must start at address 0, i.e. the kernel must have no static LDS
   hipFuncAttributes attr;
@@ -889,22 +954,22 @@ int launch_evolve(Plan* plan, int32_t n_work, int64_t lds, int32_t n_steps, int6
   JJA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
   hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
-                     plan->d_problems, plan->d_runs, plan->d_work, n_steps, step0);
+                     plan->d_problems, plan->d_runs, work, n_steps, step0);
   JJA_HIP(hipGetLastError());
   return JJA_OK;
 }
 
-template <int BS>
-int launch_adjoint(Plan* plan, int32_t n_work, int64_t lds, int64_t k_hi, int64_t k_lo,
-                   hipStream_t stream) {
-  auto* kern = adjoint_kernel<BS>;
+template <int BS, bool GLOBAL>
+int launch_adjoint(Plan* plan, const WorkItem* work, int32_t n_work, int64_t lds, int64_t k_hi,
+                   int64_t k_lo, hipStream_t stream) {
+  auto* kern = adjoint_kernel<BS, GLOBAL>;
   hipFuncAttributes attr;
   JJA_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kern)));
   if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "adjoint kernel has static LDS");
   JJA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
   hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
-                     plan->d_problems, plan->d_adj, plan->d_work, k_hi, k_lo);
+                     plan->d_problems, plan->d_adj, work, k_hi, k_lo);
   JJA_HIP(hipGetLastError());
   return JJA_OK;
 }
@@ -916,6 +981,61 @@ int launch_vortex(Plan* plan, int32_t p, const double* theta, int32_t cols, int3
                      n_out);
   JJA_HIP(hipGetLastError());
   return JJA_OK;
+}
+
+// Work items of one launch: the columns of the problems whose face store is in
+// LDS first (n_lds of them), then those with a global face store.
+int upload_work(Plan* plan, const std::vector<WorkItem>& work, hipStream_t stream) {
+  if (work.size() > plan->work_cap) {
+    if (plan->d_work) {
+      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_work));
+      if (it != plan->allocations.end()) plan->allocations.erase(it);
+      JJA_HIP(hipFree(plan->d_work));
+      plan->d_work = nullptr;
+    }
+    void* d = nullptr;
+    JJA_HIP(hipMalloc(&d, work.size() * sizeof(WorkItem)));
+    plan->allocations.push_back(d);
+    plan->d_work = static_cast<WorkItem*>(d);
+    plan->work_cap = work.size();
+  }
+  JJA_HIP(hipMemcpyAsync(plan->d_work, work.data(), sizeof(WorkItem) * work.size(),
+                         hipMemcpyHostToDevice, stream));
+  return JJA_OK;
+}
+
+template <int BS, bool GLOBAL>
+int launch_evolve_sw(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, int32_t n_steps,
+                     int64_t step0, hipStream_t stream) {
+  switch (plan->stencil_width) {
+    case 3: return launch_evolve<BS, 3, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    case 4: return launch_evolve<BS, 4, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    default: return launch_evolve<BS, 5, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+  }
+}
+
+template <bool GLOBAL>
+int launch_evolve_bs(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, int32_t n_steps,
+                     int64_t step0, hipStream_t stream) {
+  switch (plan->block_size) {
+    case 64: return launch_evolve_sw<64, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    case 128: return launch_evolve_sw<128, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    case 256: return launch_evolve_sw<256, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    case 512: return launch_evolve_sw<512, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    default: return launch_evolve_sw<1024, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+  }
+}
+
+template <bool GLOBAL>
+int launch_adjoint_bs(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, int64_t k_hi,
+                      int64_t k_lo, hipStream_t stream) {
+  switch (plan->block_size) {
+    case 64: return launch_adjoint<64, GLOBAL>(plan, work, n, lds, k_hi, k_lo, stream);
+    case 128: return launch_adjoint<128, GLOBAL>(plan, work, n, lds, k_hi, k_lo, stream);
+    case 256: return launch_adjoint<256, GLOBAL>(plan, work, n, lds, k_hi, k_lo, stream);
+    case 512: return launch_adjoint<512, GLOBAL>(plan, work, n, lds, k_hi, k_lo, stream);
+    default: return launch_adjoint<1024, GLOBAL>(plan, work, n, lds, k_hi, k_lo, stream);
+  }
 }
 
 }  // namespace
@@ -969,6 +1089,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.n_solve_slices = d.n_solve_slices;
     dp.max_stage_rows = d.max_stage_rows;
     dp.copy_rows = d.face_copy_rows;
+    dp.index_bytes = d.index_bytes;
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
@@ -982,15 +1103,24 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       if ((rc = upload(plan, d.face_code, static_cast<size_t>(d.n_face_entries), &dp.face_code))) break;
       if ((rc = upload(plan, d.slice_rows, static_cast<size_t>(d.n_slice_rows), &dp.slice_rows))) break;
       if ((rc = upload(plan, d.ent_val, static_cast<size_t>(d.n_entries), &dp.ent_val))) break;
-      if ((rc = upload(plan, d.ent_off, static_cast<size_t>(d.n_index), &dp.ent_off))) break;
+      if (d.index_bytes == 2) {
+        const uint16_t* o = nullptr;
+        if ((rc = upload(plan, static_cast<const uint16_t*>(d.ent_off), static_cast<size_t>(d.n_index), &o))) break;
+        dp.ent_off = o;
+      } else {
+        const uint32_t* o = nullptr;
+        if ((rc = upload(plan, static_cast<const uint32_t*>(d.ent_off), static_cast<size_t>(d.n_index), &o))) break;
+        dp.ent_off = o;
+      }
     }
     {
       static const double zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       if ((rc = upload(plan, zeros, 8, &dp.zero))) break;
     }
     plan->problems.push_back(dp);
-    plan->lds_bytes.push_back((static_cast<int64_t>(d.n_faces) + d.max_stage_rows) *
-                              static_cast<int64_t>(sizeof(double)));
+    const int64_t store = (static_cast<int64_t>(d.n_faces) + d.max_stage_rows) * static_cast<int64_t>(sizeof(double));
+    plan->lds_bytes.push_back(d.index_bytes == 2 ? store : 0);
+    plan->zbuf_doubles.push_back(d.index_bytes == 2 ? 0 : static_cast<int64_t>(d.n_faces) + d.max_stage_rows);
   }
   if (rc == JJA_OK) {
     const DevProblem* dprob = nullptr;
@@ -1026,13 +1156,19 @@ int64_t jja_plan_lds_bytes(const jja_plan* p, int32_t i) {
   return plan->lds_bytes[i] <= plan->lds_limit ? plan->lds_bytes[i] : 0;
 }
 
+int64_t jja_plan_zbuf_doubles(const jja_plan* p, int32_t i) {
+  const Plan* plan = reinterpret_cast<const Plan*>(p);
+  if (!plan || i < 0 || i >= static_cast<int32_t>(plan->zbuf_doubles.size())) return 0;
+  return plan->zbuf_doubles[i];
+}
+
 int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_t step0,
                void* stream_ptr) {
   Plan* plan = reinterpret_cast<Plan*>(p);
   if (!plan || !runs || n_steps < 0 || step0 < 0) return fail(JJA_EINVAL, "jja_evolve: invalid arguments");
   if (n_steps == 0) return JJA_OK;
   const int32_t np = static_cast<int32_t>(plan->problems.size());
-  std::vector<WorkItem> work;
+  std::vector<WorkItem> work, work_g;
   int64_t lds = 0;
   for (int32_t i = 0; i < np; ++i) {
     const jja_problem_run& r = runs[i];
@@ -1041,47 +1177,29 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
     if (!r.Is.base || !r.Vs.base || !r.f.base || !r.T.base)
       return fail(JJA_EINVAL, "jja_evolve: every source needs a base array");
     if (!r.Vs.zero && !r.theta_s) return fail(JJA_EINVAL, "jja_evolve: non-zero Vs needs theta_s");
-    if (plan->lds_bytes[i] > plan->lds_limit)
+    const bool global = plan->zbuf_doubles[i] > 0;
+    if (global && !r.zbuf)
+      return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " needs a global face store (zbuf)");
+    if (!global && plan->lds_bytes[i] > plan->lds_limit)
       return fail(JJA_EUNSUPPORTED, "jja_evolve: face vector of problem " + std::to_string(i) +
                                         " needs " + std::to_string(plan->lds_bytes[i]) +
                                         " bytes of LDS; limit " + std::to_string(plan->lds_limit));
-    lds = std::max(lds, plan->lds_bytes[i]);
-    for (int32_t w = 0; w < r.W; ++w) work.push_back({i, w});
-  }
-  if (work.size() > plan->work_cap) {
-    if (plan->d_work) {
-      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_work));
-      if (it != plan->allocations.end()) plan->allocations.erase(it);
-      JJA_HIP(hipFree(plan->d_work));
-      plan->d_work = nullptr;
-    }
-    void* d = nullptr;
-    JJA_HIP(hipMalloc(&d, work.size() * sizeof(WorkItem)));
-    plan->allocations.push_back(d);
-    plan->d_work = static_cast<WorkItem*>(d);
-    plan->work_cap = work.size();
+    if (!global) lds = std::max(lds, plan->lds_bytes[i]);
+    for (int32_t w = 0; w < r.W; ++w) (global ? work_g : work).push_back({i, w});
   }
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   // pageable source: the call returns once the data is staged, and the copy is
   // ordered after earlier work on the stream
   JJA_HIP(hipMemcpyAsync(plan->d_runs, runs, sizeof(jja_problem_run) * np, hipMemcpyHostToDevice, stream));
-  JJA_HIP(hipMemcpyAsync(plan->d_work, work.data(), sizeof(WorkItem) * work.size(),
-                         hipMemcpyHostToDevice, stream));
-  const int32_t n_work = static_cast<int32_t>(work.size());
+  const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
+  work.insert(work.end(), work_g.begin(), work_g.end());
+  if (int rc = upload_work(plan, work, stream)) return rc;
   if (lds < 8) lds = 8;
-  const int sw = plan->stencil_width;
-#define JJA_LAUNCH(BSZ)                                                                     \
-  return sw == 3 ? launch_evolve<BSZ, 3>(plan, n_work, lds, n_steps, step0, stream)         \
-       : sw == 4 ? launch_evolve<BSZ, 4>(plan, n_work, lds, n_steps, step0, stream)         \
-                 : launch_evolve<BSZ, 5>(plan, n_work, lds, n_steps, step0, stream)
-  switch (plan->block_size) {
-    case 64: JJA_LAUNCH(64);
-    case 128: JJA_LAUNCH(128);
-    case 256: JJA_LAUNCH(256);
-    case 512: JJA_LAUNCH(512);
-    default: JJA_LAUNCH(1024);
-  }
-#undef JJA_LAUNCH
+  if (n_lds)
+    if (int rc = launch_evolve_bs<false>(plan, plan->d_work, n_lds, lds, n_steps, step0, stream)) return rc;
+  if (n_g)
+    if (int rc = launch_evolve_bs<true>(plan, plan->d_work + n_lds, n_g, 8, n_steps, step0, stream)) return rc;
+  return JJA_OK;
 }
 
 int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t k_lo, void* stream_ptr) {
@@ -1089,7 +1207,7 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
   if (!plan || !runs || k_lo < 0 || k_hi < k_lo) return fail(JJA_EINVAL, "jja_adjoint: invalid arguments");
   if (plan->stencil_width != 3) return fail(JJA_EUNSUPPORTED, "jja_adjoint: stencil width 3 only");
   const int32_t np = static_cast<int32_t>(plan->problems.size());
-  std::vector<WorkItem> work;
+  std::vector<WorkItem> work, work_g;
   int64_t lds = 0;
   for (int32_t i = 0; i < np; ++i) {
     const jja_adjoint_run& r = runs[i];
@@ -1098,10 +1216,13 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
     if (r.hist_m0 > k_lo - 2) return fail(JJA_EINVAL, "jja_adjoint: hist must start at k_lo - 2 or earlier");
     if (r.g && (!r.g_slot || r.g_lo < r.slot_t0 || r.g_hi < r.g_lo))
       return fail(JJA_EINVAL, "jja_adjoint: g needs g_slot covering [g_lo, g_hi)");
-    if (plan->lds_bytes[i] > plan->lds_limit)
+    const bool global = plan->zbuf_doubles[i] > 0;
+    if (global && !r.zbuf)
+      return fail(JJA_EINVAL, "jja_adjoint: problem " + std::to_string(i) + " needs a global face store (zbuf)");
+    if (!global && plan->lds_bytes[i] > plan->lds_limit)
       return fail(JJA_EUNSUPPORTED, "jja_adjoint: face vector of problem " + std::to_string(i) + " exceeds LDS");
-    lds = std::max(lds, plan->lds_bytes[i]);
-    for (int32_t w = 0; w < r.W; ++w) work.push_back({i, w});
+    if (!global) lds = std::max(lds, plan->lds_bytes[i]);
+    for (int32_t w = 0; w < r.W; ++w) (global ? work_g : work).push_back({i, w});
   }
   if (!plan->d_adj) {
     void* d = nullptr;
@@ -1109,32 +1230,17 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
     plan->allocations.push_back(d);
     plan->d_adj = static_cast<jja_adjoint_run*>(d);
   }
-  if (work.size() > plan->work_cap) {
-    if (plan->d_work) {
-      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_work));
-      if (it != plan->allocations.end()) plan->allocations.erase(it);
-      JJA_HIP(hipFree(plan->d_work));
-      plan->d_work = nullptr;
-    }
-    void* d = nullptr;
-    JJA_HIP(hipMalloc(&d, work.size() * sizeof(WorkItem)));
-    plan->allocations.push_back(d);
-    plan->d_work = static_cast<WorkItem*>(d);
-    plan->work_cap = work.size();
-  }
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   JJA_HIP(hipMemcpyAsync(plan->d_adj, runs, sizeof(jja_adjoint_run) * np, hipMemcpyHostToDevice, stream));
-  JJA_HIP(hipMemcpyAsync(plan->d_work, work.data(), sizeof(WorkItem) * work.size(),
-                         hipMemcpyHostToDevice, stream));
-  const int32_t n_work = static_cast<int32_t>(work.size());
+  const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
+  work.insert(work.end(), work_g.begin(), work_g.end());
+  if (int rc = upload_work(plan, work, stream)) return rc;
   if (lds < 8) lds = 8;
-  switch (plan->block_size) {
-    case 64: return launch_adjoint<64>(plan, n_work, lds, k_hi, k_lo, stream);
-    case 128: return launch_adjoint<128>(plan, n_work, lds, k_hi, k_lo, stream);
-    case 256: return launch_adjoint<256>(plan, n_work, lds, k_hi, k_lo, stream);
-    case 512: return launch_adjoint<512>(plan, n_work, lds, k_hi, k_lo, stream);
-    default: return launch_adjoint<1024>(plan, n_work, lds, k_hi, k_lo, stream);
-  }
+  if (n_lds)
+    if (int rc = launch_adjoint_bs<false>(plan, plan->d_work, n_lds, lds, k_hi, k_lo, stream)) return rc;
+  if (n_g)
+    if (int rc = launch_adjoint_bs<true>(plan, plan->d_work + n_lds, n_g, 8, k_hi, k_lo, stream)) return rc;
+  return JJA_OK;
 }
 
 int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_t* n_out,

@@ -117,17 +117,21 @@ class Engine:
         S = self.S = sw - 1
         self.block_size = problem.block_size or default_block_size(Nj, Nf)
         self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
-                                block_size=self.block_size)
+                                block_size=self.block_size,
+                                face_store=getattr(problem, "face_store", "auto"))
         self.lib = lib = _native.lib()
         self._desc = self.host.desc()
         self.handle = C.c_void_p()
         _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
-        if lib.jja_plan_lds_bytes(self.handle, 0) == 0:
+        zd = lib.jja_plan_zbuf_doubles(self.handle, 0)
+        if zd == 0 and lib.jja_plan_lds_bytes(self.handle, 0) == 0 and Nf:
             self.close()
             raise _native.NativeError(
                 f"circuit with {Nf} faces needs {self.host.lds_bytes()} bytes of LDS per "
-                "column; larger than one workgroup's LDS")
+                "column; larger than this device's workgroup LDS")
         dev = self.dev = torch.device("cuda")
+        # global face store: one array of Nf + stage-scratch doubles per column
+        self.zbuf = torch.empty((W, zd), dtype=torch.float64, device=dev) if zd else None
         f64 = torch.float64
         cfgs = [problem.config_at_minus_1, problem.config_at_minus_2]
         if sw >= 4:
@@ -181,6 +185,7 @@ class Engine:
         run.slot_t0 = 0
         run.th_out = self.th_out.data_ptr()
         run.I_out = self.I_out.data_ptr()
+        run.zbuf = self.zbuf.data_ptr() if self.zbuf is not None else None
         T_zero = self.srcs["T"].src.zero
         self.ref_noise = (not T_zero) and problem.noise == "reference"
         if not T_zero and problem.noise == "philox":

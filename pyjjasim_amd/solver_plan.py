@@ -24,6 +24,7 @@ from . import _native
 from .ordering import nested_dissection
 
 MAX_ITER = 8          # entries per lane and solve record (kernel register budget)
+LDS_BYTES = 163840    # dynamic LDS one workgroup may use on MI355X (gfx950)
 WAVE = 64             # lanes of one wavefront: solve records are per wave
 
 STENCILS = {
@@ -314,7 +315,11 @@ class CircuitPlan:
     """All host arrays of one circuit for ``jja_plan_create``, laid out for
     workgroups of ``block_size`` threads."""
 
-    def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8):
+    def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8,
+                 face_store="auto"):
+        if face_store not in ("auto", "lds", "global"):
+            raise ValueError("face_store must be 'auto', 'lds' or 'global'")
+        self.face_store = face_store
         if stencil_width not in STENCILS:
             raise ValueError(f"stencil width must be 3, 4 or 5 (equals {stencil_width})")
         self.Nj, self.Nf = circuit._Nj(), circuit._Nf()
@@ -341,9 +346,15 @@ class CircuitPlan:
 
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.block_size, self.solve_plan, self.Nf, self.Nj
-        if Nf + sp.max_stage_rows > 65536:
-            raise ValueError(f"circuit with {Nf} faces is too large for the LDS-resident solve "
-                             "(16-bit LDS indices)")
+        # the face vector and stage scratch of a column: in the workgroup's LDS
+        # when they fit (16-bit indices), else in a global per-column array
+        store = 8 * (Nf + sp.max_stage_rows)
+        fits = store <= LDS_BYTES and Nf + sp.max_stage_rows <= 65536
+        if self.face_store == "lds" and not fits:
+            raise ValueError(f"circuit with {Nf} faces needs {store} bytes of face store per column; "
+                             f"more than one workgroup's LDS ({LDS_BYTES})")
+        self.global_store = self.face_store == "global" or not fits
+        self.index_bytes = 4 if self.global_store else 2
         pos = sp.face_pos.astype(np.int64)
         # junctions: face positions of the (up to two) faces, ascending face index
         jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
@@ -537,7 +548,7 @@ class CircuitPlan:
         self.wave_ptr = np.asarray(wave_ptr, dtype=np.int64)
         self.n_solve_exec = n_exec
         self.ent_val = np.concatenate(vals).astype(np.float64)
-        self.ent_off = np.concatenate(offs).astype(np.uint16)
+        self.ent_off = np.concatenate(offs).astype(np.uint32 if self.global_store else np.uint16)
         self.slice_rows = np.concatenate(rows_out).astype(np.int32)
         self.n_slices = len(solve)
 
@@ -577,12 +588,14 @@ class CircuitPlan:
         d.n_entries = len(self.ent_val)
         d.ent_val = P(self.ent_val, C.c_double)
         d.n_index = len(self.ent_off)
-        d.ent_off = P(self.ent_off, C.c_uint16)
+        d.ent_off = self.ent_off.ctypes.data
         d.max_stage_rows = sp.max_stage_rows
         d.face_copy_rows = self.face_copy_rows
+        d.index_bytes = self.index_bytes
         return d
 
     def lds_bytes(self):
+        """Bytes of the per-column face store (LDS or global)."""
         return 8 * (self.Nf + self.solve_plan.max_stage_rows)
 
     # ---------------------------------------------------- host emulation
@@ -590,7 +603,7 @@ class CircuitPlan:
         """Bytes of solve-program data one column loads per step: every executed
         record moves 8 values and 8 LDS indices per lane, a row per lane and its
         32-byte descriptor."""
-        return int(self.n_solve_exec * (512 * 8 + 512 * 2 + WAVE * 4 + 32))
+        return int(self.n_solve_exec * (512 * 8 + 512 * self.index_bytes + WAVE * 4 + 32))
 
     def emulate_face_and_solve(self, u):
         """Exactly the kernel's face phase and per-wave solve programs, on the host:

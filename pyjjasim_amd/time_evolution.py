@@ -21,6 +21,9 @@ Extensions (keyword-only, defaults keep reference behaviour):
                    launches of at most 2000 steps).
   block_size       Threads per workgroup (64..1024; default chosen by size).
   leaf_size        Largest leaf block of the nested-dissection factor schedule.
+  face_store       "auto" (default): each column keeps its face vector in LDS when
+                   it fits (up to about 10^4 faces) and in global memory otherwise;
+                   "lds" or "global" force one.
 
 There is no CPU fallback: without a HIP device ``compute()`` raises.
 """
@@ -73,7 +76,7 @@ class TimeEvolutionProblem:
                  store_voltage=True, store_current=True, config_at_minus_1=None,
                  config_at_minus_2=None, config_at_minus_3=None, config_at_minus_4=None,
                  stencil_width=3, *, noise="philox", seed=None, steps_per_launch=None,
-                 block_size=None, leaf_size=16):
+                 block_size=None, leaf_size=16, face_store="auto"):
         self.circuit = circuit
         self.time_step = time_step
         self.time_step_count = time_step_count
@@ -119,6 +122,7 @@ class TimeEvolutionProblem:
         self.steps_per_launch = steps_per_launch
         self.block_size = block_size
         self.leaf_size = leaf_size
+        self.face_store = face_store
 
     # ------------------------------------------------------------- getters
     def get_problem_count(self):

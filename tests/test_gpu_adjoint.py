@@ -93,3 +93,26 @@ def test_adjoint_matches_finite_differences_with_noise():
             ad = leaves[k].grad[i].item()
             print(f"  {name}{i}: adjoint {ad:.10e} fd {fd:.10e}", flush=True)
             assert abs(ad - fd) <= 1e-7 * abs(fd) + 1e-8, (name, i, ad, fd)
+
+
+def test_adjoint_global_face_store_bit_identical():
+    """The adjoint sweep with the face store in global memory gives the same bits
+    as with it in LDS."""
+    print("[adjoint] global store", flush=True)
+    c = _circuit()
+    W, dt, Nt = 2, 0.05, 50
+    rng = np.random.default_rng(21)
+    Ic, f, Is = _inputs(c, W, rng, "cuda")
+    store = np.zeros(Nt, dtype=bool)
+    store[[20, 49]] = True
+    G = torch.tensor(rng.standard_normal((c.junction_count(), W, 2)), dtype=torch.float64, device="cuda")
+    grads = {}
+    for fs in ("lds", "global"):
+        evo = pa.DifferentiableEvolution(c, time_step=dt, time_step_count=Nt, problem_count=W,
+                                         store_time_steps=store, checkpoint_every=16, block_size=64,
+                                         face_store=fs)
+        leaves = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
+        (evo(*leaves) * G).sum().backward()
+        grads[fs] = [t.grad.cpu().numpy() for t in leaves]
+    for a, b in zip(grads["lds"], grads["global"]):
+        np.testing.assert_array_equal(a, b)

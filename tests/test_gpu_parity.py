@@ -105,3 +105,47 @@ def test_gpu_rejects_custom_cpr():
                                   lambda Ic, t: 0 * t)
     with pytest.raises(ValueError, match="current-phase relation"):
         pa.TimeEvolutionProblem(c, current_phase_relation=cpr, time_step_count=5).compute()
+
+
+@pytest.mark.parametrize("T", [0.0, 0.02])
+def test_gpu_global_face_store_bit_identical_to_lds(T):
+    """The global-memory face store runs the same record program and arithmetic as
+    the LDS one: identical bits."""
+    print(f"[gpu parity] global store T={T}", flush=True)
+    c = pa.SquareArray(12, 9)
+    rng = np.random.default_rng(8)
+    c.set_critical_current(np.exp(0.1 * rng.standard_normal(c.junction_count())))
+    c.set_capacitance(0.7)
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.3, 1.5, 3)[None, :, None]
+    out = {}
+    for store in ("lds", "global"):
+        p = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=400, external_flux=0.2,
+                                    current_sources=Is, temperature=T, seed=5, block_size=128,
+                                    face_store=store)
+        r = p.compute()
+        out[store] = (r.get_theta(), r.get_voltage(), r.get_vortex_configuration())
+    for a, b in zip(out["lds"], out["global"]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_gpu_large_array_global_store_matches_oracle():
+    """SquareArray(160, 160): 25,281 faces, too many for LDS; the global face store
+    against the oracle (SuperLU) over a short horizon."""
+    print("[gpu parity] 160x160 global store", flush=True)
+    c = pa.SquareArray(160, 160)
+    rng = np.random.default_rng(2)
+    c.set_critical_current(np.exp(0.1 * rng.standard_normal(c.junction_count())))
+    Is = 0.8 * c.current_base(0.0)[:, None, None] * np.array([1.0, 1.4])[None, :, None]
+    kw = dict(time_step=0.05, time_step_count=30, external_flux=0.15, current_sources=Is)
+    r = pa.TimeEvolutionProblem(c, **kw).compute()
+    from oracle import te_oracle
+    Nt, Nj, Nf = 30, c.junction_count(), c.face_count()
+    inputs = te_oracle.Inputs(Nj, Nf, Nt, 0.15, Is, 0.0, 0.0)
+    zero = np.zeros((Nj, 2))
+    th_o, _ = te_oracle.evolve_core(c, 0.05, Nt, inputs, np.ones(Nt, dtype=bool), np.zeros(Nt, dtype=bool),
+                                    [zero, zero])
+    th = r.get_theta()
+    scale = np.abs(th_o[:, :, 2:]).max()
+    assert np.abs(th - th_o[:, :, 2:]).max() <= 1e-9 * scale
+    n_o = te_oracle.vortex(c.get_cycle_matrix(), th_o[:, :, 2:])
+    np.testing.assert_array_equal(r.get_vortex_configuration(), n_o)

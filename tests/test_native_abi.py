@@ -33,7 +33,8 @@ def test_library_exports_all_declared_symbols():
 
 
 STRUCTS = {"jja_source": _native.Source, "jja_problem_run": _native.ProblemRun,
-           "jja_problem_desc": _native.ProblemDesc, "jja_slice": _native.Slice}
+           "jja_problem_desc": _native.ProblemDesc, "jja_slice": _native.Slice,
+           "jja_adjoint_run": _native.AdjointRun}
 
 
 def test_struct_layout_matches_header(tmp_path):

@@ -109,14 +109,16 @@ def test_sliced_layout_holed_array():
                                atol=1e-12 * abs(want).max())
 
 
+@pytest.mark.parametrize("store", ["lds", "global"])
 @pytest.mark.parametrize("BS", [64, 1024])
 @pytest.mark.parametrize("name", ["sq10", "hc7", "tri8", "sqL", "het", "squid"])
-def test_plan_descriptors_pass_native_validation(name, BS):
+def test_plan_descriptors_pass_native_validation(name, BS, store):
     """Every plan the host builds passes the library's own index and offset checks
     (the ones jja_plan_create runs before any upload); no device is touched."""
     import ctypes as C
     from pyjjasim_amd import _native
-    plan = CircuitPlan(_circuits()[name], 0.05, leaf_size=8, block_size=BS)
+    plan = CircuitPlan(_circuits()[name], 0.05, leaf_size=8, block_size=BS, face_store=store)
+    assert plan.index_bytes == (4 if store == "global" else 2)
     d = plan.desc()
     NW = BS // 64
     assert all((plan.wave_ptr[w + 1] - plan.wave_ptr[w] - 4) % 3 == 0 for w in range(NW))
@@ -146,3 +148,18 @@ def test_native_validation_rejects_bad_programs():
     plan._solve_rec[w1[0]].flags &= ~_native.SLICE_BARRIER
     assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
     assert b"barrier" in lib.jja_last_error()
+
+
+def test_large_circuit_plans_a_global_face_store():
+    """A face vector beyond one workgroup's LDS is planned in global memory with
+    32-bit indices, and the host emulation of that program still solves K J = A u."""
+    c = pa.SquareArray(130, 130)
+    plan = CircuitPlan(c, 0.05, block_size=1024)
+    assert plan.global_store and plan.index_bytes == 4 and plan.ent_off.dtype == np.uint32
+    with pytest.raises(ValueError, match="LDS"):
+        CircuitPlan(c, 0.05, block_size=1024, face_store="lds")
+    small = pa.SquareArray(8, 8)
+    p2 = CircuitPlan(small, 0.05, block_size=256, face_store="global")
+    u = np.random.default_rng(4).standard_normal(small.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(p2.K), small.get_cycle_matrix() @ u)
+    np.testing.assert_allclose(p2.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
