@@ -2,6 +2,7 @@
 from .graph import *  # noqa: F401,F403
 from .circuit import *  # noqa: F401,F403
 from .time_evolution import *  # noqa: F401,F403
+from .batch import TimeEvolutionBatch  # noqa: F401
 
 
 def __getattr__(name):

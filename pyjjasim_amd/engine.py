@@ -57,12 +57,20 @@ class DeviceSource:
         self.src.zero = int(bool(zero))
         self.tensor = None
         if not self.callable:
-            dev, (s_i, s_w, s_t) = source_layout(value, n, W, Nt)
+            sep = hasattr(value, "amplitude") and hasattr(value, "base")
+            base = value.base[:, :, None] if sep else value
+            dev, (s_i, s_w, s_t) = source_layout(base, n, W, Nt)
             self.tensor = torch.from_numpy(dev).cuda()
             self.src.base = self.tensor.data_ptr()
             self.src.s_i, self.src.s_w, self.src.s_t = s_i, s_w, s_t
             self.src.t0 = 0
             self.bytes_per_step = 8 * (n if s_i else 1) * (W if s_w else 1)
+            if sep:
+                # amplitude[w, t] as [t][w]: value = base * amplitude, one product
+                amp, (a_1, a_w, a_t) = source_layout(value.amplitude[None, :, :], 1, W, Nt)
+                self.amp_tensor = torch.from_numpy(amp).cuda()
+                self.src.amp = self.amp_tensor.data_ptr()
+                self.src.a_w, self.src.a_t = a_w, a_t
         else:
             self.bytes_per_step = 8 * n * W
 
@@ -103,7 +111,10 @@ def default_block_size(Nj, Nf):
 
 
 class Engine:
-    def __init__(self, problem, th_mask, I_mask):
+    """Device state and launches of one problem. With ``shared = (handle, index,
+    host)`` the problem is problem ``index`` of a plan that a BatchEngine owns."""
+
+    def __init__(self, problem, th_mask, I_mask, shared=None):
         if not isinstance(problem.current_phase_relation, DefaultCPR):
             raise ValueError("the GPU time evolution evaluates the default current-phase "
                              "relation Ic*sin(theta) only; got a custom CurrentPhaseRelation")
@@ -115,16 +126,22 @@ class Engine:
         dt = problem._dt()
         sw = problem.stencil_width
         S = self.S = sw - 1
-        self.block_size = problem.block_size or default_block_size(Nj, Nf)
-        self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
-                                block_size=self.block_size,
-                                face_store=getattr(problem, "face_store", "auto"))
         self.lib = lib = _native.lib()
-        self._desc = self.host.desc()
-        self.handle = C.c_void_p()
-        _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
-        zd = lib.jja_plan_zbuf_doubles(self.handle, 0)
-        if zd == 0 and lib.jja_plan_lds_bytes(self.handle, 0) == 0 and Nf:
+        if shared is None:
+            self.block_size = problem.block_size or default_block_size(Nj, Nf)
+            self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
+                                    block_size=self.block_size,
+                                    face_store=getattr(problem, "face_store", "auto"))
+            self._desc = self.host.desc()
+            self.handle = C.c_void_p()
+            _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
+            self.owns_plan, self.index = True, 0
+        else:
+            self.handle, self.index, self.host = shared
+            self.block_size = self.host.block_size
+            self.owns_plan = False
+        zd = lib.jja_plan_zbuf_doubles(self.handle, self.index)
+        if zd == 0 and lib.jja_plan_lds_bytes(self.handle, self.index) == 0 and Nf:
             self.close()
             raise _native.NativeError(
                 f"circuit with {Nf} faces needs {self.host.lds_bytes()} bytes of LDS per "
@@ -198,11 +215,10 @@ class Engine:
         self.t = 0
         self._noise_buf = None
 
-    def launch(self, n_steps, stream=None):
-        """One persistent launch of n_steps steps starting at the current step."""
+    def prepare(self, n_steps):
+        """Host work before a launch of n_steps steps: callable inputs and the
+        reference-noise draws for the chunk; returns the run descriptor."""
         torch = self.torch
-        if stream is None:
-            stream = torch.cuda.current_stream().cuda_stream
         for s in self.srcs.values():
             s.fill_chunk(torch, self.t, n_steps)
         if self.ref_noise:
@@ -214,6 +230,14 @@ class Engine:
             self.run.noise_t0 = self.t
         run = self.run
         run.Is, run.Vs, run.f, run.T = (self.srcs[k].src for k in ("Is", "Vs", "f", "T"))
+        return run
+
+    def launch(self, n_steps, stream=None):
+        """One persistent launch of n_steps steps starting at the current step."""
+        torch = self.torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        run = self.prepare(n_steps)
         _native.check(self.lib.jja_evolve(self.handle, C.byref(run), n_steps, self.t, stream))
         if self.host_chunks:
             torch.cuda.current_stream().synchronize()      # chunk buffers are reused
@@ -233,7 +257,7 @@ class Engine:
         n_dev = None
         if self.n_th and self.Nf:
             n_dev = torch.empty((self.n_th, W, self.Nf), dtype=torch.int32, device=self.dev)
-            _native.check(self.lib.jja_vortex(self.handle, 0, self.th_out.data_ptr(),
+            _native.check(self.lib.jja_vortex(self.handle, self.index, self.th_out.data_ptr(),
                                               self.n_th * W, n_dev.data_ptr(), stream))
         torch.cuda.synchronize()
         th = self.th_out[:self.n_th].cpu().numpy().transpose(2, 1, 0) if self.n_th \
@@ -268,7 +292,88 @@ class Engine:
         return (self.W * (24 * self.Nj + 48 * self.Nf) + factor + 8 * self.Nj * 5 + src + out)
 
     def close(self):
-        if getattr(self, "handle", None) is not None and self.handle.value:
+        if getattr(self, "owns_plan", False) and getattr(self, "handle", None) is not None \
+                and self.handle.value:
+            self.lib.jja_plan_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchEngine:
+    """Several problems (any circuits and column counts, one stencil width and
+    step count) in one plan: every launch advances all their columns together,
+    one workgroup per column (time_evolution.py:461-582 for each problem)."""
+
+    def __init__(self, problems, masks, block_size=None, hosts=None):
+        if not problems:
+            raise ValueError("empty batch")
+        Nt = problems[0]._Nt()
+        sw = problems[0].stencil_width
+        if any(p._Nt() != Nt or p.stencil_width != sw for p in problems):
+            raise ValueError("a batch needs one time_step_count and one stencil width")
+        self.torch = _torch()
+        self.lib = _native.lib()
+        if hosts is not None:
+            self.block_size = hosts[0].block_size
+            self.hosts = list(hosts)
+        else:
+            self.block_size = block_size or max(default_block_size(p.circuit._Nj(), p.circuit._Nf())
+                                                for p in problems)
+            self.hosts = [CircuitPlan(p.circuit, p._dt(), sw, leaf_size=p.leaf_size,
+                                      block_size=self.block_size,
+                                      face_store=getattr(p, "face_store", "auto")) for p in problems]
+        descs = (_native.ProblemDesc * len(problems))(*[h.desc() for h in self.hosts])
+        self._descs = descs
+        self.handle = C.c_void_p()
+        _native.check(self.lib.jja_plan_create(len(problems), descs, C.byref(self.handle)))
+        self.engines = []
+        try:
+            for i, (p, (thm, Im)) in enumerate(zip(problems, masks)):
+                e = Engine(p, thm, Im, shared=(self.handle, i, self.hosts[i]))
+                self.engines.append(e)
+        except Exception:
+            self.close()
+            raise
+        self.Nt = Nt
+        self.t = 0
+        spl = [p.steps_per_launch for p in problems if p.steps_per_launch]
+        self.steps_per_launch = min(spl) if spl else min(e.steps_per_launch for e in self.engines)
+        self.host_chunks = any(e.host_chunks for e in self.engines)
+
+    def launch(self, n_steps, stream=None):
+        torch = self.torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        runs = (_native.ProblemRun * len(self.engines))(*[e.prepare(n_steps) for e in self.engines])
+        _native.check(self.lib.jja_evolve(self.handle, runs, n_steps, self.t, stream))
+        if self.host_chunks:
+            torch.cuda.current_stream().synchronize()
+        self.t += n_steps
+        for e in self.engines:
+            e.t = self.t
+
+    def advance(self, n_steps):
+        if self.t + n_steps > self.Nt:
+            raise ValueError("advance beyond time_step_count")
+        end = self.t + n_steps
+        while self.t < end:
+            self.launch(min(self.steps_per_launch, end - self.t))
+
+    def results(self):
+        return [e.results() for e in self.engines]
+
+    def bytes_per_step(self):
+        return sum(e.bytes_per_step() for e in self.engines)
+
+    def close(self):
+        for e in self.engines:
+            e.close()
+        if self.handle is not None and self.handle.value:
             self.lib.jja_plan_destroy(self.handle)
             self.handle = C.c_void_p()
 

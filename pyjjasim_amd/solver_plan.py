@@ -14,6 +14,7 @@ Everything here is host work that runs once; the per-step work is on the GPU.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 
 import numpy as np
@@ -342,7 +343,16 @@ class CircuitPlan:
         else:
             xy = np.zeros((0, 2))
         self.solve_plan = SolvePlan(self.K, xy, leaf_size=leaf_size)
+        self.target_iter = target_iter
         self._build_layouts(target_iter)
+
+    def relayout(self, block_size):
+        """The same circuit, factor and schedule laid out for another workgroup
+        size (the factorisation does not depend on it)."""
+        other = copy.copy(self)
+        other.block_size = block_size
+        other._build_layouts(self.target_iter)
+        return other
 
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.block_size, self.solve_plan, self.Nf, self.Nj

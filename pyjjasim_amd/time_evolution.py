@@ -34,7 +34,7 @@ import numpy as np
 from .circuit import Circuit, DefaultCPR
 from .solver_plan import STENCILS
 
-__all__ = ["TimeEvolutionProblem", "TimeEvolutionResult", "ThetaNotStored", "CurrentNotStored",
+__all__ = ["TimeEvolutionProblem", "TimeEvolutionResult", "SeparableSource", "ThetaNotStored", "CurrentNotStored",
            "VoltageNotStored", "DataAtTimepointNotStored"]
 
 
@@ -54,7 +54,43 @@ class DataAtTimepointNotStored(Exception):
     pass
 
 
+class SeparableSource:
+    """An input whose value at element i, column w and step t is
+    base[i, w] * amplitude[w, t] (base broadcastable to (N, W), amplitude to
+    (W, Nt)), e.g. a bias pattern times a drive waveform:
+
+        SeparableSource(circuit.current_base(0), Idc + 0.5 * np.sin(0.3 * dt * np.arange(Nt)))
+
+    It stands for the array base[:, :, None] * amplitude[None, :, :] with the
+    reference's broadcasting (time_evolution.py:119-130) without forming it: the
+    device evaluates the product, one multiplication, as numpy does."""
+
+    def __init__(self, base, amplitude):
+        b = np.asarray(base, dtype=np.float64)
+        a = np.asarray(amplitude, dtype=np.float64)
+        self.base = b[:, None] if b.ndim == 1 else b
+        self.amplitude = a[None, :] if a.ndim == 1 else a
+        if self.base.ndim != 2 or self.amplitude.ndim != 2:
+            raise ValueError("SeparableSource: base (N,) or (N, W), amplitude (Nt,) or (W, Nt)")
+        if self.base.shape[1] not in (1, self.amplitude.shape[0]) and self.amplitude.shape[0] != 1:
+            raise ValueError("SeparableSource: base and amplitude disagree on W")
+        self.shape = (self.base.shape[0], max(self.base.shape[1], self.amplitude.shape[0]),
+                      self.amplitude.shape[1])
+
+    def __getitem__(self, key):
+        i, w, t = key
+        return (self.base[:, :, None] * self.amplitude[None, :, :])[i, w, t] \
+            if not isinstance(t, (int, np.integer)) else \
+            (self.base * self.amplitude[:, t][None, :])[i, w]
+
+    def __array__(self, dtype=None, copy=None):
+        out = self.base[:, :, None] * self.amplitude[None, :, :]
+        return out if dtype is None else out.astype(dtype)
+
+
 def _prob_cnt(x):
+    if isinstance(x, SeparableSource):
+        return x.shape[1]
     s = np.array(x(0) if callable(x) else x).shape
     return s[1] if len(s) > 1 else 1
 
@@ -89,9 +125,16 @@ class TimeEvolutionProblem:
         self._Is_is_timedep = self._is_timedep(current_sources)
         self._Vs_is_timedep = self._is_timedep(voltage_sources)
         self._T_is_timedep = self._is_timedep(temperature)
-        bc = lambda x, n: x if callable(x) else np.broadcast_to(np.array(x), (n, W, Nt))
+        def bc(x, n):
+            if callable(x):
+                return x
+            if isinstance(x, SeparableSource):
+                if x.shape[0] not in (1, n) or x.shape[2] not in (1, Nt):
+                    raise ValueError("SeparableSource shape does not match the circuit or Nt")
+                return x
+            return np.broadcast_to(np.array(x), (n, W, Nt))
         # un-broadcast inputs: what is uploaded to the device (with zero strides)
-        self._raw = {k: (v if callable(v) else np.array(v)) for k, v in
+        self._raw = {k: (v if callable(v) or isinstance(v, SeparableSource) else np.array(v)) for k, v in
                      (("f", external_flux), ("Is", current_sources), ("Vs", voltage_sources),
                       ("T", temperature))}
         self.external_flux = bc(external_flux, Nf)
@@ -206,6 +249,8 @@ class TimeEvolutionProblem:
     def _is_timedep(x):
         if callable(x):
             return True
+        if isinstance(x, SeparableSource):
+            return x.shape[2] > 1
         s = np.array(x).shape
         return len(s) > 0 and s[-1] > 1
 
@@ -263,8 +308,9 @@ def _apply_derivative(x, index, stencil, dt):
     return acc / dt
 
 
-def time_evolution(problem: TimeEvolutionProblem):
-    """Storage masks, GPU core, voltage post-processing (time_evolution.py:422-458)."""
+def _masks(problem: TimeEvolutionProblem):
+    """Steps whose theta and current the core must keep: the stored steps plus,
+    for voltages, the stencil's earlier steps (time_evolution.py:422-440)."""
     Nt = problem._Nt()
     th_mask = problem.store_time_steps.copy() if (problem.store_theta or problem.store_voltage) \
         else np.zeros(Nt, dtype=bool)
@@ -280,7 +326,15 @@ def time_evolution(problem: TimeEvolutionProblem):
         Vth[ids] = True
         if problem.circuit._has_inductance():
             VI[ids] = True
-    th_out, I_out, n_out = _gpu_core(problem, Vth, VI)
+    return th_mask, I_mask, Vth, VI
+
+
+def _finish(problem: TimeEvolutionProblem, masks, th_out, I_out, n_out):
+    """Voltages by the stencil derivative (plus L dI/dt), helper steps and the
+    prefix removed (time_evolution.py:441-458)."""
+    th_mask, I_mask, Vth, VI = masks
+    Nj = problem.circuit._Nj()
+    off = problem.stencil_width - 1
     V_out = None
     if problem.store_voltage:
         ts = np.flatnonzero(problem.store_time_steps[Vth])
@@ -299,6 +353,13 @@ def time_evolution(problem: TimeEvolutionProblem):
                                I_out if problem.store_current else None,
                                V_out if problem.store_voltage else None,
                                vortex=n_out if problem.store_theta else None)
+
+
+def time_evolution(problem: TimeEvolutionProblem):
+    """Storage masks, GPU core, voltage post-processing (time_evolution.py:422-458)."""
+    masks = _masks(problem)
+    th_out, I_out, n_out = _gpu_core(problem, masks[2], masks[3])
+    return _finish(problem, masks, th_out, I_out, n_out)
 
 
 def _gpu_core(problem: TimeEvolutionProblem, th_mask, I_mask):

@@ -163,3 +163,22 @@ def test_large_circuit_plans_a_global_face_store():
     u = np.random.default_rng(4).standard_normal(small.junction_count())
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(p2.K), small.get_cycle_matrix() @ u)
     np.testing.assert_allclose(p2.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
+
+
+@pytest.mark.parametrize("bs", [128, 256, 512, 1024])
+def test_relayout_plan_validates_and_solves(bs):
+    """A plan re-laid out for another block size keeps the factor and passes
+    the library's checks; its wave programs still solve K J = A u."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    a = pa.TriangularArray(9, 7)
+    a.set_inductance(0.7)
+    plan = CircuitPlan(a, 0.05, block_size=1024)
+    p2 = plan.relayout(bs)
+    assert p2.block_size == bs
+    assert _native.lib().jja_plan_validate(1, C.byref(p2.desc())) == 0, _native.lib().jja_last_error()
+    assert p2.solve_plan is plan.solve_plan and plan.block_size == 1024
+    u = np.random.default_rng(bs).standard_normal(a.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), a.get_cycle_matrix() @ u)
+    np.testing.assert_allclose(p2.emulate_face_and_solve(u), want, rtol=1e-12,
+                               atol=1e-12 * abs(want).max())
