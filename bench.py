@@ -17,7 +17,10 @@ after ``--warmup`` untimed steps. Reported:
              library calls), timed on this host on a bounded sample of steps
 
 Usage: python bench.py [--gpus 1] [--steps 20000] [--warmup 1000] [--plain-steps 0]
-       [--no-cpu] [--copy-calibration MIB]
+       [--no-cpu] [--no-adjoint] [--copy-calibration MIB]
+       python bench.py --workload uneven_batch [--steps 10000] [--warmup 200] [--problems 64]
+         (64 seeded problems of uneven size, one launch per chunk, autotuned; its own
+          JSON line)
 """
 import argparse
 import json
@@ -173,6 +176,92 @@ def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=5, fd_ste
                          "max_rel_err": float(max(errs))}}
 
 
+def uneven_problems(pa, Nt, n_problems=64, seed=3003):
+    """Uneven batch: n_problems seeded draws of lattice (square, honeycomb,
+    triangular) and count n in [16, 256]; junction self-inductance L = 1, R = 1,
+    C = 0; f ~ U[0, 0.5]; Shapiro drive Is(t) = (Idc + 0.5 sin(0.3 t)) *
+    current_base(0) with Idc ~ U[0.5, 1.5]; T = 0, dt = 0.05; W = 1 each; theta
+    stored at the last step."""
+    rng = np.random.default_rng(seed)
+    kinds = [pa.SquareArray, pa.HoneycombArray, pa.TriangularArray]
+    dt = 0.05
+    wave = np.sin(0.3 * dt * np.arange(Nt))
+    probs, names = [], []
+    for _ in range(n_problems):
+        K = kinds[rng.integers(3)]
+        n = int(rng.integers(16, 257))
+        f = float(rng.uniform(0.0, 0.5))
+        Idc = float(rng.uniform(0.5, 1.5))
+        c = K(n, n)
+        c.set_inductance(1.0)
+        drive = pa.SeparableSource(c.current_base(0.0), Idc + 0.5 * wave)
+        probs.append(pa.TimeEvolutionProblem(c, time_step=dt, time_step_count=Nt, external_flux=f,
+                                             current_sources=drive, store_time_steps=[Nt - 1],
+                                             store_voltage=False, store_current=False))
+        names.append(f"{K.__name__}({n},{n})")
+    return probs, names
+
+
+def uneven_bench(pa, torch, steps, warmup, plain_steps, n_problems=64):
+    """The uneven batch in one launch per chunk, autotuned; beside it the same
+    problems one at a time (the plain loop) on a sample of steps."""
+    from pyjjasim_amd.engine import BatchEngine
+    Nt = warmup + steps
+    t0 = time.perf_counter()
+    probs, names = uneven_problems(pa, Nt, n_problems)
+    tb = pa.TimeEvolutionBatch(probs, autotune=True)
+    eng = tb.make_engine()
+    setup_s = time.perf_counter() - t0
+    Njs = np.array([p.circuit._Nj() for p in probs])
+    try:
+        eng.advance(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.advance(steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        bytes_per_step = eng.bytes_per_step()
+        for p, (th, _, nv) in zip(probs, eng.results()):
+            A = p.circuit.get_cycle_matrix()
+            n_host = (-A @ np.round(th[:, :, -1] / (2.0 * np.pi))).astype(int)
+            assert np.array_equal(n_host, nv[:, :, -1]), "device vortex counts disagree with theta"
+        hosts = eng.hosts
+    finally:
+        eng.close()
+    value = float(Njs.sum()) * steps / wall
+    plain = None
+    if plain_steps > 0:
+        per_step = 0.0
+        for i, p in enumerate(probs):
+            e = BatchEngine([p], [(np.zeros(Nt, bool), np.zeros(Nt, bool))], hosts=[hosts[i]])
+            try:
+                e.launch(5)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                e.advance(plain_steps)
+                torch.cuda.synchronize()
+                per_step += (time.perf_counter() - t1) / plain_steps
+            finally:
+                e.close()
+        plain = {"value": float(Njs.sum()) / per_step, "unit": "junction-steps/s",
+                 "note": f"the same problems one at a time, {plain_steps} steps each",
+                 "batched_over_plain": value * per_step / float(Njs.sum())}
+    big = int(np.argmax(Njs))
+    return {"metric": "junction-steps/s (uneven batch, FP64)", "value": value,
+            "unit": "junction-steps/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": wall / steps * 1e3, "dtype": "f64",
+            "config": {"workload": f"uneven_batch_{n_problems}", "problems": n_problems,
+                       "junctions_total": int(Njs.sum()), "junctions_min": int(Njs.min()),
+                       "junctions_max": int(Njs.max()), "largest": names[big],
+                       "global_store_problems": int(sum(h.global_store for h in hosts)),
+                       "tuning": {k: (v if k != "block_size_s_per_step" else
+                                      {str(b): t * 1e3 for b, t in v.items()})
+                                  for k, v in tb.tuning.items()},
+                       "tuning_unit": "ms per step"},
+            "hbm_GBps_algorithmic": bytes_per_step * steps / wall / 1e9,
+            "host_setup_s": setup_s, "plain": plain}
+
+
 def measured_traffic(steps_per_launch):
     """HBM bytes per evolve_kernel launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor), or
@@ -219,6 +308,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--workload", choices=["iv_sweep", "uneven_batch"], default="iv_sweep",
+                    help="iv_sweep: SquareArray(100,100) W=256 (default); uneven_batch: "
+                         "64 problems of uneven size in one launch per chunk")
+    ap.add_argument("--problems", type=int, default=64, help="uneven_batch: problem count")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -233,6 +326,12 @@ def main():
     torch.cuda.set_device(0)
     if args.copy_calibration:
         copy_calibration(torch, args.copy_calibration)
+        return
+    if args.workload == "uneven_batch":
+        out = uneven_bench(pa, torch, args.steps, args.warmup, args.plain_steps, args.problems)
+        out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "data": "synthetic (seeded lattices, sizes, flux and drive)"})
+        print(json.dumps(out), flush=True)
         return
 
     wall, launch_ms, th, nv, c, info = run_gpu(pa, torch, args.n, args.W, args.steps, args.warmup,

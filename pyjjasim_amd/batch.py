@@ -5,7 +5,9 @@ problem, each identical to ``problem.compute()``: the problems (any circuits,
 sizes and column counts; one stencil width and step count) share one libjja
 plan, and every launch advances the columns of all of them, one workgroup per
 column. Circuits whose face vector fits in LDS keep it there; larger ones use
-the global face store, in a second launch of the same step range.
+the global face store, in the same launch. Workgroups are ordered largest
+problem first, so the hardware's in-order dispatch is a longest-first list
+schedule.
 
 ``autotune=True`` picks the workgroup size and the launch length by measuring:
 each candidate workgroup size is timed on a short run of the batch (the factors
@@ -14,20 +16,66 @@ the shortest whose measured launch overhead stays under 1% of its run time.
 """
 from __future__ import annotations
 
+import multiprocessing
+import os
 import time
+from concurrent.futures import ProcessPoolExecutor
 
 import numpy as np
 
-from .engine import BatchEngine, _torch
+from .engine import BatchEngine, _torch, default_block_size
 from .solver_plan import CircuitPlan
 from .time_evolution import TimeEvolutionProblem, _finish, _masks
 
-__all__ = ["TimeEvolutionBatch"]
+__all__ = ["TimeEvolutionBatch", "build_plans"]
+
+
+_THREAD_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "BLIS_NUM_THREADS")
+
+
+def _plan_one(args):
+    circuit, dt, sw, leaf, bs, store = args
+    return CircuitPlan(circuit, dt, sw, leaf_size=leaf, block_size=bs, face_store=store)
+
+
+def build_plans(problems, block_size, workers=None):
+    """Host plans (ordering, factor, record layouts) of the problems' circuits,
+    built in up to ``workers`` processes (default: the CPUs this process may run
+    on, at most 16; one process when the batch is small). Worker processes
+    only run host code; they are started fresh (spawn), never forked from a
+    process that may hold a device context."""
+    jobs = [(p.circuit, p._dt(), p.stencil_width, p.leaf_size, block_size,
+             getattr(p, "face_store", "auto")) for p in problems]
+    size = sum(p.circuit._Nj() for p in problems)
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)), len(jobs)) if size > 200_000 else 1
+    if workers <= 1:
+        return [_plan_one(j) for j in jobs]
+    # largest first, so the pool's tail is short
+    order = sorted(range(len(jobs)), key=lambda i: -jobs[i][0]._Nj())
+    ctx = multiprocessing.get_context("spawn")
+    # one thread per worker: the workers' BLAS and OpenMP pools would otherwise
+    # each take every CPU (workers read this environment when they start)
+    saved = {k: os.environ.get(k) for k in _THREAD_VARS}
+    os.environ.update({k: "1" for k in _THREAD_VARS})
+    try:
+        with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+            built = list(ex.map(_plan_one, [jobs[i] for i in order]))
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    out = [None] * len(jobs)
+    for i, plan in zip(order, built):
+        out[i] = plan
+    return out
 
 
 class TimeEvolutionBatch:
     def __init__(self, problems, block_size=None, steps_per_launch=None, autotune=False,
-                 block_sizes=(256, 512, 1024), tune_steps=40):
+                 block_sizes=(256, 512, 1024), tune_steps=40, workers=None):
         self.problems = list(problems)
         if not self.problems:
             raise ValueError("TimeEvolutionBatch needs at least one problem")
@@ -38,6 +86,7 @@ class TimeEvolutionBatch:
         self.autotune = autotune
         self.block_sizes = tuple(block_sizes)
         self.tune_steps = int(tune_steps)
+        self.workers = workers
         self.tuning = None
 
     # --------------------------------------------------------------- tuning
@@ -48,9 +97,7 @@ class TimeEvolutionBatch:
         """Time each candidate workgroup size on tune_steps steps; returns
         (block_size, hosts, record)."""
         torch = _torch()
-        base = [CircuitPlan(p.circuit, p._dt(), p.stencil_width, leaf_size=p.leaf_size,
-                            block_size=self.block_sizes[0],
-                            face_store=getattr(p, "face_store", "auto")) for p in self.problems]
+        base = build_plans(self.problems, self.block_sizes[0], self.workers)
         steps = min(self.tune_steps, self.problems[0]._Nt())
         record = {}
         best = None
@@ -90,24 +137,39 @@ class TimeEvolutionBatch:
         return int(min(4000, max(100, 100 * fixed / step)))
 
     # ---------------------------------------------------------------- run
-    def compute(self):
-        masks = [_masks(p) for p in self.problems]
-        core_masks = [(m[2], m[3]) for m in masks]
-        hosts = None
+    def make_engine(self, masks=None):
+        """The batch's BatchEngine at step 0, tuned when autotune is set (the
+        caller advances and closes it). masks: per problem (theta, current)
+        store masks over the time steps; by default each problem's own."""
+        if masks is None:
+            masks = [_masks(p)[2:4] for p in self.problems]
         if self.autotune:
-            bs, hosts, record = self._tune(core_masks)
+            bs, hosts, record = self._tune(masks)
             self.block_size = bs
             self.tuning = {"block_size_s_per_step": record, "block_size": bs}
-        eng = BatchEngine(self.problems, core_masks, block_size=self.block_size, hosts=hosts)
+        else:
+            self.block_size = self.block_size or max(
+                default_block_size(p.circuit._Nj(), p.circuit._Nf()) for p in self.problems)
+            hosts = build_plans(self.problems, self.block_size, self.workers)
+        eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=hosts)
         try:
             if self.autotune and not self.steps_per_launch:
                 spl = self._launch_length(eng)
                 eng.close()
-                eng = BatchEngine(self.problems, core_masks, block_size=self.block_size, hosts=hosts)
+                eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=eng.hosts)
                 eng.steps_per_launch = spl
                 self.tuning["steps_per_launch"] = spl
             elif self.steps_per_launch:
                 eng.steps_per_launch = int(self.steps_per_launch)
+        except Exception:
+            eng.close()
+            raise
+        return eng
+
+    def compute(self):
+        masks = [_masks(p) for p in self.problems]
+        eng = self.make_engine([(m[2], m[3]) for m in masks])
+        try:
             eng.advance(eng.Nt)
             cores = eng.results()
         finally:

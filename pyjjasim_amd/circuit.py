@@ -100,6 +100,13 @@ class Circuit:
         self._Msq_factorized = None
         self._Asq_factorized = None
 
+    def __getstate__(self):
+        # cached factorisations are rebuilt on demand, not pickled
+        state = dict(self.__dict__)
+        state["_Msq_factorized"] = None
+        state["_Asq_factorized"] = None
+        return state
+
     # ------------------------------------------------------------- counts
     def _Nn(self):
         return self.graph.node_count()

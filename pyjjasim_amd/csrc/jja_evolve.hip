@@ -638,16 +638,10 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
 }
 
+// n_steps steps of column w of one problem (one workgroup).
 template <int BS, int SW, bool GLOBAL>
-__global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict__ probs,
-                                                    const jja_problem_run* __restrict__ runs,
-                                                    const WorkItem* __restrict__ work,
-                                                    int32_t n_steps, int64_t step0) {
-  extern __shared__ double lds[];
-  const WorkItem wi = work[blockIdx.x];
-  const DevProblem& P = probs[wi.problem];
-  const jja_problem_run& R = runs[wi.problem];
-  const int64_t w = wi.column;
+__device__ __forceinline__ void evolve_column(const DevProblem& P, const jja_problem_run& R, int64_t w,
+                                              double* lds, int32_t n_steps, int64_t step0) {
   // face vector by solve position, stage scratch after it
   FaceStore<GLOBAL> zs;
   if constexpr (GLOBAL)
@@ -670,6 +664,28 @@ __global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict
     // finish step t and prepare step t+1 in one pass
     junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     __syncthreads();
+  }
+}
+
+// One workgroup per work item (problem, column). STORE: 0 face stores in LDS,
+// 1 in global memory, 2 either, as each problem's index width says (a batch of
+// small and large circuits in one launch).
+template <int BS, int SW, int STORE>
+__global__ __launch_bounds__(BS) void evolve_kernel(const DevProblem* __restrict__ probs,
+                                                    const jja_problem_run* __restrict__ runs,
+                                                    const WorkItem* __restrict__ work,
+                                                    int32_t n_steps, int64_t step0) {
+  extern __shared__ double lds[];
+  const WorkItem wi = work[blockIdx.x];
+  const DevProblem& P = probs[wi.problem];
+  const jja_problem_run& R = runs[wi.problem];
+  if constexpr (STORE == 2) {
+    if (P.index_bytes == 4)
+      evolve_column<BS, SW, true>(P, R, wi.column, lds, n_steps, step0);
+    else
+      evolve_column<BS, SW, false>(P, R, wi.column, lds, n_steps, step0);
+  } else {
+    evolve_column<BS, SW, STORE == 1>(P, R, wi.column, lds, n_steps, step0);
   }
 }
 
@@ -814,6 +830,7 @@ struct Plan {
   std::vector<DevProblem> problems;
   std::vector<int64_t> lds_bytes;     // per problem; 0 for a global face store
   std::vector<int64_t> zbuf_doubles;  // per problem; 0 for an LDS face store
+  std::vector<int64_t> cost;          // per problem: entries one step of a column reads
   DevProblem* d_problems = nullptr;
   jja_problem_run* d_runs = nullptr;
   WorkItem* d_work = nullptr;
@@ -942,10 +959,10 @@ int validate(const jja_problem_desc& d) {
   return JJA_OK;
 }
 
-template <int BS, int SW, bool GLOBAL>
+template <int BS, int SW, int STORE>
 int launch_evolve(Plan* plan, const WorkItem* work, int32_t n_work, int64_t lds, int32_t n_steps,
                   int64_t step0, hipStream_t stream) {
-  auto* kern = evolve_kernel<BS, SW, GLOBAL>;
+  auto* kern = evolve_kernel<BS, SW, STORE>;
   // the solve reads LDS at the plan's absolute byte offsets: the dynamic LDS
   // array must start at address 0, i.e. the kernel must have no static LDS
   hipFuncAttributes attr;
@@ -983,8 +1000,7 @@ int launch_vortex(Plan* plan, int32_t p, const double* theta, int32_t cols, int3
   return JJA_OK;
 }
 
-// Work items of one launch: the columns of the problems whose face store is in
-// LDS first (n_lds of them), then those with a global face store.
+// Copies the work items of one launch to the device (grows the array as needed).
 int upload_work(Plan* plan, const std::vector<WorkItem>& work, hipStream_t stream) {
   if (work.size() > plan->work_cap) {
     if (plan->d_work) {
@@ -1004,26 +1020,36 @@ int upload_work(Plan* plan, const std::vector<WorkItem>& work, hipStream_t strea
   return JJA_OK;
 }
 
-template <int BS, bool GLOBAL>
+template <int BS, int STORE>
 int launch_evolve_sw(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, int32_t n_steps,
                      int64_t step0, hipStream_t stream) {
   switch (plan->stencil_width) {
-    case 3: return launch_evolve<BS, 3, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
-    case 4: return launch_evolve<BS, 4, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
-    default: return launch_evolve<BS, 5, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    case 3: return launch_evolve<BS, 3, STORE>(plan, work, n, lds, n_steps, step0, stream);
+    case 4: return launch_evolve<BS, 4, STORE>(plan, work, n, lds, n_steps, step0, stream);
+    default: return launch_evolve<BS, 5, STORE>(plan, work, n, lds, n_steps, step0, stream);
   }
 }
 
-template <bool GLOBAL>
+template <int STORE>
 int launch_evolve_bs(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, int32_t n_steps,
                      int64_t step0, hipStream_t stream) {
   switch (plan->block_size) {
-    case 64: return launch_evolve_sw<64, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
-    case 128: return launch_evolve_sw<128, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
-    case 256: return launch_evolve_sw<256, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
-    case 512: return launch_evolve_sw<512, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
-    default: return launch_evolve_sw<1024, GLOBAL>(plan, work, n, lds, n_steps, step0, stream);
+    case 64: return launch_evolve_sw<64, STORE>(plan, work, n, lds, n_steps, step0, stream);
+    case 128: return launch_evolve_sw<128, STORE>(plan, work, n, lds, n_steps, step0, stream);
+    case 256: return launch_evolve_sw<256, STORE>(plan, work, n, lds, n_steps, step0, stream);
+    case 512: return launch_evolve_sw<512, STORE>(plan, work, n, lds, n_steps, step0, stream);
+    default: return launch_evolve_sw<1024, STORE>(plan, work, n, lds, n_steps, step0, stream);
   }
+}
+
+// The work items of a launch, largest problems first: workgroups are
+// dispatched in index order as compute units free up, so this is a
+// longest-first list schedule of uneven problems (the sort is stable: the
+// columns of a problem, and problems of equal cost, keep their order).
+void order_work(const Plan* plan, std::vector<WorkItem>& work) {
+  std::stable_sort(work.begin(), work.end(), [plan](const WorkItem& a, const WorkItem& b) {
+    return plan->cost[a.problem] > plan->cost[b.problem];
+  });
 }
 
 template <bool GLOBAL>
@@ -1120,6 +1146,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     plan->problems.push_back(dp);
     const int64_t store = (static_cast<int64_t>(d.n_faces) + d.max_stage_rows) * static_cast<int64_t>(sizeof(double));
     plan->lds_bytes.push_back(d.index_bytes == 2 ? store : 0);
+    plan->cost.push_back(d.n_entries + d.n_face_entries + 8 * static_cast<int64_t>(d.n_junctions));
     plan->zbuf_doubles.push_back(d.index_bytes == 2 ? 0 : static_cast<int64_t>(d.n_faces) + d.max_stage_rows);
   }
   if (rc == JJA_OK) {
@@ -1191,15 +1218,16 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
   // pageable source: the call returns once the data is staged, and the copy is
   // ordered after earlier work on the stream
   JJA_HIP(hipMemcpyAsync(plan->d_runs, runs, sizeof(jja_problem_run) * np, hipMemcpyHostToDevice, stream));
+  // one launch: a kernel for one kind of face store, or one that takes either
   const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
   work.insert(work.end(), work_g.begin(), work_g.end());
+  order_work(plan, work);
   if (int rc = upload_work(plan, work, stream)) return rc;
   if (lds < 8) lds = 8;
-  if (n_lds)
-    if (int rc = launch_evolve_bs<false>(plan, plan->d_work, n_lds, lds, n_steps, step0, stream)) return rc;
-  if (n_g)
-    if (int rc = launch_evolve_bs<true>(plan, plan->d_work + n_lds, n_g, 8, n_steps, step0, stream)) return rc;
-  return JJA_OK;
+  const int32_t n = n_lds + n_g;
+  if (n_g == 0) return launch_evolve_bs<0>(plan, plan->d_work, n, lds, n_steps, step0, stream);
+  if (n_lds == 0) return launch_evolve_bs<1>(plan, plan->d_work, n, 8, n_steps, step0, stream);
+  return launch_evolve_bs<2>(plan, plan->d_work, n, lds, n_steps, step0, stream);
 }
 
 int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t k_lo, void* stream_ptr) {
@@ -1233,6 +1261,8 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   JJA_HIP(hipMemcpyAsync(plan->d_adj, runs, sizeof(jja_adjoint_run) * np, hipMemcpyHostToDevice, stream));
   const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
+  order_work(plan, work);
+  order_work(plan, work_g);
   work.insert(work.end(), work_g.begin(), work_g.end());
   if (int rc = upload_work(plan, work, stream)) return rc;
   if (lds < 8) lds = 8;

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import copy
 import ctypes as C
+import heapq
 
 import numpy as np
 import scipy.linalg
@@ -476,71 +477,74 @@ class CircuitPlan:
         work gets a no-op record), so all waves meet at every phase boundary.
         Record data: value `it` of lane l at base + (it/2)*128 + 2l + it%2, LDS
         index `it` (in doubles) of lane l at ibase + 8l + it, row of lane l at
-        rowbase + l/G. Block 0 of every array is zero (the no-op record).
+        rowbase + l/G. Block 0 of every array is zero (the no-op record); record
+        r of the data arrays has base = ibase = 512 r.
         """
         NW = self.block_size // WAVE
         vals = [np.zeros(512)]
         offs = [np.zeros(512, dtype=np.int64)]
         rows_out = [-np.ones(WAVE, dtype=np.int64)]
-        nv, ni, nr = 512, 512, WAVE
+        n_rec, nr = 1, WAVE
         noop = (0, 0, 0, 0, 0, 1)
         progs = [[] for _ in range(NW)]
-        lane = np.arange(WAVE)
         for (r0, rows, ptr, idx, val, kflag) in phases:
             ptr = np.asarray(ptr, dtype=np.int64)
+            rows = np.asarray(rows, dtype=np.int64)
             lens = np.diff(ptr)
             need = np.maximum(1, -(-lens // target_iter))
             Gsel = np.minimum(WAVE, 2 ** np.ceil(np.log2(need)).astype(np.int64))
-            chains = []
+            chains = []   # (g, rows, records, first record, rowbase, iterations)
             for g in sorted(set(Gsel.tolist())):
                 sel = np.nonzero(Gsel == g)[0]
                 sel = sel[np.argsort(-lens[sel], kind="stable")]
                 R = WAVE // g
-                for i in range(0, len(sel), R):
-                    rr = sel[i:i + R]
-                    n_it = -(-int(lens[rr[0]]) // g)
-                    nch = max(1, -(-n_it // MAX_ITER))
-                    v = np.zeros((nch, MAX_ITER, WAVE))
-                    o = np.zeros((nch, MAX_ITER, WAVE), dtype=np.int64)
-                    L = lens[rr]
-                    slot = np.repeat(np.arange(len(rr)), L)
-                    e = np.arange(int(L.sum())) - np.repeat(np.cumsum(L) - L, L)
-                    src = np.repeat(ptr[rr], L) + e
-                    k = e // g
-                    ln = slot * g + e % g
-                    v[k // MAX_ITER, k % MAX_ITER, ln] = val[src]
-                    o[k // MAX_ITER, k % MAX_ITER, ln] = idx[src]
-                    rws = -np.ones(WAVE, dtype=np.int64)
-                    rws[:len(rr)] = rows[rr]
-                    chains.append((g, len(rr), nch, v, o, rws, n_it))
+                n_ch = -(-len(sel) // R)
+                pos = np.arange(len(sel))
+                n_it = -(-lens[sel[::R]] // g)             # the chain's first row is its longest
+                nch = np.maximum(1, -(-n_it // MAX_ITER))
+                rec0 = np.cumsum(nch) - nch
+                v = np.zeros((int(nch.sum()), MAX_ITER, WAVE))
+                o = np.zeros(v.shape, dtype=np.int64)
+                L = lens[sel]
+                who = np.repeat(pos, L)
+                e = np.arange(int(L.sum())) - np.repeat(np.cumsum(L) - L, L)
+                src = np.repeat(ptr[sel], L) + e
+                k = e // g
+                rec = rec0[who // R] + k // MAX_ITER
+                ln = (who % R) * g + e % g
+                v[rec, k % MAX_ITER, ln] = val[src]
+                o[rec, k % MAX_ITER, ln] = idx[src]
+                # record layouts: value it of lane l at (it/2)*128 + 2l + it%2,
+                # index it of lane l at 8l + it
+                vals.append(v.reshape(-1, MAX_ITER // 2, 2, WAVE).transpose(0, 1, 3, 2).reshape(-1))
+                offs.append(o.transpose(0, 2, 1).reshape(-1))
+                rws = -np.ones(n_ch * R, dtype=np.int64)
+                rws[:len(sel)] = rows[sel]
+                rows_out.append(rws)
+                nrow = np.minimum(R, len(sel) - np.arange(n_ch) * R)
+                for c in range(n_ch):
+                    chains.append((g, int(nrow[c]), int(nch[c]), n_rec + int(rec0[c]), nr + c * R,
+                                   int(n_it[c])))
+                n_rec += int(nch.sum())
+                nr += n_ch * R
             order = sorted(range(len(chains)), key=lambda c: -chains[c][2])
-            load = np.zeros(NW, dtype=np.int64)
+            # least-loaded wave first, the lowest index among equals
+            heap = [(0, w) for w in range(NW)]
             assign = [[] for _ in range(NW)]
             for c in order:
-                w = int(np.argmin(load))
-                load[w] += chains[c][2]
+                load, w = heapq.heappop(heap)
                 assign[w].append(c)
+                heapq.heappush(heap, (load + chains[c][2], w))
             for w in range(NW):
                 recs = []
                 for c in assign[w]:
-                    g, nrow, nch, v, o, rws, n_it = chains[c]
-                    rowbase = nr
-                    rows_out.append(rws[:WAVE // g])
-                    nr += WAVE // g
+                    g, nrow, nch, first, rowbase, n_it = chains[c]
                     for ch in range(nch):
-                        vb = np.zeros(512)
-                        ob = np.zeros(512, dtype=np.int64)
-                        for it in range(MAX_ITER):
-                            vb[(it // 2) * 128 + 2 * lane + it % 2] = v[ch, it]
-                            ob[8 * lane + it] = o[ch, it]
-                        vals.append(vb)
-                        offs.append(ob)
                         flags = g | kflag | (nrow << 16)
                         if ch == nch - 1:
                             flags |= _native.SLICE_FINISH
-                        recs.append((nv, ni, rowbase, r0, min(MAX_ITER, n_it - ch * MAX_ITER), flags))
-                        nv += 512
-                        ni += 512
+                        b = 512 * (first + ch)
+                        recs.append((b, b, rowbase, r0, min(MAX_ITER, n_it - ch * MAX_ITER), flags))
                 if not recs:
                     recs.append(noop)
                 b_, ib_, rb_, r0_, it_, fl_ = recs[-1]
@@ -565,6 +569,18 @@ class CircuitPlan:
     def wave_program(self, w):
         """Executed records of wave w (the four lookahead records stripped)."""
         return self.solve_slices[int(self.wave_ptr[w]):int(self.wave_ptr[w + 1]) - 4]
+
+    def __getstate__(self):
+        # the ctypes record arrays are rebuilt from the record lists
+        state = dict(self.__dict__)
+        state.pop("_face_rec", None)
+        state.pop("_solve_rec", None)
+        return state
+
+    def __setstate__(self, state):
+        self.__dict__.update(state)
+        self._face_rec = self._slice_array(self.face_slices)
+        self._solve_rec = self._slice_array(self.solve_slices)
 
     @staticmethod
     def _slice_array(recs):

@@ -182,3 +182,26 @@ def test_relayout_plan_validates_and_solves(bs):
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), a.get_cycle_matrix() @ u)
     np.testing.assert_allclose(p2.emulate_face_and_solve(u), want, rtol=1e-12,
                                atol=1e-12 * abs(want).max())
+
+
+def test_plans_pickle_and_build_in_worker_processes():
+    """Plans built in worker processes equal those built here, record arrays
+    included (they travel by pickle and are rebuilt on arrival)."""
+    import pickle
+    import ctypes as C
+    from pyjjasim_amd import _native
+    from pyjjasim_amd.batch import build_plans
+    circuits = [pa.SquareArray(12, 9), pa.HoneycombArray(6, 5), pa.TriangularArray(7, 7)]
+    # current_base caches a SuperLU factor on the circuit: it must not travel
+    probs = [pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=10,
+                                     current_sources=c.current_base(0.0)[:, None, None]) for c in circuits]
+    here = build_plans(probs, 256, workers=1)
+    there = build_plans(probs, 256, workers=2)
+    for a, b in zip(here, there):
+        for name in ("ent_val", "ent_off", "slice_rows", "face_code_sliced", "junc_pos_code", "coef"):
+            np.testing.assert_array_equal(getattr(a, name), getattr(b, name))
+        assert a.solve_slices == b.solve_slices and a.face_slices == b.face_slices
+        assert bytes(a._solve_rec) == bytes(b._solve_rec)
+        assert _native.lib().jja_plan_validate(1, C.byref(b.desc())) == 0
+        c2 = pickle.loads(pickle.dumps(a))
+        assert bytes(c2._face_rec) == bytes(a._face_rec)
