@@ -53,6 +53,10 @@ int fail(int code, const std::string& msg) {
 constexpr double kTwoPi = 6.283185307179586;  // 2 * np.pi
 constexpr int kMaxIter = 8;                   // entries per lane and record
 constexpr int kWave = 64;                     // lanes of a wavefront (solve records are per wave)
+// buffer resource word 3 for gfx9-family raw buffers (no format conversion),
+// and a byte offset beyond any resource range (ranges stay below 2^31)
+constexpr int kBufferWord3 = 0x00020000;
+constexpr uint32_t kOutOfRange = 0x80000000u;
 
 // Pointers read out of the descriptor structs are generic; casting them to the
 // global (1) or constant (4) address space lets the compiler use global and
@@ -92,7 +96,8 @@ __device__ __forceinline__ jja_slice load_rec(const jja_slice* prog, int32_t k) 
 struct DevProblem {
   int32_t Nj, Nf, sw, BS;
   int32_t n_face_slices, n_solve_slices, max_stage_rows, copy_rows;
-  int32_t index_bytes, pad_;
+  int32_t index_bytes;
+  int32_t val_bytes;        // bytes of ent_val (below 2^31: the solve reads it through a buffer resource)
   const double* Ic;
   const double* Rv;
   const double* coef;
@@ -232,17 +237,27 @@ __device__ __forceinline__ uint32_t slice_index(const SliceRegs<GLOBAL>& r, int 
 // Issue the loads of one solve record: always the same vector loads, each a
 // scalar base (from the record) plus a per-lane offset that does not change
 // between records, so no address arithmetic runs per record.
+// The record values go through a buffer resource over ent_val: a value pair at
+// or past the record's iteration count is read at an offset beyond the
+// resource's range, which returns zeros without a memory access, so a short
+// record moves only the bytes it uses and every record issues the same loads.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t value_resource(const DevProblem& P) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(P.ent_val), 0, P.val_bytes, kBufferWord3);
+}
+
 template <bool GLOBAL>
-__device__ __forceinline__ void load_slice(const DevProblem& P, const jja_slice& m, int lane,
-                                           SliceRegs<GLOBAL>& r) {
+__device__ __forceinline__ void load_slice(const DevProblem& P, __amdgpu_buffer_rsrc_t vres,
+                                           const jja_slice& m, int lane, SliceRegs<GLOBAL>& r) {
   const int lg = __builtin_ctz(static_cast<unsigned>(m.flags & 0xff));
-  const GAS vdouble2* __restrict__ vp = gbl(reinterpret_cast<const vdouble2*>(P.ent_val + m.base)) + lane;
+  const uint32_t voff = static_cast<uint32_t>(m.base) * 8u + static_cast<uint32_t>(lane) * 16u;
   // the raw row; whether this lane's slot holds one is decided when the record
   // is processed (a select here would wait for the load)
   r.row = gbl(P.slice_rows)[m.rowbase + (lane >> lg)];
 #pragma unroll
   for (int q = 0; q < kMaxIter / 2; ++q) {
-    const vdouble2 d = vp[q * kWave];
+    const uint32_t off = 2 * q < m.n_iter ? voff + q * 16u * kWave : kOutOfRange;
+    const vdouble2 d = __builtin_bit_cast(
+        vdouble2, __builtin_amdgcn_raw_buffer_load_b128(vres, static_cast<int>(off), 0, 0));
     r.v[2 * q] = d.x;
     r.v[2 * q + 1] = d.y;
   }
@@ -308,21 +323,22 @@ __device__ __forceinline__ void solve(const DevProblem& P, const FaceStore<GLOBA
   const int32_t nk = __builtin_amdgcn_readfirstlane(wp[1]) - k0 - 4;
   const jja_slice* __restrict__ prog = P.solve_slices + k0;
   jja_slice m0 = load_rec(prog, 0), m1 = load_rec(prog, 1), m2 = load_rec(prog, 2);
+  const __amdgpu_buffer_rsrc_t vres = value_resource(P);
   SliceRegs<GLOBAL> A, B, C;
-  load_slice(P, m0, lane, A);
-  load_slice(P, m1, lane, B);
+  load_slice(P, vres, m0, lane, A);
+  load_slice(P, vres, m1, lane, B);
   double acc = 0.0;
   for (int32_t k = 0; k < nk; k += 3) {
     const jja_slice m3 = load_rec(prog, k + 3);
-    load_slice(P, m2, lane, C);
+    load_slice(P, vres, m2, lane, C);
     __builtin_amdgcn_sched_barrier(0);
     process_slice(zs, P.Nf, lane, m0, A, acc);
     const jja_slice m4 = load_rec(prog, k + 4);
-    load_slice(P, m3, lane, A);
+    load_slice(P, vres, m3, lane, A);
     __builtin_amdgcn_sched_barrier(0);
     process_slice(zs, P.Nf, lane, m1, B, acc);
     const jja_slice m5 = load_rec(prog, k + 5);
-    load_slice(P, m4, lane, B);
+    load_slice(P, vres, m4, lane, B);
     __builtin_amdgcn_sched_barrier(0);
     process_slice(zs, P.Nf, lane, m2, C, acc);
     m0 = m3;
@@ -920,6 +936,8 @@ int validate(const jja_problem_desc& d) {
     return fail(JJA_EINVAL, "solve_wave_ptr must cover the solve records");
   if (d.n_entries < kMaxIter * kWave || d.n_index < kMaxIter * kWave)
     return fail(JJA_EINVAL, "solve arrays too short");
+  if (d.n_entries * static_cast<int64_t>(sizeof(double)) >= (int64_t{1} << 31))
+    return fail(JJA_EUNSUPPORTED, "solve values of one problem exceed 2 GiB");
   if (d.face_copy_rows < 0 || d.face_copy_rows > Nf || d.face_copy_rows > ms)
     return fail(JJA_EINVAL, "face_copy_rows out of range");
   const int64_t lds_end = static_cast<int64_t>(Nf) + ms;
@@ -1116,6 +1134,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.max_stage_rows = d.max_stage_rows;
     dp.copy_rows = d.face_copy_rows;
     dp.index_bytes = d.index_bytes;
+    dp.val_bytes = static_cast<int32_t>(d.n_entries * static_cast<int64_t>(sizeof(double)));
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
