@@ -51,7 +51,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr double kTwoPi = 6.283185307179586;  // 2 * np.pi
-constexpr int kMaxIter = 8;                   // entries per lane and record
+constexpr int kMaxIter = 8;                   // entries per lane and solve record
+constexpr int kFaceIter = 4;                  // junction codes per thread and face record
+constexpr int kFaceGroup = 4;                 // face records whose loads are issued together
 constexpr int kWave = 64;                     // lanes of a wavefront (solve records are per wave)
 // buffer resource word 3 for gfx9-family raw buffers (no format conversion),
 // and a byte offset beyond any resource range (ranges stay below 2^31)
@@ -111,6 +113,7 @@ struct DevProblem {
   const void* ent_off;      // LDS indices: uint16 (face store in LDS) or uint32 (in global memory)
   const int32_t* wave_ptr;  // [BS/64 + 1]
   const double* zero;       // 8 zero doubles: absent input terms read here
+  const int32_t* face_fid;  // [n_slice_rows]: face index of a face record's row slot, or -1
 };
 
 struct WorkItem {
@@ -285,9 +288,11 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, __amdgpu_buffer_
 template <bool GLOBAL>
 __device__ __forceinline__ void process_slice(const FaceStore<GLOBAL>& zs, int32_t Nf, int lane,
                                               const jja_slice& m, const SliceRegs<GLOBAL>& r, double& acc) {
+  // iterations at or past the record's count have zero values: their gathers
+  // are skipped (a uniform branch) and contribute an exact zero product
   double zv[kMaxIter];
 #pragma unroll
-  for (int i = 0; i < kMaxIter; ++i) zv[i] = zs.gather(slice_index(r, i));
+  for (int i = 0; i < kMaxIter; ++i) zv[i] = (i < 4 || i < m.n_iter) ? zs.gather(slice_index(r, i)) : 0.0;
 #pragma unroll
   for (int i = 0; i < kMaxIter; ++i) acc = __builtin_fma(r.v[i], zv[i], acc);
   if (m.flags & JJA_SLICE_FINISH) {
@@ -347,40 +352,6 @@ __device__ __forceinline__ void solve(const DevProblem& P, const FaceStore<GLOBA
   }
 }
 
-// Face phase: b_p = sum +-u_j in ascending j, minus 2 pi f (time_evolution.py:560-569).
-// A face longer than one record continues in the next record (flag FINISH on
-// its last one).
-template <int BS, class Z>
-__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f,
-                                           const GAS double* __restrict__ u, const Z& zs,
-                                           int64_t w, int64_t t) {
-  const int tid = threadIdx.x;
-  double acc = 0.0;
-  for (int32_t k = 0; k < P.n_face_slices; ++k) {
-    const jja_slice m = load_rec(P.face_slices, k);
-    const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + tid;
-    const bool live = tid < (m.flags >> 16);
-    int32_t cd[kMaxIter];
-#pragma unroll
-    for (int i = 0; i < kMaxIter; ++i) cd[i] = (live && i < m.n_iter) ? code[i * BS] : -1;
-    double uu[kMaxIter];
-#pragma unroll
-    for (int i = 0; i < kMaxIter; ++i) uu[i] = cd[i] >= 0 ? u[cd[i] >> 1] : 0.0;
-#pragma unroll
-    for (int i = 0; i < kMaxIter; ++i)
-      if (cd[i] >= 0) acc = (cd[i] & 1) ? acc - uu[i] : acc + uu[i];
-    if ((m.flags & JJA_SLICE_FINISH) && live) {
-      const int32_t pos = gbl(P.slice_rows)[m.rowbase + tid];
-      if (pos >= 0) {
-        if (f && !f->zero) acc = acc - kTwoPi * src_at(*f, gbl(P.face_of_pos)[pos], w, t);
-        zs.wr(pos, acc);
-        if (pos < P.copy_rows) zs.wr(P.Nf + pos, acc);  // first stage: scratch = b
-      }
-      acc = 0.0;
-    }
-  }
-}
-
 // One input term of one column at one step: element i is p[i * si] * amp.
 // amp is exactly 1 for a term without amplitude; a term that is absent reads
 // an exact +0.0 from the plan's zero page (si = 0), which leaves every sum it
@@ -405,6 +376,70 @@ __device__ __forceinline__ TermView term_view(const jja_source& s, bool present,
 
 __device__ __forceinline__ double term_at(const TermView& v, int32_t i) {
   return v.p[static_cast<int64_t>(i) * v.si] * v.amp;
+}
+
+// Face phase: b_p = sum +-u_j in ascending j, minus 2 pi f (time_evolution.py:560-569).
+// A face longer than one record continues in the next record (flag FINISH on
+// its last one). Records go in groups of kFaceGroup: the code, row and face
+// loads of the group are issued together, then its u gathers and flux values,
+// so a group costs two memory round trips.
+struct FaceRec {
+  int32_t cd[kFaceIter];  // +-junction codes, -1 past the face's length
+  int32_t pos;            // solve position written at FINISH, or -1
+  int32_t fid;            // face index of that position (flux input), or -1
+  bool finish;
+};
+
+template <int BS>
+__device__ __forceinline__ void face_rec_load(const DevProblem& P, int32_t k, bool present, int tid,
+                                              FaceRec& r) {
+  const jja_slice m = load_rec(P.face_slices, k);
+  const bool live = present && tid < (m.flags >> 16);
+  const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + tid;
+#pragma unroll
+  for (int i = 0; i < kFaceIter; ++i) r.cd[i] = (live && i < m.n_iter) ? code[i * BS] : -1;
+  r.finish = present && (m.flags & JJA_SLICE_FINISH);
+  const bool rowed = live && r.finish;
+  r.pos = rowed ? gbl(P.slice_rows)[m.rowbase + tid] : -1;
+  r.fid = rowed ? gbl(P.face_fid)[m.rowbase + tid] : -1;
+}
+
+template <int BS, class Z>
+__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f,
+                                           const GAS double* __restrict__ u, const Z& zs,
+                                           int64_t w, int64_t t) {
+  const int tid = threadIdx.x;
+  // flux of face q at this step is fv.p[q * fv.si] * fv.amp; an absent flux
+  // reads an exact zero, and acc - 2 pi * 0 leaves acc unchanged
+  const TermView fv = term_view(f ? *f : jja_source{}, f && !f->zero, P.zero, w, t);
+  double acc = 0.0;
+  const int32_t n = P.n_face_slices;
+  for (int32_t k = 0; k < n; k += kFaceGroup) {
+    FaceRec r[kFaceGroup];
+#pragma unroll
+    for (int g = 0; g < kFaceGroup; ++g) face_rec_load<BS>(P, k + g < n ? k + g : k, k + g < n, tid, r[g]);
+    double uu[kFaceGroup][kFaceIter], fq[kFaceGroup];
+#pragma unroll
+    for (int g = 0; g < kFaceGroup; ++g) {
+#pragma unroll
+      for (int i = 0; i < kFaceIter; ++i) uu[g][i] = r[g].cd[i] >= 0 ? u[r[g].cd[i] >> 1] : 0.0;
+      fq[g] = r[g].fid >= 0 ? term_at(fv, r[g].fid) : 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < kFaceGroup; ++g) {
+#pragma unroll
+      for (int i = 0; i < kFaceIter; ++i)
+        if (r[g].cd[i] >= 0) acc = (r[g].cd[i] & 1) ? acc - uu[g][i] : acc + uu[g][i];
+      if (r[g].finish) {
+        if (r[g].pos >= 0) {
+          acc = acc - kTwoPi * fq[g];
+          zs.wr(r[g].pos, acc);
+          if (r[g].pos < P.copy_rows) zs.wr(P.Nf + r[g].pos, acc);  // first stage: scratch = b
+        }
+        acc = 0.0;
+      }
+    }
+  }
 }
 
 // Everything one junction pass needs that is the same for all its junctions.
@@ -921,6 +956,7 @@ int validate(const jja_problem_desc& d) {
     const jja_slice& m = d.face_slices[k];
     if (int rc = check_slice(m, d.n_face_entries, false)) return rc;
     if ((m.flags & 0xff) != 1) return fail(JJA_EINVAL, "face slices use one lane per row");
+    if (m.n_iter > kFaceIter) return fail(JJA_EINVAL, "face slices hold at most 4 codes per thread");
     for (int32_t i = 0; i < BS; ++i) {
       const int32_t r = d.slice_rows[m.rowbase + i];
       if (r < -1 || r >= Nf) return fail(JJA_EINVAL, "face slice row out of range");
@@ -1147,6 +1183,19 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       if ((rc = upload(plan, d.solve_wave_ptr, static_cast<size_t>(d.block_size / kWave + 1), &dp.wave_ptr))) break;
       if ((rc = upload(plan, d.face_code, static_cast<size_t>(d.n_face_entries), &dp.face_code))) break;
       if ((rc = upload(plan, d.slice_rows, static_cast<size_t>(d.n_slice_rows), &dp.slice_rows))) break;
+      {
+        // face index of every row slot a face record finishes (the flux input)
+        std::vector<int32_t> fid(static_cast<size_t>(d.n_slice_rows), -1);
+        for (int32_t k = 0; k < d.n_face_slices; ++k) {
+          const jja_slice& m = d.face_slices[k];
+          if (!(m.flags & JJA_SLICE_FINISH)) continue;
+          for (int32_t i = 0; i < (m.flags >> 16); ++i) {
+            const int32_t pos = d.slice_rows[m.rowbase + i];
+            if (pos >= 0) fid[static_cast<size_t>(m.rowbase) + i] = d.face_of_pos[pos];
+          }
+        }
+        if ((rc = upload(plan, fid.data(), fid.size(), &dp.face_fid))) break;
+      }
       if ((rc = upload(plan, d.ent_val, static_cast<size_t>(d.n_entries), &dp.ent_val))) break;
       if (d.index_bytes == 2) {
         const uint16_t* o = nullptr;

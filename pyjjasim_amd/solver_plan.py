@@ -26,6 +26,7 @@ from . import _native
 from .ordering import nested_dissection
 
 MAX_ITER = 8          # entries per lane and solve record (kernel register budget)
+FACE_ITER = 4         # junction codes per thread and face record
 LDS_BYTES = 163840    # dynamic LDS one workgroup may use on MI355X (gfx950)
 WAVE = 64             # lanes of one wavefront: solve records are per wave
 
@@ -375,22 +376,22 @@ class CircuitPlan:
         rows_all = []
         n_rows = 0
 
-        def records(L, ent_offset, r0, kind_flags, barrier):
-            """Split slices into <= MAX_ITER-entry records (one row may span several)."""
+        def records(L, ent_offset, r0, kind_flags, barrier, max_iter=MAX_ITER):
+            """Split slices into <= max_iter-entry records (one row may span several)."""
             nonlocal n_rows
             out = []
             for k in range(len(L["base"])):
                 it, G = int(L["iters"][k]), int(L["group"][k])
                 R = BS // G
                 live = int(np.count_nonzero(L["rows"][L["rowbase"][k]:L["rowbase"][k] + R] >= 0))
-                n_chunks = max(1, -(-it // MAX_ITER))
+                n_chunks = max(1, -(-it // max_iter))
                 for c in range(n_chunks):
                     flags = G | kind_flags | (live << 16)
                     if c == n_chunks - 1:
                         flags |= _native.SLICE_FINISH
-                    out.append((int(L["base"][k]) + ent_offset + c * MAX_ITER * BS,
+                    out.append((int(L["base"][k]) + ent_offset + c * max_iter * BS,
                                 int(L["rowbase"][k]) + n_rows, r0,
-                                min(MAX_ITER, it - c * MAX_ITER), flags))
+                                min(max_iter, it - c * max_iter), flags))
             if barrier and out:
                 b, rb, rr, it, fl = out[-1]
                 out[-1] = (b, rb, rr, it, fl | _native.SLICE_BARRIER)
@@ -398,16 +399,17 @@ class CircuitPlan:
             n_rows += len(L["rows"])
             return out
 
-        # faces by position, one thread each; entries: junction codes (sign in bit 0)
+        # faces in circuit order, one thread each, each written to its solve
+        # position; entries: junction codes (sign in bit 0). Neighbouring faces
+        # share junctions and have nearby junction indices, so the lanes of a
+        # wave gather u from few cache lines.
         fp = self.face_ptr.astype(np.int64)
-        lens = np.diff(fp)[self.face_of_pos] if Nf else np.zeros(0, np.int64)
-        ptr_by_pos = np.concatenate(([0], np.cumsum(lens)))
-        codes = np.concatenate([self.face_code[fp[f]:fp[f + 1]] for f in self.face_of_pos]) \
-            if Nf else np.zeros(0, np.int64)
-        F = sliced(ptr_by_pos, codes, np.ones(len(codes)), BS, np.arange(Nf), fixed_group=1,
-                   target_iter=MAX_ITER)
+        pos_of_face = np.empty(Nf, dtype=np.int64)
+        pos_of_face[self.face_of_pos] = np.arange(Nf)
+        F = sliced(fp, self.face_code.astype(np.int64), np.ones(int(fp[-1]) if Nf else 0), BS,
+                   pos_of_face, fixed_group=1, target_iter=FACE_ITER)
         self.face_code_sliced = np.where(F["val"] != 0, F["idx"], -1).astype(np.int32)
-        self.face_slices = records(F, 0, 0, 0, False)
+        self.face_slices = records(F, 0, 0, 0, False, max_iter=FACE_ITER)
         # solve program: forward stages ascending, backward stages descending
         rb, bsz, boff = sp.row_block.astype(np.int64), sp.row_bsize.astype(np.int64), sp.row_boff
         per_stage = []
