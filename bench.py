@@ -21,6 +21,9 @@ Usage: python bench.py [--gpus 1] [--steps 20000] [--warmup 1000] [--plain-steps
        python bench.py --workload uneven_batch [--steps 10000] [--warmup 200] [--problems 64]
          (64 seeded problems of uneven size, one launch per chunk, autotuned; its own
           JSON line)
+       python bench.py --workload large --steps 200 [--checkpoint 1000]
+         (SquareArray(512,512) W=16 forward + adjoint d loss/d f on a bounded sample of
+          steps; its own JSON line)
 """
 import argparse
 import json
@@ -262,6 +265,57 @@ def uneven_bench(pa, torch, steps, warmup, plain_steps, n_problems=64):
             "host_setup_s": setup_s, "plain": plain}
 
 
+def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000):
+    """Large workload: SquareArray(n, n), W bias points linspace(0.2, 1.2) *
+    current_base(0), beta_C = 1, junction self-inductance beta_L = 0.5, per-face
+    flux f = 0.1 + 0.02 z (seeded), T = 0.05, dt = 0.05; the forward pass and the
+    adjoint gradient d loss / d f (loss: mean time-averaged voltage of the biased
+    junctions), checkpoints every ``checkpoint`` steps. Timed over ``steps``
+    steps (a bounded sample of the full horizon; every step costs the same)."""
+    dt = 0.05
+    t0 = time.perf_counter()
+    c = pa.SquareArray(n, n)
+    c.set_capacitance(1.0)
+    c.set_inductance(0.5)
+    Nj, Nf = c.junction_count(), c.face_count()
+    base = c.current_base(0.0)
+    dev = torch.device("cuda")
+    biased = torch.tensor(np.abs(base) > 0, device=dev)
+    Is = torch.tensor(base[:, None] * np.linspace(0.2, 1.2, W)[None, :], dtype=torch.float64, device=dev)
+    f0 = 0.1 + 0.02 * np.random.default_rng(505).standard_normal(Nf)
+    Ic = torch.ones(Nj, dtype=torch.float64, device=dev)
+    store = np.zeros(steps, dtype=bool)
+    store[-1] = True
+    evo = pa.DifferentiableEvolution(c, time_step=dt, time_step_count=steps, problem_count=W,
+                                     temperature=0.05, store_time_steps=store,
+                                     checkpoint_every=min(checkpoint, steps), seed=7)
+    setup_s = time.perf_counter() - t0
+    f = torch.tensor(f0, dtype=torch.float64, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th = evo(Ic, f, Is)[:, :, -1]
+    loss = th[biased].mean() / (steps * dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = f.grad.detach()
+    fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+    return {"metric": "junction-steps/s (large array forward, FP64)", "value": Nj * W / fwd,
+            "unit": "junction-steps/s", "steps": steps, "dtype": "f64",
+            "config": {"workload": f"large_square{n}_W{W}", "junctions": Nj, "faces": Nf, "W": W,
+                       "beta_L": 0.5, "T": 0.05, "checkpoint": min(checkpoint, steps),
+                       "face_store": "global" if evo.eng.host.global_store else "lds",
+                       "solve_records_per_column_step": int(evo.eng.host.n_solve_exec)},
+            "forward_ms_per_step": fwd * 1e3, "backward_ms_per_step": bwd * 1e3,
+            "grad_over_fwd": bwd / fwd,
+            "full_horizon_estimate_s": {"steps": full_steps, "forward": fwd * full_steps,
+                                        "backward": bwd * full_steps},
+            "grad_f_finite": bool(torch.isfinite(g).all().item()),
+            "host_setup_s": setup_s}
+
+
 def measured_traffic(steps_per_launch):
     """HBM bytes per evolve_kernel launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor), or
@@ -308,9 +362,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--workload", choices=["iv_sweep", "uneven_batch"], default="iv_sweep",
+    ap.add_argument("--workload", choices=["iv_sweep", "uneven_batch", "large"], default="iv_sweep",
                     help="iv_sweep: SquareArray(100,100) W=256 (default); uneven_batch: "
-                         "64 problems of uneven size in one launch per chunk")
+                         "64 problems of uneven size in one launch per chunk; large: "
+                         "SquareArray(512,512) W=16 forward + adjoint on --steps steps")
+    ap.add_argument("--checkpoint", type=int, default=1000, help="large: checkpoint interval")
     ap.add_argument("--problems", type=int, default=64, help="uneven_batch: problem count")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device copy of MIB MiB (counter calibration)")
@@ -326,6 +382,12 @@ def main():
     torch.cuda.set_device(0)
     if args.copy_calibration:
         copy_calibration(torch, args.copy_calibration)
+        return
+    if args.workload == "large":
+        out = large_bench(pa, torch, args.steps, args.checkpoint)
+        out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "data": "synthetic (seeded flux disorder, seeded in-kernel thermal noise)"})
+        print(json.dumps(out), flush=True)
         return
     if args.workload == "uneven_batch":
         out = uneven_bench(pa, torch, args.steps, args.warmup, args.plain_steps, args.problems)

@@ -113,7 +113,7 @@ typedef struct jja_problem_desc {
   int32_t n_junctions;
   int32_t n_faces;
   int32_t stencil_width;    /* 3, 4 or 5 (time_evolution.py:401-408) */
-  int32_t block_size;       /* threads per workgroup the layouts are built for */
+  int32_t block_size;       /* threads per column the layouts are built for (parts workgroups) */
   const double* Ic;         /* [Nj] critical current */
   const double* Rv;         /* [Nj] 1/(dt R) */
   const double* coef;       /* [stencil_width][Nj] c_k = S1[k] Rv + S2[k] Cv (time_evolution.py:477-499) */
@@ -144,7 +144,13 @@ typedef struct jja_problem_desc {
    *    indices; each lane's 8 indices are two 16-byte loads), for circuits whose
    *    face vector does not fit in LDS. */
   int32_t index_bytes;
-  int32_t pad_;
+  /* workgroups per column: 1, or 2..32 (a power of two) with index_bytes 4. A
+   * column of a split problem is worked by `parts` workgroups of
+   * block_size/parts threads that meet at column barriers in global memory
+   * (jja_evolve and jja_adjoint then launch cooperatively: all workgroups of
+   * the launch must be resident at once). Every problem of a plan has the same
+   * block_size/parts. */
+  int32_t parts;
 } jja_problem_desc;
 
 /*
@@ -155,6 +161,13 @@ int jja_plan_validate(int32_t n_problems, const jja_problem_desc* problems);
 /* Create a plan for n_problems circuits on the current HIP device. */
 int jja_plan_create(int32_t n_problems, const jja_problem_desc* problems, jja_plan** out);
 int jja_plan_destroy(jja_plan* plan);
+/*
+ * Status of the plan's split launches so far (waits for the device): JJA_OK, or
+ * JJA_EHIP if a column barrier of a split launch gave up waiting (its results
+ * are invalid). No counterpart in the reference: a split column is this
+ * library's own way to spread one large circuit over many compute units.
+ */
+int jja_plan_status(jja_plan* plan);
 /* Dynamic LDS bytes one column of problem p needs (0 if it cannot run). */
 int64_t jja_plan_lds_bytes(const jja_plan* plan, int32_t p);
 /* Doubles per column of the global face store problem p needs (0: it uses LDS). */

@@ -47,7 +47,7 @@ class ProblemDesc(C.Structure):
         ("n_entries", C.c_int64), ("ent_val", f64p), ("n_index", C.c_int64),
         ("ent_off", C.c_void_p),
         ("max_stage_rows", C.c_int32), ("face_copy_rows", C.c_int32),
-        ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("pad_", C.c_int32),
+        ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("parts", C.c_int32),
     ]
 
 
@@ -88,7 +88,7 @@ _lib = None
 
 EXPORTS = ["jja_cholesky", "jja_factor_get", "jja_factor_free", "jja_factor_last_error",
            "jja_plan_validate", "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes",
-           "jja_plan_zbuf_doubles", "jja_evolve",
+           "jja_plan_zbuf_doubles", "jja_plan_status", "jja_evolve",
            "jja_vortex", "jja_adjoint", "jja_last_error"]
 
 
@@ -112,6 +112,8 @@ def lib():
     L.jja_plan_lds_bytes.restype = C.c_int64
     L.jja_plan_zbuf_doubles.argtypes = [C.c_void_p, C.c_int32]
     L.jja_plan_zbuf_doubles.restype = C.c_int64
+    L.jja_plan_status.argtypes = [C.c_void_p]
+    L.jja_plan_status.restype = C.c_int
     L.jja_evolve.argtypes = [C.c_void_p, C.POINTER(ProblemRun), C.c_int32, C.c_int64, C.c_void_p]
     L.jja_vortex.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     L.jja_adjoint.argtypes = [C.c_void_p, C.POINTER(AdjointRun), C.c_int64, C.c_int64, C.c_void_p]

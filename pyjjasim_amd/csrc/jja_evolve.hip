@@ -114,19 +114,15 @@ struct DevProblem {
   const int32_t* wave_ptr;  // [BS/64 + 1]
   const double* zero;       // 8 zero doubles: absent input terms read here
   const int32_t* face_fid;  // [n_slice_rows]: face index of a face record's row slot, or -1
+  int32_t parts;            // workgroups per column (1, or more with a global face store)
+  int32_t pad_;
 };
 
 struct WorkItem {
   int32_t problem;
-  int32_t column;
+  int32_t column;  // column (times the problem's parts, plus the part, in a split launch)
+  int32_t slot;    // the column's barrier counter in a split launch
 };
-
-__device__ __forceinline__ double src_at(const jja_source& s, int64_t i, int64_t w, int64_t t) {
-  const int64_t tt = t - s.t0;
-  double v = gbl(s.base)[i * s.s_i + w * s.s_w + tt * s.s_t];
-  if (s.amp) v = v * gbl(s.amp)[w * s.a_w + tt * s.a_t];
-  return v;
-}
 
 // Workgroup barrier that orders LDS traffic only: global loads in flight (the
 // prefetched solve entries) stay in flight across it.
@@ -190,34 +186,96 @@ __device__ __forceinline__ double group_sum_last(double v, int G) {
 }
 
 // Where a column keeps its face vector z (by solve position) and the stage
-// scratch t after it. FaceStore<false>: the workgroup's LDS, addressed by the
-// plan's 16-bit indices as absolute LDS addresses (the evolve and adjoint
-// kernels have no static LDS, checked on the host before every launch).
-// FaceStore<true>: a per-column array in global memory for circuits whose face
-// vector does not fit in LDS, addressed by 32-bit indices. The arithmetic on
-// the values is the same in both.
+// scratch t after it, and which threads work on the column. FaceStore<false>:
+// the workgroup's LDS, addressed by the plan's 16-bit indices as absolute LDS
+// addresses (the evolve and adjoint kernels have no static LDS, checked on the
+// host before every launch). FaceStore<true>: a per-column array in global
+// memory for circuits whose face vector does not fit in LDS, addressed by
+// 32-bit indices. SplitStore: the global array of a column that several
+// workgroups (its parts) share. The arithmetic on the values is the same in all.
+//
+// vt and vbs are the thread's index within the column and the column's thread
+// count (threadIdx.x and the block size, or part * blockDim + threadIdx.x and
+// parts * blockDim). sync() orders all memory between the column's threads;
+// barrier() orders the face vector only (the solve's phase boundaries).
 template <bool GLOBAL>
 struct FaceStore;
 
 template <>
 struct FaceStore<false> {
+  static constexpr bool kGlobal = false;
   double* p;
+  int32_t vt, vbs;
   __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
   __device__ __forceinline__ double gather(uint32_t i) const {
     return *reinterpret_cast<const LAS double*>(static_cast<uintptr_t>(i) << 3);
   }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
   // orders LDS traffic only: global loads in flight stay in flight
-  __device__ __forceinline__ static void barrier() { lds_barrier(); }
+  __device__ __forceinline__ void barrier() const { lds_barrier(); }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
 template <>
 struct FaceStore<true> {
+  static constexpr bool kGlobal = true;
   GAS double* p;
+  int32_t vt, vbs;
   __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
   __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
-  __device__ __forceinline__ static void barrier() { __syncthreads(); }
+  __device__ __forceinline__ void barrier() const { __syncthreads(); }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+
+// Waits of one column barrier before a workgroup gives up (each wait sleeps
+// and polls; far longer than any phase of a step takes).
+constexpr uint32_t kSpinLimit = 1u << 21;
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt 0 only: expcnt and
+// lgkmcnt fields at their maxima
+constexpr int kWaitVmem = 0x0F70;
+
+struct SplitStore {
+  static constexpr bool kGlobal = true;
+  GAS double* p;
+  int32_t vt, vbs;
+  uint32_t* ctr;         // the column's arrival counter, zero at launch
+  uint32_t parts;
+  int32_t* err;          // launch status: set when a wait gave up
+  int32_t* stopped;      // this workgroup's flag (LDS): it no longer waits
+  mutable uint32_t gen;  // arrivals to wait for (thread 0)
+  __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
+  __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
+  __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
+  // Every wave waits for its own memory operations (stores reach the L2,
+  // which the L1 writes through to), the workgroup meets, and thread 0 alone
+  // releases at device scope (one L2 write-back per workgroup), arrives, polls
+  // the counter with relaxed loads (an acquire per poll would invalidate the
+  // L2 under the working workgroups) and acquires once (one L1 and L2
+  // invalidation, which covers the compute unit's other waves); then the
+  // workgroup meets again. A wait that gives up sets err and stops this
+  // workgroup's waits for the rest of the launch (its results are then
+  // invalid, and the host reports the error).
+  __device__ __forceinline__ void sync() const {
+    __builtin_amdgcn_s_waitcnt(kWaitVmem);
+    __syncthreads();
+    if (threadIdx.x == 0 && !*stopped) {
+      gen += parts;
+      __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t spins = 0;
+      while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
+        if (++spins > kSpinLimit) {
+          *stopped = 1;
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void barrier() const { sync(); }
 };
 
 // Entries of one solve record for one lane, held in registers: 8 values and
@@ -285,9 +343,9 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, __amdgpu_buffer_
 }
 
 // Gather, accumulate, and (flag FINISH) reduce and write one record's rows.
-template <bool GLOBAL>
-__device__ __forceinline__ void process_slice(const FaceStore<GLOBAL>& zs, int32_t Nf, int lane,
-                                              const jja_slice& m, const SliceRegs<GLOBAL>& r, double& acc) {
+template <class Z>
+__device__ __forceinline__ void process_slice(const Z& zs, int32_t Nf, int lane, const jja_slice& m,
+                                              const SliceRegs<Z::kGlobal>& r, double& acc) {
   // iterations at or past the record's count have zero values: their gathers
   // are skipped (a uniform branch) and contribute an exact zero product
   double zv[kMaxIter];
@@ -307,7 +365,7 @@ __device__ __forceinline__ void process_slice(const FaceStore<GLOBAL>& zs, int32
     }
     acc = 0.0;
   }
-  if (m.flags & JJA_SLICE_BARRIER) FaceStore<GLOBAL>::barrier();
+  if (m.flags & JJA_SLICE_BARRIER) zs.barrier();
 }
 
 // The whole forward + backward solve. Each wave walks its own record program,
@@ -319,10 +377,11 @@ __device__ __forceinline__ void process_slice(const FaceStore<GLOBAL>& zs, int32
 // of three and four never-executed records follow it, so the loop has a single
 // exit and every lookahead load stays inside the program. Waves meet only at
 // BARRIER records (one per phase, the same number in every wave).
-template <bool GLOBAL>
-__device__ __forceinline__ void solve(const DevProblem& P, const FaceStore<GLOBAL>& zs) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+template <class Z>
+__device__ __forceinline__ void solve(const DevProblem& P, const Z& zs) {
+  constexpr bool GLOBAL = Z::kGlobal;
+  const int lane = zs.vt & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
   const CAS int32_t* wp = (const CAS int32_t*)(P.wave_ptr) + wave;
   const int32_t k0 = __builtin_amdgcn_readfirstlane(wp[0]);
   const int32_t nk = __builtin_amdgcn_readfirstlane(wp[1]) - k0 - 4;
@@ -390,14 +449,13 @@ struct FaceRec {
   bool finish;
 };
 
-template <int BS>
 __device__ __forceinline__ void face_rec_load(const DevProblem& P, int32_t k, bool present, int tid,
-                                              FaceRec& r) {
+                                              int32_t vbs, FaceRec& r) {
   const jja_slice m = load_rec(P.face_slices, k);
   const bool live = present && tid < (m.flags >> 16);
   const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + tid;
 #pragma unroll
-  for (int i = 0; i < kFaceIter; ++i) r.cd[i] = (live && i < m.n_iter) ? code[i * BS] : -1;
+  for (int i = 0; i < kFaceIter; ++i) r.cd[i] = (live && i < m.n_iter) ? code[i * vbs] : -1;
   r.finish = present && (m.flags & JJA_SLICE_FINISH);
   const bool rowed = live && r.finish;
   r.pos = rowed ? gbl(P.slice_rows)[m.rowbase + tid] : -1;
@@ -408,7 +466,7 @@ template <int BS, class Z>
 __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f,
                                            const GAS double* __restrict__ u, const Z& zs,
                                            int64_t w, int64_t t) {
-  const int tid = threadIdx.x;
+  const int tid = zs.vt;
   // flux of face q at this step is fv.p[q * fv.si] * fv.amp; an absent flux
   // reads an exact zero, and acc - 2 pi * 0 leaves acc unchanged
   const TermView fv = term_view(f ? *f : jja_source{}, f && !f->zero, P.zero, w, t);
@@ -417,7 +475,8 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source
   for (int32_t k = 0; k < n; k += kFaceGroup) {
     FaceRec r[kFaceGroup];
 #pragma unroll
-    for (int g = 0; g < kFaceGroup; ++g) face_rec_load<BS>(P, k + g < n ? k + g : k, k + g < n, tid, r[g]);
+    for (int g = 0; g < kFaceGroup; ++g)
+      face_rec_load(P, k + g < n ? k + g : k, k + g < n, tid, zs.vbs, r[g]);
     double uu[kFaceGroup][kFaceIter], fq[kFaceGroup];
 #pragma unroll
     for (int g = 0; g < kFaceGroup; ++g) {
@@ -460,6 +519,7 @@ struct JunctionStep {
   uint64_t seed;
   uint32_t noise_col;
   uint64_t tx;
+  int32_t vt, vbs;             // the thread's index in the column, the column's thread count
 };
 
 // ---------------------------------------------------------------- general
@@ -550,10 +610,10 @@ __device__ __forceinline__ double take_normal(double z0, double z1, int src, boo
 // wave take part, whether or not their junctions exist.
 template <int BS>
 __device__ __forceinline__ void wave_normals(const JunctionStep& J, int32_t k, double& zA, double& zB) {
-  const int lane = threadIdx.x & 63;
-  const int32_t wave_base = static_cast<int32_t>(threadIdx.x) - lane;
+  const int lane = J.vt & 63;
+  const int32_t wave_base = J.vt - lane;
   const int32_t it = lane < 32 ? k : k + 1;
-  const int32_t j_even = it * BS + wave_base + 2 * (lane & 31);
+  const int32_t j_even = it * J.vbs + wave_base + 2 * (lane & 31);
   double z0, z1;
   normal_pair(J.seed, static_cast<uint32_t>(j_even), J.noise_col, J.tx, &z0, &z1);
   zA = take_normal(z0, z1, lane >> 1, lane & 1);
@@ -565,12 +625,12 @@ __device__ __forceinline__ void wave_normals(const JunctionStep& J, int32_t k, d
 template <int BS, int SW, bool UPDATE, bool PREP, class Z>
 __device__ __forceinline__ void junction_loop(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
-  const int32_t tid = threadIdx.x;
+  const int32_t tid = J.vt, vbs = J.vbs;
   const bool gauss = PREP && !J.t_zero && !J.noise;
-  for (int32_t k = 0; k * BS < Nj; k += 2) {
+  for (int32_t k = 0; k * vbs < Nj; k += 2) {
     double zA = 0.0, zB = 0.0;
     if (gauss) wave_normals<BS>(J, k, zA, zB);
-    const int32_t jA = k * BS + tid, jB = jA + BS;
+    const int32_t jA = k * vbs + tid, jB = jA + vbs;
     if (jA < Nj) junction_one<SW, UPDATE, PREP, Z>(P, J, zs, jA, zA);
     // keep the two junctions apart: interleaving them would double the live
     // registers and spill
@@ -624,11 +684,11 @@ __device__ __forceinline__ void jfast_compute(const JunctionStep& J, const Z& zs
 template <int BS, class Z>
 __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
-  const int32_t tid = threadIdx.x;
+  const int32_t tid = J.vt, vbs = J.vbs;
   const int32_t last = Nj - 1;
   const bool gauss = !J.t_zero;
-  for (int32_t k = 0; k * BS < Nj; k += 2) {
-    const int32_t jA = k * BS + tid, jB = jA + BS;
+  for (int32_t k = 0; k * vbs < Nj; k += 2) {
+    const int32_t jA = k * vbs + tid, jB = jA + vbs;
     // loads of junctions past the end read the last junction and are not used
     JFast A, B;
     jfast_load(P, J, min(jA, last), A);
@@ -656,6 +716,8 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
     return static_cast<size_t>(r < 0 ? r + S : r) * slot_stride;
   };
   JunctionStep J;
+  J.vt = zs.vt;
+  J.vbs = zs.vbs;
   J.th_n = th + slot(t);
 #pragma unroll
   for (int k = 0; k < 4; ++k) J.th_k[k] = th + (k < S ? slot(tx - 1 - k) : 0);
@@ -689,16 +751,10 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
 }
 
-// n_steps steps of column w of one problem (one workgroup).
-template <int BS, int SW, bool GLOBAL>
-__device__ __forceinline__ void evolve_column(const DevProblem& P, const jja_problem_run& R, int64_t w,
-                                              double* lds, int32_t n_steps, int64_t step0) {
-  // face vector by solve position, stage scratch after it
-  FaceStore<GLOBAL> zs;
-  if constexpr (GLOBAL)
-    zs.p = gblw(R.zbuf) + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
-  else
-    zs.p = lds;
+// n_steps steps of column w of one problem, by the threads the store names.
+template <int BS, int SW, class Z>
+__device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_problem_run& R, int64_t w,
+                                             const Z& zs, int32_t n_steps, int64_t step0) {
   const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
   GAS double* __restrict__ th = gblw(R.theta) + static_cast<size_t>(w) * P.Nj;
   GAS double* __restrict__ ths = R.theta_s ? gblw(R.theta_s) + static_cast<size_t>(w) * P.Nj : nullptr;
@@ -706,16 +762,67 @@ __device__ __forceinline__ void evolve_column(const DevProblem& P, const jja_pro
   const uint32_t noise_col = static_cast<uint32_t>(R.column_offset + w);
 
   junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, step0, false, step0, true, noise_col);
-  __syncthreads();
+  zs.sync();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
     face_phase<BS>(P, &R.f, u, zs, w, t);
-    FaceStore<GLOBAL>::barrier();
+    zs.barrier();
     solve(P, zs);
     // finish step t and prepare step t+1 in one pass
     junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
-    __syncthreads();
+    zs.sync();
   }
+}
+
+// Column w of one problem in one workgroup: face vector by solve position,
+// stage scratch after it.
+template <int BS, int SW, bool GLOBAL>
+__device__ __forceinline__ void evolve_column(const DevProblem& P, const jja_problem_run& R, int64_t w,
+                                              double* lds, int32_t n_steps, int64_t step0) {
+  FaceStore<GLOBAL> zs;
+  if constexpr (GLOBAL)
+    zs.p = gblw(R.zbuf) + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
+  else
+    zs.p = lds;
+  zs.vt = threadIdx.x;
+  zs.vbs = BS;
+  evolve_steps<BS, SW>(P, R, w, zs, n_steps, step0);
+}
+
+// A column shared by P.parts workgroups (part `part` of the work item's column
+// w): face vector in global memory, a column barrier wherever the one-workgroup
+// kernel has a workgroup barrier.
+__device__ __forceinline__ SplitStore split_store(const DevProblem& P, GAS double* zbuf, int64_t w,
+                                                  int32_t part, uint32_t* ctrs, const WorkItem& wi,
+                                                  int32_t* err, int32_t* stopped, int BS) {
+  SplitStore zs;
+  zs.p = zbuf + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
+  zs.vt = part * BS + static_cast<int32_t>(threadIdx.x);
+  zs.vbs = P.parts * BS;
+  zs.ctr = ctrs + wi.slot;
+  zs.parts = static_cast<uint32_t>(P.parts);
+  zs.err = err;
+  zs.stopped = stopped;
+  zs.gen = 0;
+  return zs;
+}
+
+template <int BS, int SW>
+__global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __restrict__ probs,
+                                                          const jja_problem_run* __restrict__ runs,
+                                                          const WorkItem* __restrict__ work,
+                                                          int32_t n_steps, int64_t step0, uint32_t* ctrs,
+                                                          int32_t* err) {
+  __shared__ int32_t stopped;
+  if (threadIdx.x == 0) stopped = 0;
+  __syncthreads();
+  const WorkItem wi = work[blockIdx.x];
+  const DevProblem& P = probs[wi.problem];
+  const jja_problem_run& R = runs[wi.problem];
+  const int64_t w = wi.column / P.parts;
+  const int32_t part = wi.column % P.parts;
+  const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, &stopped, BS);
+  evolve_steps<BS, SW>(P, R, w, zs, n_steps, step0);
 }
 
 // One workgroup per work item (problem, column). STORE: 0 face stores in LDS,
@@ -759,12 +866,12 @@ __device__ __forceinline__ double adj_g(const jja_adjoint_run& A, const AdjCol& 
 }
 
 // lambda_k += g_k (complete) and u = lambda_k / c0: the first step of a sweep.
-template <int BS>
+template <class Z>
 __device__ __forceinline__ void adjoint_prime(const DevProblem& P, const jja_adjoint_run& A,
-                                              const AdjCol& C, int64_t k) {
+                                              const AdjCol& C, const Z& zs, int64_t k) {
   const GAS double* __restrict__ c0 = gbl(P.coef);
   GAS double* __restrict__ lk = C.lam + (k & 1) * C.stride;
-  for (int32_t j = threadIdx.x; j < P.Nj; j += BS) {
+  for (int32_t j = zs.vt; j < P.Nj; j += zs.vbs) {
     const double l = lk[j] + adj_g(A, C, k, j);
     lk[j] = l;
     C.u[j] = l / c0[j];
@@ -775,7 +882,7 @@ __device__ __forceinline__ void adjoint_prime(const DevProblem& P, const jja_adj
 // mu_k = (A^T J' - lambda_k)/c0 and the updates of lambda_{k-1}, lambda_{k-2}
 // and of the gradients; then lambda_{k-1} gets g_{k-1} and u = lambda_{k-1}/c0
 // prepares step k-1.
-template <int BS, class Z>
+template <class Z>
 __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja_adjoint_run& A,
                                                   const AdjCol& C, const Z& zs, int64_t k, int64_t w) {
   const int32_t Nj = P.Nj;
@@ -787,7 +894,7 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
   const GAS double* __restrict__ h2 = C.hist + static_cast<size_t>(k - 2 - C.hist_m0) * C.stride;
   GAS double* __restrict__ gIc = gblw(A.grad_Ic) + static_cast<size_t>(w) * Nj;
   GAS double* __restrict__ gIs = gblw(A.grad_Is) + static_cast<size_t>(w) * Nj;
-  for (int32_t j = threadIdx.x; j < Nj; j += BS) {
+  for (int32_t j = zs.vt; j < Nj; j += zs.vbs) {
     const vint2 code = jcode[j];
     const double c0 = coef[j], c1 = coef[static_cast<int64_t>(Nj) + j], c2 = coef[2 * static_cast<int64_t>(Nj) + j];
     const double lam = lk[j], lprev = lk1[j], t1 = h1[j], t2 = h2[j], Ic = C.Ic[j];
@@ -809,16 +916,9 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
   }
 }
 
-template <int BS, bool GLOBAL>
-__global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restrict__ probs,
-                                                     const jja_adjoint_run* __restrict__ runs,
-                                                     const WorkItem* __restrict__ work, int64_t k_hi,
-                                                     int64_t k_lo) {
-  extern __shared__ double lds[];
-  const WorkItem wi = work[blockIdx.x];
-  const DevProblem& P = probs[wi.problem];
-  const jja_adjoint_run& A = runs[wi.problem];
-  const int64_t w = wi.column;
+// The column of work item wi: lambda, history, gradients and upstream
+// gradient of column w.
+__device__ __forceinline__ AdjCol adjoint_column(const DevProblem& P, const jja_adjoint_run& A, int64_t w) {
   const size_t stride = static_cast<size_t>(A.W) * P.Nj;
   const size_t col = static_cast<size_t>(w) * P.Nj;
   AdjCol C;
@@ -829,26 +929,67 @@ __global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restric
   C.hist_m0 = A.hist_m0;
   C.stride = stride;
   C.Ic = gbl(A.Ic ? A.Ic : P.Ic);
+  return C;
+}
+
+// Steps k_hi down to k_lo of the adjoint sweep of column w.
+template <int BS, class Z>
+__device__ __forceinline__ void adjoint_steps(const DevProblem& P, const jja_adjoint_run& A, int64_t w,
+                                              const Z& zs, int64_t k_hi, int64_t k_lo) {
+  const AdjCol C = adjoint_column(P, A, w);
   GAS double* __restrict__ gf = gblw(A.grad_f) + static_cast<size_t>(w) * P.Nf;
+  if (A.prime) adjoint_prime(P, A, C, zs, k_hi);
+  zs.sync();
+  for (int64_t k = k_hi; k >= k_lo; --k) {
+    face_phase<BS>(P, nullptr, C.u, zs, w, k);
+    zs.barrier();
+    solve(P, zs);
+    for (int32_t p = zs.vt; p < P.Nf; p += zs.vbs) gf[p] = gf[p] - kTwoPi * zs.rd(p);
+    adjoint_junctions(P, A, C, zs, k, w);
+    zs.sync();
+  }
+}
+
+template <int BS, bool GLOBAL>
+__global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restrict__ probs,
+                                                     const jja_adjoint_run* __restrict__ runs,
+                                                     const WorkItem* __restrict__ work, int64_t k_hi,
+                                                     int64_t k_lo) {
+  extern __shared__ double lds[];
+  const WorkItem wi = work[blockIdx.x];
+  const DevProblem& P = probs[wi.problem];
+  const jja_adjoint_run& A = runs[wi.problem];
+  const int64_t w = wi.column;
   FaceStore<GLOBAL> zs;
   if constexpr (GLOBAL)
     zs.p = gblw(A.zbuf) + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
   else
     zs.p = lds;
-  if (A.prime) adjoint_prime<BS>(P, A, C, k_hi);
+  zs.vt = threadIdx.x;
+  zs.vbs = BS;
+  adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __restrict__ probs,
+                                                           const jja_adjoint_run* __restrict__ runs,
+                                                           const WorkItem* __restrict__ work, int64_t k_hi,
+                                                           int64_t k_lo, uint32_t* ctrs, int32_t* err) {
+  __shared__ int32_t stopped;
+  if (threadIdx.x == 0) stopped = 0;
   __syncthreads();
-  for (int64_t k = k_hi; k >= k_lo; --k) {
-    face_phase<BS>(P, nullptr, C.u, zs, w, k);
-    FaceStore<GLOBAL>::barrier();
-    solve(P, zs);
-    for (int32_t p = threadIdx.x; p < P.Nf; p += BS) gf[p] = gf[p] - kTwoPi * zs.rd(p);
-    adjoint_junctions<BS>(P, A, C, zs, k, w);
-    __syncthreads();
-  }
+  const WorkItem wi = work[blockIdx.x];
+  const DevProblem& P = probs[wi.problem];
+  const jja_adjoint_run& A = runs[wi.problem];
+  const int64_t w = wi.column / P.parts;
+  const int32_t part = wi.column % P.parts;
+  const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, &stopped, BS);
+  adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
 }
 
 // n[col][f] = -sum_j A[f,j] rint(theta_j / 2pi) (time_evolution.py:734-755): one
-// workgroup per phase vector, one thread per face position.
+// workgroup per phase vector, one thread per face position (a workgroup takes
+// the slots of all parts of a split plan in turn).
 template <int BS>
 __global__ __launch_bounds__(BS) void vortex_kernel(const DevProblem* __restrict__ probs, int32_t p,
                                                     const double* __restrict__ theta,
@@ -856,21 +997,24 @@ __global__ __launch_bounds__(BS) void vortex_kernel(const DevProblem* __restrict
   const DevProblem& P = probs[p];
   const int64_t col = blockIdx.x;
   const GAS double* __restrict__ th = gbl(theta) + col * P.Nj;
-  const int tid = threadIdx.x;
-  int64_t acc = 0;
-  for (int32_t k = 0; k < P.n_face_slices; ++k) {
-    const jja_slice m = load_rec(P.face_slices, k);
-    const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + tid;
-    for (int32_t it = 0; it < m.n_iter; ++it) {
-      const int32_t cd = code[static_cast<int64_t>(it) * BS];
-      if (cd < 0) continue;
-      const int64_t kk = static_cast<int64_t>(rint(th[cd >> 1] / kTwoPi));
-      acc += (cd & 1) ? kk : -kk;
-    }
-    if (m.flags & JJA_SLICE_FINISH) {
-      const int32_t pos = gbl(P.slice_rows)[m.rowbase + tid];
-      if (pos >= 0) gblw(n_out)[col * P.Nf + gbl(P.face_of_pos)[pos]] = static_cast<int32_t>(acc);
-      acc = 0;
+  const int64_t vbs = static_cast<int64_t>(P.parts) * BS;
+  for (int32_t part = 0; part < P.parts; ++part) {
+    const int32_t vt = part * BS + static_cast<int32_t>(threadIdx.x);
+    int64_t acc = 0;
+    for (int32_t k = 0; k < P.n_face_slices; ++k) {
+      const jja_slice m = load_rec(P.face_slices, k);
+      const GAS int32_t* __restrict__ code = gbl(P.face_code) + m.base + vt;
+      for (int32_t it = 0; it < m.n_iter; ++it) {
+        const int32_t cd = code[it * vbs];
+        if (cd < 0) continue;
+        const int64_t kk = static_cast<int64_t>(rint(th[cd >> 1] / kTwoPi));
+        acc += (cd & 1) ? kk : -kk;
+      }
+      if (m.flags & JJA_SLICE_FINISH) {
+        const int32_t pos = gbl(P.slice_rows)[m.rowbase + vt];
+        if (pos >= 0) gblw(n_out)[col * P.Nf + gbl(P.face_of_pos)[pos]] = static_cast<int32_t>(acc);
+        acc = 0;
+      }
     }
   }
 }
@@ -890,6 +1034,11 @@ struct Plan {
   int32_t block_size = 1024;
   int32_t stencil_width = 3;
   jja_adjoint_run* d_adj = nullptr;  // allocated at the first jja_adjoint call
+  int32_t parts = 1;                 // workgroups per column (block_size is one workgroup's)
+  uint32_t* d_ctrs = nullptr;        // split launches: one barrier counter per column
+  size_t ctr_cap = 0;
+  int32_t* d_err = nullptr;          // split launches: set when a column barrier gave up
+  int32_t n_cu = 0;                  // compute units of the device
 };
 
 template <typename T>
@@ -918,8 +1067,13 @@ int validate(const jja_problem_desc& d) {
   if (Nj <= 0 || Nf < 0 || d.stencil_width < 3 || d.stencil_width > 5 || d.n_face_slices < 0 ||
       d.n_solve_slices < 0 || ms < 0)
     return fail(JJA_EINVAL, "jja_plan_create: bad sizes");
-  if (BS != 64 && BS != 128 && BS != 256 && BS != 512 && BS != 1024)
-    return fail(JJA_EINVAL, "jja_plan_create: block_size must be 64, 128, 256, 512 or 1024");
+  const int32_t parts = d.parts > 0 ? d.parts : 1;
+  if (!pow2_le(parts, 32)) return fail(JJA_EINVAL, "jja_plan_create: parts must be a power of two <= 32");
+  const int32_t wg = BS / parts;
+  if (BS % parts || (wg != 64 && wg != 128 && wg != 256 && wg != 512 && wg != 1024))
+    return fail(JJA_EINVAL, "jja_plan_create: block_size/parts must be 64, 128, 256, 512 or 1024");
+  if (parts > 1 && d.index_bytes != 4)
+    return fail(JJA_EINVAL, "jja_plan_create: a column split over workgroups needs the global face store");
   if (!d.Ic || !d.Rv || !d.coef || !d.junc_code)
     return fail(JJA_EINVAL, "jja_plan_create: missing junction arrays");
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
@@ -1106,6 +1260,94 @@ void order_work(const Plan* plan, std::vector<WorkItem>& work) {
   });
 }
 
+// Work items of a split launch, from one item per column: every part of every
+// column, each column with its own barrier counter (slot). When the column
+// count is a multiple of 8, the parts of a column sit at block indices equal
+// modulo 8: workgroups go to the eight XCDs in turn by block index, so the
+// parts of a column then share one L2.
+std::vector<WorkItem> split_work(const Plan* plan, std::vector<WorkItem> cols) {
+  order_work(plan, cols);
+  constexpr size_t kXcd = 8;
+  const size_t G = static_cast<size_t>(plan->parts), nc = cols.size();
+  std::vector<WorkItem> out(nc * G);
+  for (size_t c = 0; c < nc; ++c)
+    for (size_t g = 0; g < G; ++g) {
+      const size_t b = nc % kXcd == 0 ? ((c / kXcd) * G + g) * kXcd + c % kXcd : c * G + g;
+      out[b] = {cols[c].problem, static_cast<int32_t>(cols[c].column * G + g), static_cast<int32_t>(c)};
+    }
+  return out;
+}
+
+// Barrier counters of n columns, zeroed on the stream before a split launch.
+int reset_counters(Plan* plan, size_t n, hipStream_t stream) {
+  if (n > plan->ctr_cap) {
+    if (plan->d_ctrs) {
+      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_ctrs));
+      if (it != plan->allocations.end()) plan->allocations.erase(it);
+      JJA_HIP(hipFree(plan->d_ctrs));
+      plan->d_ctrs = nullptr;
+    }
+    void* d = nullptr;
+    JJA_HIP(hipMalloc(&d, n * sizeof(uint32_t)));
+    plan->allocations.push_back(d);
+    plan->d_ctrs = static_cast<uint32_t*>(d);
+    plan->ctr_cap = n;
+  }
+  JJA_HIP(hipMemsetAsync(plan->d_ctrs, 0, n * sizeof(uint32_t), stream));
+  return JJA_OK;
+}
+
+// Cooperative launch of a split kernel: all n workgroups resident at once (the
+// column barriers need every part of a column running), or an error.
+template <typename... Args>
+int launch_cooperative(Plan* plan, const void* kern, int BS, int32_t n, hipStream_t stream, Args... args) {
+  int per_cu = 0;
+  JJA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, 0));
+  if (static_cast<int64_t>(per_cu) * plan->n_cu < n)
+    return fail(JJA_EUNSUPPORTED, "split launch of " + std::to_string(n) + " workgroups; the device holds " +
+                                      std::to_string(static_cast<int64_t>(per_cu) * plan->n_cu) + " at once");
+  void* argv[] = {static_cast<void*>(&args)...};
+  JJA_HIP(hipLaunchCooperativeKernel(kern, dim3(n), dim3(BS), argv, 0, stream));
+  return JJA_OK;
+}
+
+template <int BS>
+int launch_evolve_split_bs(Plan* plan, int32_t n, int32_t n_steps, int64_t step0, hipStream_t stream) {
+  const DevProblem* probs = plan->d_problems;
+  const jja_problem_run* runs = plan->d_runs;
+  const WorkItem* work = plan->d_work;
+  return launch_cooperative(plan, reinterpret_cast<const void*>(evolve_split_kernel<BS, 3>), BS, n, stream,
+                            probs, runs, work, n_steps, step0, plan->d_ctrs, plan->d_err);
+}
+
+int launch_evolve_split(Plan* plan, int32_t n, int32_t n_steps, int64_t step0, hipStream_t stream) {
+  if (plan->stencil_width != 3) return fail(JJA_EUNSUPPORTED, "split columns: stencil width 3 only");
+  switch (plan->block_size) {
+    case 256: return launch_evolve_split_bs<256>(plan, n, n_steps, step0, stream);
+    case 512: return launch_evolve_split_bs<512>(plan, n, n_steps, step0, stream);
+    case 1024: return launch_evolve_split_bs<1024>(plan, n, n_steps, step0, stream);
+    default: return fail(JJA_EUNSUPPORTED, "split columns: workgroups of 256, 512 or 1024 threads");
+  }
+}
+
+template <int BS>
+int launch_adjoint_split_bs(Plan* plan, int32_t n, int64_t k_hi, int64_t k_lo, hipStream_t stream) {
+  const DevProblem* probs = plan->d_problems;
+  const jja_adjoint_run* runs = plan->d_adj;
+  const WorkItem* work = plan->d_work;
+  return launch_cooperative(plan, reinterpret_cast<const void*>(adjoint_split_kernel<BS>), BS, n, stream,
+                            probs, runs, work, k_hi, k_lo, plan->d_ctrs, plan->d_err);
+}
+
+int launch_adjoint_split(Plan* plan, int32_t n, int64_t k_hi, int64_t k_lo, hipStream_t stream) {
+  switch (plan->block_size) {
+    case 256: return launch_adjoint_split_bs<256>(plan, n, k_hi, k_lo, stream);
+    case 512: return launch_adjoint_split_bs<512>(plan, n, k_hi, k_lo, stream);
+    case 1024: return launch_adjoint_split_bs<1024>(plan, n, k_hi, k_lo, stream);
+    default: return fail(JJA_EUNSUPPORTED, "split columns: workgroups of 256, 512 or 1024 threads");
+  }
+}
+
 template <bool GLOBAL>
 int launch_adjoint_bs(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, int64_t k_hi,
                       int64_t k_lo, hipStream_t stream) {
@@ -1127,8 +1369,8 @@ const char* jja_last_error(void) { return g_error.c_str(); }
 int jja_plan_validate(int32_t n_problems, const jja_problem_desc* descs) {
   if (n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_validate: invalid arguments");
   for (int32_t p = 0; p < n_problems; ++p) {
-    if (descs[p].block_size != descs[0].block_size)
-      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size");
+    if (descs[p].block_size != descs[0].block_size || descs[p].parts != descs[0].parts)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size and parts");
     if (descs[p].stencil_width != descs[0].stencil_width)
       return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
     if (int rc = validate(descs[p])) return rc;
@@ -1140,13 +1382,14 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
   if (!out || n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_create: invalid arguments");
   *out = nullptr;
   for (int32_t p = 1; p < n_problems; ++p)
-    if (descs[p].block_size != descs[0].block_size)
-      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size");
+    if (descs[p].block_size != descs[0].block_size || descs[p].parts != descs[0].parts)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size and parts");
   for (int32_t p = 1; p < n_problems; ++p)
     if (descs[p].stencil_width != descs[0].stencil_width)
       return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
   Plan* plan = new Plan();
-  plan->block_size = descs[0].block_size;
+  plan->parts = descs[0].parts > 0 ? descs[0].parts : 1;
+  plan->block_size = descs[0].block_size / plan->parts;
   plan->stencil_width = descs[0].stencil_width;
   int rc = JJA_OK;
   hipDeviceProp_t prop;
@@ -1156,6 +1399,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     return fail(JJA_EHIP, "jja_plan_create: no HIP device");
   }
   plan->lds_limit = static_cast<int64_t>(prop.sharedMemPerBlock);
+  plan->n_cu = prop.multiProcessorCount;
   for (int32_t p = 0; p < n_problems && rc == JJA_OK; ++p) {
     const jja_problem_desc& d = descs[p];
     rc = validate(d);
@@ -1165,6 +1409,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.Nf = d.n_faces;
     dp.sw = d.stencil_width;
     dp.BS = d.block_size;
+    dp.parts = plan->parts;
     dp.n_face_slices = d.n_face_slices;
     dp.n_solve_slices = d.n_solve_slices;
     dp.max_stage_rows = d.max_stage_rows;
@@ -1231,12 +1476,32 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       plan->d_runs = static_cast<jja_problem_run*>(d);
     }
   }
+  if (rc == JJA_OK && plan->parts > 1) {
+    void* d = nullptr;
+    if (hipMalloc(&d, sizeof(int32_t)) != hipSuccess || hipMemset(d, 0, sizeof(int32_t)) != hipSuccess) {
+      rc = fail(JJA_ENOMEM, "jja_plan_create: device allocation failed");
+    } else {
+      plan->allocations.push_back(d);
+      plan->d_err = static_cast<int32_t*>(d);
+    }
+  }
   if (rc != JJA_OK) {
     free_plan(plan);
     return rc;
   }
   *out = reinterpret_cast<jja_plan*>(plan);
   return JJA_OK;
+}
+
+int jja_plan_status(jja_plan* p) {
+  Plan* plan = reinterpret_cast<Plan*>(p);
+  if (!plan) return fail(JJA_EINVAL, "jja_plan_status: invalid arguments");
+  if (!plan->d_err) return JJA_OK;
+  int32_t e = 0;
+  JJA_HIP(hipMemcpy(&e, plan->d_err, sizeof(e), hipMemcpyDeviceToHost));
+  if (!e) return JJA_OK;
+  JJA_HIP(hipMemset(plan->d_err, 0, sizeof(int32_t)));
+  return fail(JJA_EHIP, "a column barrier of a split launch gave up waiting; its results are invalid");
 }
 
 int jja_plan_destroy(jja_plan* p) {
@@ -1286,6 +1551,13 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
   // pageable source: the call returns once the data is staged, and the copy is
   // ordered after earlier work on the stream
   JJA_HIP(hipMemcpyAsync(plan->d_runs, runs, sizeof(jja_problem_run) * np, hipMemcpyHostToDevice, stream));
+  if (plan->parts > 1) {
+    // every problem of a split plan has the global face store (validated)
+    const std::vector<WorkItem> split = split_work(plan, work_g);
+    if (int rc = reset_counters(plan, work_g.size(), stream)) return rc;
+    if (int rc = upload_work(plan, split, stream)) return rc;
+    return launch_evolve_split(plan, static_cast<int32_t>(split.size()), n_steps, step0, stream);
+  }
   // one launch: a kernel for one kind of face store, or one that takes either
   const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
   work.insert(work.end(), work_g.begin(), work_g.end());
@@ -1328,6 +1600,12 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
   }
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   JJA_HIP(hipMemcpyAsync(plan->d_adj, runs, sizeof(jja_adjoint_run) * np, hipMemcpyHostToDevice, stream));
+  if (plan->parts > 1) {
+    const std::vector<WorkItem> split = split_work(plan, work_g);
+    if (int rc = reset_counters(plan, work_g.size(), stream)) return rc;
+    if (int rc = upload_work(plan, split, stream)) return rc;
+    return launch_adjoint_split(plan, static_cast<int32_t>(split.size()), k_hi, k_lo, stream);
+  }
   const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
   order_work(plan, work);
   order_work(plan, work_g);

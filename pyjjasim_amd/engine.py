@@ -110,6 +110,23 @@ def default_block_size(Nj, Nf):
     return 1024
 
 
+def device_compute_units():
+    return _torch().cuda.get_device_properties(0).multi_processor_count
+
+
+def auto_parts(host, columns, stencil_width, max_parts=16):
+    """Workgroups per column for a plan: 1 for an LDS face store, else the
+    largest power of two up to max_parts whose workgroups, one per compute
+    unit, all fit on the device."""
+    if not host.global_store or stencil_width != 3 or host.block_size not in (256, 512, 1024):
+        return 1
+    n_cu = device_compute_units()
+    parts = 1
+    while parts < max_parts and 2 * parts * columns <= n_cu:
+        parts *= 2
+    return parts
+
+
 class Engine:
     """Device state and launches of one problem. With ``shared = (handle, index,
     host)`` the problem is problem ``index`` of a plan that a BatchEngine owns."""
@@ -132,6 +149,11 @@ class Engine:
             self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
                                     block_size=self.block_size,
                                     face_store=getattr(problem, "face_store", "auto"))
+            parts = getattr(problem, "parts", None)
+            if parts is None:
+                parts = auto_parts(self.host, W, sw)
+            if parts > 1:
+                self.host = self.host.relayout(self.block_size, parts)
             self._desc = self.host.desc()
             self.handle = C.c_void_p()
             _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
@@ -260,6 +282,7 @@ class Engine:
             _native.check(self.lib.jja_vortex(self.handle, self.index, self.th_out.data_ptr(),
                                               self.n_th * W, n_dev.data_ptr(), stream))
         torch.cuda.synchronize()
+        _native.check(self.lib.jja_plan_status(self.handle))
         th = self.th_out[:self.n_th].cpu().numpy().transpose(2, 1, 0) if self.n_th \
             else np.zeros((Nj, W, 0))
         I = self.I_out[:self.n_I].cpu().numpy().transpose(2, 1, 0) if self.n_I \

@@ -316,13 +316,20 @@ def cycle_codes(A, Nf, Nj):
 
 class CircuitPlan:
     """All host arrays of one circuit for ``jja_plan_create``, laid out for
-    workgroups of ``block_size`` threads."""
+    workgroups of ``block_size`` threads; with ``parts`` > 1 a column is worked
+    by that many workgroups (the layouts are built for parts * block_size
+    threads, and the face store is global)."""
 
     def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8,
-                 face_store="auto"):
+                 face_store="auto", parts=1):
         if face_store not in ("auto", "lds", "global"):
             raise ValueError("face_store must be 'auto', 'lds' or 'global'")
+        if parts < 1 or parts & (parts - 1) or parts > 32:
+            raise ValueError("parts must be a power of two from 1 to 32")
+        if parts > 1 and face_store == "lds":
+            raise ValueError("a column split over workgroups needs the global face store")
         self.face_store = face_store
+        self.parts = parts
         if stencil_width not in STENCILS:
             raise ValueError(f"stencil width must be 3, 4 or 5 (equals {stencil_width})")
         self.Nj, self.Nf = circuit._Nj(), circuit._Nf()
@@ -348,16 +355,25 @@ class CircuitPlan:
         self.target_iter = target_iter
         self._build_layouts(target_iter)
 
-    def relayout(self, block_size):
+    def relayout(self, block_size, parts=None):
         """The same circuit, factor and schedule laid out for another workgroup
-        size (the factorisation does not depend on it)."""
+        size or part count (the factorisation does not depend on them)."""
         other = copy.copy(self)
         other.block_size = block_size
+        if parts is not None:
+            if parts > 1 and self.face_store == "lds":
+                raise ValueError("a column split over workgroups needs the global face store")
+            other.parts = parts
         other._build_layouts(self.target_iter)
         return other
 
+    @property
+    def column_threads(self):
+        """Threads that work on one column: the layouts are built for this many."""
+        return self.block_size * self.parts
+
     def _build_layouts(self, target_iter=8):
-        BS, sp, Nf, Nj = self.block_size, self.solve_plan, self.Nf, self.Nj
+        BS, sp, Nf, Nj = self.column_threads, self.solve_plan, self.Nf, self.Nj
         # the face vector and stage scratch of a column: in the workgroup's LDS
         # when they fit (16-bit indices), else in a global per-column array
         store = 8 * (Nf + sp.max_stage_rows)
@@ -365,7 +381,7 @@ class CircuitPlan:
         if self.face_store == "lds" and not fits:
             raise ValueError(f"circuit with {Nf} faces needs {store} bytes of face store per column; "
                              f"more than one workgroup's LDS ({LDS_BYTES})")
-        self.global_store = self.face_store == "global" or not fits
+        self.global_store = self.face_store == "global" or not fits or self.parts > 1
         self.index_bytes = 4 if self.global_store else 2
         pos = sp.face_pos.astype(np.int64)
         # junctions: face positions of the (up to two) faces, ascending face index
@@ -482,7 +498,7 @@ class CircuitPlan:
         rowbase + l/G. Block 0 of every array is zero (the no-op record); record
         r of the data arrays has base = ibase = 512 r.
         """
-        NW = self.block_size // WAVE
+        NW = self.column_threads // WAVE
         vals = [np.zeros(512)]
         offs = [np.zeros(512, dtype=np.int64)]
         rows_out = [-np.ones(WAVE, dtype=np.int64)]
@@ -598,7 +614,8 @@ class CircuitPlan:
         P = _native.ptr
         d = _native.ProblemDesc()
         d.n_junctions, d.n_faces = self.Nj, self.Nf
-        d.stencil_width, d.block_size = self.stencil_width, self.block_size
+        d.stencil_width, d.block_size = self.stencil_width, self.column_threads
+        d.parts = self.parts
         d.Ic = P(self.Ic, C.c_double)
         d.Rv = P(self.Rv, C.c_double)
         d.coef = P(self.coef, C.c_double)
@@ -638,7 +655,7 @@ class CircuitPlan:
         u (Nj,) = x/c0 - theta_s  ->  J (Nf,) = K^-1 (A u)  (f = 0). Lanes
         accumulate with FMA-free numpy adds of products, so sums agree with the
         kernel to rounding, not bit for bit."""
-        BS, Nf = self.block_size, self.Nf
+        BS, Nf = self.column_threads, self.Nf
         sp = self.solve_plan
         zs = np.zeros(Nf)
         ts = np.zeros(max(sp.max_stage_rows, 1))

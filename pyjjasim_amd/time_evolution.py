@@ -24,6 +24,10 @@ Extensions (keyword-only, defaults keep reference behaviour):
   face_store       "auto" (default): each column keeps its face vector in LDS when
                    it fits (up to about 10^4 faces) and in global memory otherwise;
                    "lds" or "global" force one.
+  parts            Workgroups per column for circuits with a global face store:
+                   None (default) picks the largest power of two up to 16 that
+                   keeps all columns' workgroups resident on the device; 1 keeps
+                   one workgroup per column. Results do not depend on it.
 
 There is no CPU fallback: without a HIP device ``compute()`` raises.
 """
@@ -112,7 +116,7 @@ class TimeEvolutionProblem:
                  store_voltage=True, store_current=True, config_at_minus_1=None,
                  config_at_minus_2=None, config_at_minus_3=None, config_at_minus_4=None,
                  stencil_width=3, *, noise="philox", seed=None, steps_per_launch=None,
-                 block_size=None, leaf_size=16, face_store="auto"):
+                 block_size=None, leaf_size=16, face_store="auto", parts=None):
         self.circuit = circuit
         self.time_step = time_step
         self.time_step_count = time_step_count
@@ -166,6 +170,7 @@ class TimeEvolutionProblem:
         self.block_size = block_size
         self.leaf_size = leaf_size
         self.face_store = face_store
+        self.parts = parts
 
     # ------------------------------------------------------------- getters
     def get_problem_count(self):

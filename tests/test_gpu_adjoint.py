@@ -10,11 +10,13 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-def _circuit():
+def _circuit(inductance=0.0):
     c = pa.SquareArray(5, 4)
     rng = np.random.default_rng(3)
     c.set_resistance(0.7 + 0.6 * rng.random(c.junction_count()))
     c.set_capacitance(0.5 + rng.random(c.junction_count()))
+    if inductance:
+        c.set_inductance(inductance)
     return c
 
 
@@ -28,11 +30,11 @@ def _inputs(c, W, rng, dev):
     return Ic, f, Is
 
 
-@pytest.mark.parametrize("Nt,ck", [(60, 16), (60, 60), (25, 1)])
-def test_adjoint_matches_cpu_autograd(Nt, ck):
+@pytest.mark.parametrize("Nt,ck,L", [(60, 16, 0.0), (60, 60, 0.0), (25, 1, 0.0), (60, 16, 0.5)])
+def test_adjoint_matches_cpu_autograd(Nt, ck, L):
     from adjoint_ref import transcribe
-    print(f"[adjoint] Nt={Nt} checkpoint={ck}", flush=True)
-    c = _circuit()
+    print(f"[adjoint] Nt={Nt} checkpoint={ck} inductance={L}", flush=True)
+    c = _circuit(L)
     W, dt = 3, 0.05
     rng = np.random.default_rng(11)
     store = np.zeros(Nt, dtype=bool)

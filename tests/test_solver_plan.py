@@ -205,3 +205,40 @@ def test_plans_pickle_and_build_in_worker_processes():
         assert _native.lib().jja_plan_validate(1, C.byref(b.desc())) == 0
         c2 = pickle.loads(pickle.dumps(a))
         assert bytes(c2._face_rec) == bytes(a._face_rec)
+
+
+@pytest.mark.parametrize("bs,parts", [(64, 2), (256, 4), (1024, 2)])
+def test_split_column_plans_validate_and_solve(bs, parts):
+    """A column split over `parts` workgroups is laid out for parts * bs threads
+    with the global face store; the library accepts it and its programs solve
+    K J = A u."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    c = _circuits()["tri8"]
+    plan = CircuitPlan(c, 0.05, leaf_size=8, block_size=bs, parts=parts)
+    assert plan.global_store and plan.column_threads == bs * parts
+    d = plan.desc()
+    assert d.block_size == bs * parts and d.parts == parts
+    assert _native.lib().jja_plan_validate(1, C.byref(d)) == 0, _native.lib().jja_last_error()
+    u = np.random.default_rng(5).standard_normal(c.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
+    np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
+    p1 = CircuitPlan(c, 0.05, leaf_size=8, block_size=bs).relayout(bs, parts)
+    assert p1.column_threads == bs * parts and p1.global_store
+
+
+def test_split_column_plans_rejected_where_unsupported():
+    import ctypes as C
+    from pyjjasim_amd import _native
+    c = _circuits()["sq10"]
+    with pytest.raises(ValueError, match="global face store"):
+        CircuitPlan(c, 0.05, block_size=256, parts=2, face_store="lds")
+    with pytest.raises(ValueError, match="power of two"):
+        CircuitPlan(c, 0.05, block_size=256, parts=3)
+    plan = CircuitPlan(c, 0.05, block_size=256, parts=2)
+    d = plan.desc()
+    d.index_bytes = 2                    # split plans need the global store
+    assert _native.lib().jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
+    d = plan.desc()
+    d.parts = 3
+    assert _native.lib().jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
