@@ -257,10 +257,9 @@ def uneven_bench(pa, torch, steps, warmup, plain_steps, n_problems=64):
                        "junctions_total": int(Njs.sum()), "junctions_min": int(Njs.min()),
                        "junctions_max": int(Njs.max()), "largest": names[big],
                        "global_store_problems": int(sum(h.global_store for h in hosts)),
-                       "tuning": {k: (v if k != "block_size_s_per_step" else
-                                      {str(b): t * 1e3 for b, t in v.items()})
+                       "tuning": {k: (v if k != "s_per_step" else {c: t * 1e3 for c, t in v.items()})
                                   for k, v in tb.tuning.items()},
-                       "tuning_unit": "ms per step"},
+                       "tuning_unit": "ms per step, by workgroup size x parts per column"},
             "hbm_GBps_algorithmic": bytes_per_step * steps / wall / 1e9,
             "host_setup_s": setup_s, "plain": plain}
 

@@ -9,10 +9,12 @@ the global face store, in the same launch. Workgroups are ordered largest
 problem first, so the hardware's in-order dispatch is a longest-first list
 schedule.
 
-``autotune=True`` picks the workgroup size and the launch length by measuring:
-each candidate workgroup size is timed on a short run of the batch (the factors
-are computed once; only the record layouts change), and the launch length is
-the shortest whose measured launch overhead stays under 1% of its run time.
+``autotune=True`` picks the workgroup size, the parts per column and the launch
+length by measuring: each candidate (workgroup size, parts) is timed on a short
+run of the batch (the factors are computed once; only the record layouts
+change), and the launch length is the shortest whose measured launch overhead
+stays under 1% of its run time. With parts > 1 every column is worked by that
+many workgroups meeting at column barriers (all face stores global).
 """
 from __future__ import annotations
 
@@ -23,7 +25,7 @@ from concurrent.futures import ProcessPoolExecutor
 
 import numpy as np
 
-from .engine import BatchEngine, _torch, default_block_size
+from .engine import BatchEngine, _torch, default_block_size, device_compute_units
 from .solver_plan import CircuitPlan
 from .time_evolution import TimeEvolutionProblem, _finish, _masks
 
@@ -75,7 +77,7 @@ def build_plans(problems, block_size, workers=None):
 
 class TimeEvolutionBatch:
     def __init__(self, problems, block_size=None, steps_per_launch=None, autotune=False,
-                 block_sizes=(256, 512, 1024), tune_steps=40, workers=None):
+                 block_sizes=(256, 512, 1024), tune_steps=40, workers=None, parts=1, max_parts=16):
         self.problems = list(problems)
         if not self.problems:
             raise ValueError("TimeEvolutionBatch needs at least one problem")
@@ -87,22 +89,39 @@ class TimeEvolutionBatch:
         self.block_sizes = tuple(block_sizes)
         self.tune_steps = int(tune_steps)
         self.workers = workers
+        self.parts = int(parts)
+        self.max_parts = int(max_parts)
         self.tuning = None
 
     # --------------------------------------------------------------- tuning
     def _engine(self, hosts, masks):
         return BatchEngine(self.problems, masks, hosts=hosts)
 
+    def _candidates(self):
+        """(workgroup size, parts) pairs to time: every block size with one
+        workgroup per column, and 1024-thread workgroups with 2, 4, ... parts per
+        column (all face stores global) while every part stays resident."""
+        out = [(bs, 1) for bs in self.block_sizes]
+        if any(p.stencil_width != 3 or getattr(p, "face_store", "auto") == "lds" for p in self.problems):
+            return out
+        cols = sum(p.problem_count for p in self.problems)
+        n_cu = device_compute_units()
+        parts = 2
+        while parts <= self.max_parts and parts * cols <= n_cu:
+            out.append((1024, parts))
+            parts *= 2
+        return out
+
     def _tune(self, masks):
-        """Time each candidate workgroup size on tune_steps steps; returns
-        (block_size, hosts, record)."""
+        """Time each candidate (workgroup size, parts) on tune_steps steps;
+        returns (block_size, parts, hosts, record)."""
         torch = _torch()
         base = build_plans(self.problems, self.block_sizes[0], self.workers)
         steps = min(self.tune_steps, self.problems[0]._Nt())
         record = {}
         best = None
-        for bs in self.block_sizes:
-            hosts = [h if h.block_size == bs else h.relayout(bs) for h in base]
+        for bs, parts in self._candidates():
+            hosts = [h if (h.block_size, h.parts) == (bs, parts) else h.relayout(bs, parts) for h in base]
             eng = self._engine(hosts, masks)
             try:
                 eng.launch(1)                            # first launch warms up
@@ -113,10 +132,10 @@ class TimeEvolutionBatch:
                 dt = (time.perf_counter() - t0) / steps
             finally:
                 eng.close()
-            record[bs] = dt
-            if best is None or dt < record[best[0]]:
-                best = (bs, hosts)
-        return best[0], best[1], record
+            record[(bs, parts)] = dt
+            if best is None or dt < record[best[:2]]:
+                best = (bs, parts, hosts)
+        return best[0], best[1], best[2], record
 
     def _launch_length(self, eng):
         """Shortest launch length whose measured fixed cost per launch stays below
@@ -144,13 +163,16 @@ class TimeEvolutionBatch:
         if masks is None:
             masks = [_masks(p)[2:4] for p in self.problems]
         if self.autotune:
-            bs, hosts, record = self._tune(masks)
-            self.block_size = bs
-            self.tuning = {"block_size_s_per_step": record, "block_size": bs}
+            bs, parts, hosts, record = self._tune(masks)
+            self.block_size, self.parts = bs, parts
+            self.tuning = {"s_per_step": {f"{b}x{q}": t for (b, q), t in record.items()},
+                           "block_size": bs, "parts": parts}
         else:
             self.block_size = self.block_size or max(
                 default_block_size(p.circuit._Nj(), p.circuit._Nf()) for p in self.problems)
             hosts = build_plans(self.problems, self.block_size, self.workers)
+            if self.parts > 1:
+                hosts = [h.relayout(self.block_size, self.parts) for h in hosts]
         eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=hosts)
         try:
             if self.autotune and not self.steps_per_launch:

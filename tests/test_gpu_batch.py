@@ -48,7 +48,8 @@ def test_gpu_batch_autotune():
     probs = _problems()[:2]
     tb = pa.TimeEvolutionBatch(probs, autotune=True, block_sizes=(128, 256), tune_steps=10)
     batch = tb.compute()
-    assert tb.tuning["block_size"] in (128, 256)
+    assert (tb.tuning["block_size"], tb.tuning["parts"]) in [(128, 1), (256, 1)] + \
+        [(1024, q) for q in (2, 4, 8, 16)]
     assert 100 <= tb.tuning["steps_per_launch"] <= 4000
     for p, rb in zip(probs, batch):
         _same(rb, p.compute())
@@ -72,3 +73,13 @@ def test_gpu_separable_source_matches_oracle():
                                     [zero, zero])
     th = r.get_theta()
     assert np.abs(th - th_o[:, :, 2:]).max() <= 1e-9 * np.abs(th_o).max()
+
+
+def test_gpu_batch_split_columns_match_individual_runs():
+    """Every column of the batch worked by four workgroups (all face stores
+    global): the results of each problem's own compute()."""
+    print("[batch] four parts per column", flush=True)
+    probs = _problems()
+    batch = pa.TimeEvolutionBatch(probs, block_size=256, parts=4).compute()
+    for p, rb in zip(probs, batch):
+        _same(rb, p.compute())
