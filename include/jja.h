@@ -146,10 +146,11 @@ typedef struct jja_problem_desc {
   int32_t index_bytes;
   /* workgroups per column: 1, or 2..32 (a power of two) with index_bytes 4. A
    * column of a split problem is worked by `parts` workgroups of
-   * block_size/parts threads that meet at column barriers in global memory
-   * (jja_evolve and jja_adjoint then launch cooperatively: all workgroups of
-   * the launch must be resident at once). Every problem of a plan has the same
-   * block_size/parts. */
+   * block_size/parts threads that meet at column barriers in global memory.
+   * Every problem of a plan has the same block_size/parts; part counts may
+   * differ, but once any problem is split, every problem of the plan needs
+   * index_bytes 4, and jja_evolve / jja_adjoint launch cooperatively (all
+   * workgroups of the launch resident at once, or JJA_EUNSUPPORTED). */
   int32_t parts;
 } jja_problem_desc;
 

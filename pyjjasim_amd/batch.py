@@ -18,6 +18,7 @@ many workgroups meeting at column barriers (all face stores global).
 """
 from __future__ import annotations
 
+import heapq
 import multiprocessing
 import os
 import time
@@ -33,6 +34,24 @@ __all__ = ["TimeEvolutionBatch", "build_plans"]
 
 
 _THREAD_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "BLIS_NUM_THREADS")
+
+
+def balanced_parts(hosts, columns, n_cu, max_parts=16):
+    """Parts per problem for one split launch: all start at one, and the problem
+    whose columns take longest (factor entries / parts) has its parts doubled
+    while every workgroup of the launch still fits on the device."""
+    parts = [1] * len(hosts)
+    used = sum(columns)
+    heap = [(-float(len(h.ent_val) + 8 * h.Nj), i) for i, h in enumerate(hosts)]
+    heapq.heapify(heap)
+    while heap:
+        _, i = heapq.heappop(heap)
+        if parts[i] >= max_parts or used + columns[i] * parts[i] > n_cu:
+            break                      # the slowest problem cannot speed up further
+        used += columns[i] * parts[i]
+        parts[i] *= 2
+        heapq.heappush(heap, (-float(len(hosts[i].ent_val) + 8 * hosts[i].Nj) / parts[i], i))
+    return parts
 
 
 def _plan_one(args):
@@ -89,7 +108,7 @@ class TimeEvolutionBatch:
         self.block_sizes = tuple(block_sizes)
         self.tune_steps = int(tune_steps)
         self.workers = workers
-        self.parts = int(parts)
+        self.parts = parts if parts == "balanced" else int(parts)
         self.max_parts = int(max_parts)
         self.tuning = None
 
@@ -110,7 +129,22 @@ class TimeEvolutionBatch:
         while parts <= self.max_parts and parts * cols <= n_cu:
             out.append((1024, parts))
             parts *= 2
+        if len(self.problems) > 1:
+            out.append((1024, "balanced"))
         return out
+
+    def _hosts_for(self, base, bs, parts):
+        """Plans laid out for workgroups of bs threads and the given parts: an
+        int for every problem, or "balanced" (balanced_parts). With any split
+        column, every problem uses the global face store."""
+        if parts == "balanced":
+            cols = [p.problem_count for p in self.problems]
+            per = balanced_parts(base, cols, device_compute_units(), self.max_parts)
+        else:
+            per = [int(parts)] * len(base)
+        split = max(per) > 1
+        return [h if (h.block_size, h.parts) == (bs, q) and (h.global_store or not split)
+                else h.relayout(bs, q, "global" if split else None) for h, q in zip(base, per)]
 
     def _tune(self, masks):
         """Time each candidate (workgroup size, parts) on tune_steps steps;
@@ -121,7 +155,7 @@ class TimeEvolutionBatch:
         record = {}
         best = None
         for bs, parts in self._candidates():
-            hosts = [h if (h.block_size, h.parts) == (bs, parts) else h.relayout(bs, parts) for h in base]
+            hosts = self._hosts_for(base, bs, parts)
             eng = self._engine(hosts, masks)
             try:
                 eng.launch(1)                            # first launch warms up
@@ -166,13 +200,14 @@ class TimeEvolutionBatch:
             bs, parts, hosts, record = self._tune(masks)
             self.block_size, self.parts = bs, parts
             self.tuning = {"s_per_step": {f"{b}x{q}": t for (b, q), t in record.items()},
-                           "block_size": bs, "parts": parts}
+                           "block_size": bs, "parts": parts,
+                           "parts_per_problem": [h.parts for h in hosts]}
         else:
             self.block_size = self.block_size or max(
                 default_block_size(p.circuit._Nj(), p.circuit._Nf()) for p in self.problems)
             hosts = build_plans(self.problems, self.block_size, self.workers)
-            if self.parts > 1:
-                hosts = [h.relayout(self.block_size, self.parts) for h in hosts]
+            if self.parts == "balanced" or self.parts > 1:
+                hosts = self._hosts_for(hosts, self.block_size, self.parts)
         eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=hosts)
         try:
             if self.autotune and not self.steps_per_launch:

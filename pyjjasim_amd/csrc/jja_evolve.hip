@@ -257,6 +257,10 @@ struct SplitStore {
   // workgroup's waits for the rest of the launch (its results are then
   // invalid, and the host reports the error).
   __device__ __forceinline__ void sync() const {
+    if (parts == 1) {  // a column of one workgroup (in a launch with split columns)
+      __syncthreads();
+      return;
+    }
     __builtin_amdgcn_s_waitcnt(kWaitVmem);
     __syncthreads();
     if (threadIdx.x == 0 && !*stopped) {
@@ -1061,6 +1065,28 @@ void free_plan(Plan* plan) {
 
 bool pow2_le(int32_t g, int32_t limit) { return g >= 1 && g <= limit && (g & (g - 1)) == 0; }
 
+int32_t parts_of(const jja_problem_desc& d) { return d.parts > 0 ? d.parts : 1; }
+
+// The problems of one plan share the stencil width and the workgroup size
+// (block_size / parts); the part count may differ between problems, but if any
+// column is split, every problem keeps its face store in global memory (one
+// launch runs them all).
+int check_plan_shapes(int32_t n, const jja_problem_desc* descs) {
+  bool split = false;
+  for (int32_t p = 0; p < n; ++p) split = split || parts_of(descs[p]) > 1;
+  for (int32_t p = 0; p < n; ++p) {
+    const jja_problem_desc& d = descs[p];
+    if (d.stencil_width != descs[0].stencil_width)
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
+    if (parts_of(d) <= 0 || d.block_size % parts_of(d) ||
+        d.block_size / parts_of(d) != descs[0].block_size / parts_of(descs[0]))
+      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size/parts");
+    if (split && d.index_bytes != 4)
+      return fail(JJA_EINVAL, "jja_plan_create: with a split column, every problem needs the global face store");
+  }
+  return JJA_OK;
+}
+
 // Every index the kernels follow is checked here, once, on the host.
 int validate(const jja_problem_desc& d) {
   const int32_t Nj = d.n_junctions, Nf = d.n_faces, BS = d.block_size, ms = d.max_stage_rows;
@@ -1261,20 +1287,38 @@ void order_work(const Plan* plan, std::vector<WorkItem>& work) {
 }
 
 // Work items of a split launch, from one item per column: every part of every
-// column, each column with its own barrier counter (slot). When the column
-// count is a multiple of 8, the parts of a column sit at block indices equal
-// modulo 8: workgroups go to the eight XCDs in turn by block index, so the
-// parts of a column then share one L2.
+// column (its problem's part count), each column with its own barrier counter
+// (slot), largest problems first. When all columns have the same part count and
+// the column count is a multiple of 8, the parts of a column sit at block
+// indices equal modulo 8: workgroups go to the eight XCDs in turn by block
+// index, so the parts of a column then share one L2.
 std::vector<WorkItem> split_work(const Plan* plan, std::vector<WorkItem> cols) {
   order_work(plan, cols);
   constexpr size_t kXcd = 8;
-  const size_t G = static_cast<size_t>(plan->parts), nc = cols.size();
-  std::vector<WorkItem> out(nc * G);
-  for (size_t c = 0; c < nc; ++c)
-    for (size_t g = 0; g < G; ++g) {
-      const size_t b = nc % kXcd == 0 ? ((c / kXcd) * G + g) * kXcd + c % kXcd : c * G + g;
-      out[b] = {cols[c].problem, static_cast<int32_t>(cols[c].column * G + g), static_cast<int32_t>(c)};
+  const size_t nc = cols.size();
+  bool uniform = true;
+  size_t total = 0;
+  for (const WorkItem& c : cols) {
+    const size_t G = static_cast<size_t>(plan->problems[c.problem].parts);
+    uniform = uniform && G == static_cast<size_t>(plan->problems[cols[0].problem].parts);
+    total += G;
+  }
+  std::vector<WorkItem> out;
+  out.reserve(total);
+  if (uniform && nc % kXcd == 0) {
+    const size_t G = static_cast<size_t>(plan->problems[cols[0].problem].parts);
+    out.resize(total);
+    for (size_t c = 0; c < nc; ++c)
+      for (size_t g = 0; g < G; ++g)
+        out[((c / kXcd) * G + g) * kXcd + c % kXcd] = {
+            cols[c].problem, static_cast<int32_t>(cols[c].column * G + g), static_cast<int32_t>(c)};
+  } else {
+    for (size_t c = 0; c < nc; ++c) {
+      const int32_t G = plan->problems[cols[c].problem].parts;
+      for (int32_t g = 0; g < G; ++g)
+        out.push_back({cols[c].problem, cols[c].column * G + g, static_cast<int32_t>(c)});
     }
+  }
   return out;
 }
 
@@ -1368,28 +1412,20 @@ const char* jja_last_error(void) { return g_error.c_str(); }
 
 int jja_plan_validate(int32_t n_problems, const jja_problem_desc* descs) {
   if (n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_validate: invalid arguments");
-  for (int32_t p = 0; p < n_problems; ++p) {
-    if (descs[p].block_size != descs[0].block_size || descs[p].parts != descs[0].parts)
-      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size and parts");
-    if (descs[p].stencil_width != descs[0].stencil_width)
-      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
+  if (int rc = check_plan_shapes(n_problems, descs)) return rc;
+  for (int32_t p = 0; p < n_problems; ++p)
     if (int rc = validate(descs[p])) return rc;
-  }
   return JJA_OK;
 }
 
 int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan** out) {
   if (!out || n_problems <= 0 || !descs) return fail(JJA_EINVAL, "jja_plan_create: invalid arguments");
   *out = nullptr;
-  for (int32_t p = 1; p < n_problems; ++p)
-    if (descs[p].block_size != descs[0].block_size || descs[p].parts != descs[0].parts)
-      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size and parts");
-  for (int32_t p = 1; p < n_problems; ++p)
-    if (descs[p].stencil_width != descs[0].stencil_width)
-      return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same stencil width");
+  if (int rc = check_plan_shapes(n_problems, descs)) return rc;
   Plan* plan = new Plan();
-  plan->parts = descs[0].parts > 0 ? descs[0].parts : 1;
-  plan->block_size = descs[0].block_size / plan->parts;
+  plan->parts = 1;
+  for (int32_t p = 0; p < n_problems; ++p) plan->parts = std::max(plan->parts, parts_of(descs[p]));
+  plan->block_size = descs[0].block_size / parts_of(descs[0]);
   plan->stencil_width = descs[0].stencil_width;
   int rc = JJA_OK;
   hipDeviceProp_t prop;
@@ -1409,7 +1445,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.Nf = d.n_faces;
     dp.sw = d.stencil_width;
     dp.BS = d.block_size;
-    dp.parts = plan->parts;
+    dp.parts = parts_of(d);
     dp.n_face_slices = d.n_face_slices;
     dp.n_solve_slices = d.n_solve_slices;
     dp.max_stage_rows = d.max_stage_rows;

@@ -355,15 +355,20 @@ class CircuitPlan:
         self.target_iter = target_iter
         self._build_layouts(target_iter)
 
-    def relayout(self, block_size, parts=None):
+    def relayout(self, block_size, parts=None, face_store=None):
         """The same circuit, factor and schedule laid out for another workgroup
-        size or part count (the factorisation does not depend on them)."""
+        size, part count or face store (the factorisation depends on none of
+        them)."""
         other = copy.copy(self)
         other.block_size = block_size
+        if face_store is not None:
+            if face_store not in ("auto", "lds", "global"):
+                raise ValueError("face_store must be 'auto', 'lds' or 'global'")
+            other.face_store = face_store
         if parts is not None:
-            if parts > 1 and self.face_store == "lds":
-                raise ValueError("a column split over workgroups needs the global face store")
             other.parts = parts
+        if other.parts > 1 and other.face_store == "lds":
+            raise ValueError("a column split over workgroups needs the global face store")
         other._build_layouts(self.target_iter)
         return other
 

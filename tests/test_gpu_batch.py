@@ -83,3 +83,14 @@ def test_gpu_batch_split_columns_match_individual_runs():
     batch = pa.TimeEvolutionBatch(probs, block_size=256, parts=4).compute()
     for p, rb in zip(probs, batch):
         _same(rb, p.compute())
+
+
+def test_gpu_batch_balanced_parts_match_individual_runs():
+    """Problems of one launch with different part counts (the largest split,
+    the others whole, all face stores global): each problem's own results."""
+    print("[batch] balanced parts per problem", flush=True)
+    probs = _problems()
+    tb = pa.TimeEvolutionBatch(probs, block_size=256, parts="balanced", max_parts=8)
+    batch = tb.compute()
+    for p, rb in zip(probs, batch):
+        _same(rb, p.compute())

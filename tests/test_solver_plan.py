@@ -242,3 +242,35 @@ def test_split_column_plans_rejected_where_unsupported():
     d = plan.desc()
     d.parts = 3
     assert _native.lib().jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
+
+
+def test_balanced_parts_allocation():
+    """The slowest problem gets its parts doubled while the device holds every
+    workgroup; small problems stay at one part."""
+    from types import SimpleNamespace
+    from pyjjasim_amd.batch import balanced_parts
+    hosts = [SimpleNamespace(ent_val=np.zeros(n), Nj=0) for n in (64000, 16000, 1000, 1000)]
+    parts = balanced_parts(hosts, [1, 1, 1, 1], n_cu=16, max_parts=16)
+    # 1,1,1,1 (4 workgroups) -> first 2, 4, 8 for the largest (cost 64000 / parts),
+    # then 2 for the second; doubling the largest again would need 20 > 16
+    assert parts == [8, 2, 1, 1]
+    # with no room for splits nothing changes
+    assert balanced_parts(hosts, [4, 4, 4, 4], n_cu=16) == [1, 1, 1, 1]
+
+
+def test_mixed_part_plans_validate():
+    """Problems of one plan may have different part counts (same workgroup
+    size); once one is split, all need the global face store."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    a = CircuitPlan(_circuits()["sq10"], 0.05, block_size=256, parts=4)
+    b = CircuitPlan(_circuits()["hc7"], 0.05, block_size=256, face_store="global")
+    descs = (_native.ProblemDesc * 2)(a.desc(), b.desc())
+    assert _native.lib().jja_plan_validate(2, descs) == 0, _native.lib().jja_last_error()
+    c = CircuitPlan(_circuits()["hc7"], 0.05, block_size=256)          # LDS store
+    assert not c.global_store
+    descs = (_native.ProblemDesc * 2)(a.desc(), c.desc())
+    assert _native.lib().jja_plan_validate(2, descs) == _native.JJA_EINVAL
+    d = CircuitPlan(_circuits()["hc7"], 0.05, block_size=128, face_store="global")
+    descs = (_native.ProblemDesc * 2)(a.desc(), d.desc())
+    assert _native.lib().jja_plan_validate(2, descs) == _native.JJA_EINVAL
