@@ -350,11 +350,11 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, __amdgpu_buffer_
 template <class Z>
 __device__ __forceinline__ void process_slice(const Z& zs, int32_t Nf, int lane, const jja_slice& m,
                                               const SliceRegs<Z::kGlobal>& r, double& acc) {
-  // iterations at or past the record's count have zero values: their gathers
-  // are skipped (a uniform branch) and contribute an exact zero product
+  // iterations at or past the record's count have zero values and index 0:
+  // their gathers read one broadcast address and contribute exact zeros
   double zv[kMaxIter];
 #pragma unroll
-  for (int i = 0; i < kMaxIter; ++i) zv[i] = (i < 4 || i < m.n_iter) ? zs.gather(slice_index(r, i)) : 0.0;
+  for (int i = 0; i < kMaxIter; ++i) zv[i] = zs.gather(slice_index(r, i));
 #pragma unroll
   for (int i = 0; i < kMaxIter; ++i) acc = __builtin_fma(r.v[i], zv[i], acc);
   if (m.flags & JJA_SLICE_FINISH) {
