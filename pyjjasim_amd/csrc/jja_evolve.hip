@@ -157,8 +157,10 @@ __device__ __forceinline__ void normal_pair(uint64_t seed, uint32_t j, uint32_t 
   const double u2 = static_cast<double>(b) * 0x1.0p-53;
   const double rho = sqrt(-2.0 * log(u1));
   const double ang = kTwoPi * u2;
-  *z0 = rho * cos(ang);
-  *z1 = rho * sin(ang);
+  double sn, cs;
+  jja_sincos(ang, &sn, &cs);  // one argument reduction for both
+  *z0 = rho * cs;
+  *z1 = rho * sn;
 }
 
 // One 64-bit DPP move (two 32-bit halves); rows outside ROW_MASK and lanes
