@@ -111,6 +111,47 @@ def cpu_baseline(pa, n, W, budget_s):
                        f"{t:.1f} s total")
 
 
+def _cpu_slice(args):
+    """One worker of the all-cores baseline: the oracle on columns w0..w1 of the
+    bench's IV sweep for `steps` steps; returns the evolve call's seconds."""
+    n, W, w0, w1, steps = args
+    import pyjjasim_amd as pa
+    from oracle import te_oracle as to
+    c, _ = make_problem(pa, n, W, 10)
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.0, 2.0, W)[None, w0:w1, None]
+    np.random.seed(w0)
+    t0 = time.perf_counter()
+    to.evolve(c, dt=0.05, Nt=steps, f=0.25, Is=Is, T=0.01, store_time_steps=[steps - 1],
+              store_current=False, store_voltage=False)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(n, W, steps, max_workers=16):
+    """The oracle with the W columns sharded over worker processes (one thread
+    each), as the reference's own multiprocessing example shards work; the
+    rate uses the slowest worker's time (process start-up excluded)."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    workers = max(1, min(max_workers, len(os.sched_getaffinity(0)), W))
+    bounds = np.linspace(0, W, workers + 1).astype(int)
+    jobs = [(n, W, int(a), int(b), steps) for a, b in zip(bounds[:-1], bounds[1:]) if b > a]
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    os.environ.update({k: "1" for k in env})
+    try:
+        with ProcessPoolExecutor(max_workers=len(jobs), mp_context=multiprocessing.get_context("spawn")) as ex:
+            times = list(ex.map(_cpu_slice, jobs))
+    finally:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    Nj = 2 * n * (n - 1)
+    return dict(value=Nj * W * steps / max(times), unit="junction-steps/s", cores=len(jobs), kind="port",
+                sample=f"{steps} steps of SquareArray({n},{n}) W={W} T=0.01, columns sharded over "
+                       f"{len(jobs)} single-threaded processes; slowest worker {max(times):.1f} s")
+
+
 def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=5, fd_steps=200):
     """Differentiation workload: SquareArray(n, n), trainable per-junction Ic
     (log-normal, sigma 0.2, seed A), loss = mean over W bias points (0.5..2.0 Ic)
@@ -449,6 +490,8 @@ def main():
         out["adjoint"] = adjoint_bench(pa, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pa, args.n, W, args.cpu_seconds)
+        steps = int(out["cpu_baseline"]["sample"].split()[0])
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(args.n, W, steps)
     print(json.dumps(out), flush=True)
 
 

@@ -90,7 +90,7 @@ class DifferentiableEvolution:
         self.n_seg = -(-self.Nt // self.C)
 
     def __call__(self, Ic, f, Is):
-        return _EvolveFn.apply(Ic, f, Is, self)
+        return _apply(Ic, f, Is, self)
 
     # ------------------------------------------------------------------ forward
     def _set_inputs(self, Ic, f, Is):
@@ -174,10 +174,6 @@ class DifferentiableEvolution:
         return gIc.sum(0), gf_face, gIs
 
 
-class _EvolveFn:
-    pass
-
-
 def _make_fn():
     torch = _torch()
 
@@ -198,14 +194,10 @@ def _make_fn():
     return EvolveFn
 
 
-class _Lazy:
-    fn = None
+_FN = []   # the autograd Function, made on first use (torch imports lazily)
 
 
 def _apply(Ic, f, Is, evo):
-    if _Lazy.fn is None:
-        _Lazy.fn = _make_fn()
-    return _Lazy.fn.apply(Ic, f, Is, evo)
-
-
-_EvolveFn.apply = staticmethod(_apply)
+    if not _FN:
+        _FN.append(_make_fn())
+    return _FN[0].apply(Ic, f, Is, evo)

@@ -1,13 +1,15 @@
-"""Many time-evolution problems in one persistent launch per chunk of steps.
+"""Many time-evolution problems in one launch per chunk of steps.
 
 ``TimeEvolutionBatch(problems).compute()`` returns one TimeEvolutionResult per
 problem, each identical to ``problem.compute()``: the problems (any circuits,
 sizes and column counts; one stencil width and step count) share one libjja
-plan, and every launch advances the columns of all of them, one workgroup per
-column. Circuits whose face vector fits in LDS keep it there; larger ones use
-the global face store, in the same launch. Workgroups are ordered largest
-problem first, so the hardware's in-order dispatch is a longest-first list
-schedule.
+plan, and every launch advances the columns of all of them. By default a
+column is one workgroup: circuits whose face vector fits in LDS keep it there,
+larger ones use the global face store, in the same launch. With parts > 1 (or
+"balanced", a part count per problem) large problems' columns are split over
+several workgroups, and every face store is global. Workgroups are ordered
+largest problem first, so the hardware's in-order dispatch is a longest-first
+list schedule.
 
 ``autotune=True`` picks the workgroup size, the parts per column and the launch
 length by measuring: each candidate (workgroup size, parts) is timed on a short
@@ -95,6 +97,13 @@ def build_plans(problems, block_size, workers=None):
 
 
 class TimeEvolutionBatch:
+    """problems: TimeEvolutionProblem objects (one step count and stencil width).
+    block_size: threads per workgroup (default sized by the largest circuit).
+    steps_per_launch: steps per launch (default the problems' own).
+    autotune: time the (workgroup size, parts) candidates and the launch length.
+    parts: workgroups per column, 1, a power of two, or "balanced" (per problem).
+    workers: processes that build the host plans (default by batch size)."""
+
     def __init__(self, problems, block_size=None, steps_per_launch=None, autotune=False,
                  block_sizes=(256, 512, 1024), tune_steps=40, workers=None, parts=1, max_parts=16):
         self.problems = list(problems)

@@ -332,7 +332,9 @@ class Engine:
 class BatchEngine:
     """Several problems (any circuits and column counts, one stencil width and
     step count) in one plan: every launch advances all their columns together,
-    one workgroup per column (time_evolution.py:461-582 for each problem)."""
+    each column worked by its problem's part count of workgroups
+    (time_evolution.py:461-582 for each problem). ``hosts`` are prepared plans
+    (CircuitPlan, possibly with parts > 1); otherwise plans are built here."""
 
     def __init__(self, problems, masks, block_size=None, hosts=None):
         if not problems:
