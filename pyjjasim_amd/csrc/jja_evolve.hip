@@ -115,8 +115,14 @@ struct DevProblem {
   const double* zero;       // 8 zero doubles: absent input terms read here
   const int32_t* face_fid;  // [n_slice_rows]: face index of a face record's row slot, or -1
   int32_t parts;            // workgroups per column (1, or more with a global face store)
-  int32_t pad_;
+  int32_t coef_si;          // junction stride of coef: 1, or 0 when every row is one value
 };
+
+// Stencil coefficient k of junction j (an identical value for every junction
+// is read from one address).
+__device__ __forceinline__ double coef_at(const DevProblem& P, int k, int32_t j) {
+  return gbl(P.coef)[static_cast<int64_t>(k) * P.Nj + static_cast<int64_t>(j) * P.coef_si];
+}
 
 struct WorkItem {
   int32_t problem;
@@ -537,9 +543,7 @@ struct JunctionStep {
 template <int SW, bool UPDATE, bool PREP, class Z>
 __device__ __forceinline__ void junction_one(const DevProblem& P, const JunctionStep& J,
                                              const Z& zs, int32_t j, double zj) {
-  const int32_t Nj = P.Nj;
-  const GAS double* __restrict__ coef = gbl(P.coef);
-  const double c0 = coef[j];
+  const double c0 = coef_at(P, 0, j);
   vint2 code{-1, -1};
   double xo = 0.0, isn = 0.0, thsv = 0.0, vsv = 0.0;
   if (UPDATE) {
@@ -555,7 +559,7 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
     for (int k = UPDATE ? 1 : 0; k < SW - 1; ++k) t[k] = J.th_k[k][j];
     Ic = J.Ic[j];
 #pragma unroll
-    for (int k = 1; k < SW; ++k) c[k] = coef[static_cast<int64_t>(k) * Nj + j];
+    for (int k = 1; k < SW; ++k) c[k] = coef_at(P, k, j);
     isx = term_at(J.Is_x, j);
     sg = term_at(J.sig_x, j);
     if (J.from_T) rv = gbl(P.Rv)[j];
@@ -658,14 +662,12 @@ struct JFast {
 
 __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionStep& J, int32_t j,
                                            JFast& L) {
-  const int32_t Nj = P.Nj;
-  const GAS double* __restrict__ coef = gbl(P.coef);
   L.code = gbl(reinterpret_cast<const vint2*>(P.junc_code))[j];
-  L.c0 = coef[j];
+  L.c0 = coef_at(P, 0, j);
   L.xo = J.th_n[j];
   L.t2 = J.th_k[1][j];
-  L.c1 = coef[static_cast<int64_t>(Nj) + j];
-  L.c2 = coef[2 * static_cast<int64_t>(Nj) + j];
+  L.c1 = coef_at(P, 1, j);
+  L.c2 = coef_at(P, 2, j);
   L.Ic = J.Ic[j];
   L.isx = term_at(J.Is_x, j);
   L.sg = term_at(J.sig_x, j);
@@ -875,12 +877,11 @@ __device__ __forceinline__ double adj_g(const jja_adjoint_run& A, const AdjCol& 
 template <class Z>
 __device__ __forceinline__ void adjoint_prime(const DevProblem& P, const jja_adjoint_run& A,
                                               const AdjCol& C, const Z& zs, int64_t k) {
-  const GAS double* __restrict__ c0 = gbl(P.coef);
   GAS double* __restrict__ lk = C.lam + (k & 1) * C.stride;
   for (int32_t j = zs.vt; j < P.Nj; j += zs.vbs) {
     const double l = lk[j] + adj_g(A, C, k, j);
     lk[j] = l;
-    C.u[j] = l / c0[j];
+    C.u[j] = l / coef_at(P, 0, j);
   }
 }
 
@@ -892,7 +893,6 @@ template <class Z>
 __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja_adjoint_run& A,
                                                   const AdjCol& C, const Z& zs, int64_t k, int64_t w) {
   const int32_t Nj = P.Nj;
-  const GAS double* __restrict__ coef = gbl(P.coef);
   const GAS vint2* __restrict__ jcode = gbl(reinterpret_cast<const vint2*>(P.junc_code));
   GAS double* __restrict__ lk = C.lam + (k & 1) * C.stride;
   GAS double* __restrict__ lk1 = C.lam + ((k - 1) & 1) * C.stride;
@@ -902,7 +902,7 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
   GAS double* __restrict__ gIs = gblw(A.grad_Is) + static_cast<size_t>(w) * Nj;
   for (int32_t j = zs.vt; j < Nj; j += zs.vbs) {
     const vint2 code = jcode[j];
-    const double c0 = coef[j], c1 = coef[static_cast<int64_t>(Nj) + j], c2 = coef[2 * static_cast<int64_t>(Nj) + j];
+    const double c0 = coef_at(P, 0, j), c1 = coef_at(P, 1, j), c2 = coef_at(P, 2, j);
     const double lam = lk[j], lprev = lk1[j], t1 = h1[j], t2 = h2[j], Ic = C.Ic[j];
     const double gj = adj_g(A, C, k - 1, j);
     const double gi = gIc[j], gs = gIs[j];
@@ -1458,6 +1458,13 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
     if ((rc = upload(plan, d.coef, Nj * d.stencil_width, &dp.coef))) break;
+    dp.coef_si = 0;  // every coefficient row one value (uniform R and C): read one address
+    for (int32_t k = 0; k < d.stencil_width && !dp.coef_si; ++k)
+      for (size_t j = 1; j < Nj; ++j)
+        if (d.coef[k * Nj + j] != d.coef[k * Nj]) {
+          dp.coef_si = 1;
+          break;
+        }
     if ((rc = upload(plan, d.junc_code, 2 * Nj, &dp.junc_code))) break;
     if (Nf) {
       if ((rc = upload(plan, d.face_of_pos, Nf, &dp.face_of_pos))) break;

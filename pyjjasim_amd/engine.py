@@ -202,8 +202,11 @@ class Engine:
         if not T_src.src.zero and not T_src.callable and not problem._T_is_timedep:
             Traw = np.broadcast_to(np.array(problem._raw["T"], dtype=np.float64), (Nj, W, Nt))[:, :, 0]
             sig = np.sqrt(2.0 * Traw * self.host.Rv[:, None])
-            self.sigma = DeviceSource(torch, sig[:, :, None] if np.ptp(sig, axis=1).any() else sig[:, :1, None],
-                                      Nj, W, Nt, False)
+            if not np.ptp(sig):
+                sig = sig[:1, :1]                 # one value: every junction reads one address
+            elif not np.ptp(sig, axis=1).any():
+                sig = sig[:, :1]
+            self.sigma = DeviceSource(torch, sig[:, :, None], Nj, W, Nt, False)
         self.th_mask, self.I_mask = th_mask, I_mask
         th_slot = np.where(th_mask, np.cumsum(th_mask) - 1, -1).astype(np.int32)
         I_slot = np.where(I_mask, np.cumsum(I_mask) - 1, -1).astype(np.int32)
