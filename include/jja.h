@@ -222,9 +222,10 @@ typedef struct jja_problem_run {
 /*
  * Advance every column of every problem by n_steps steps, starting at absolute
  * step step0, on `stream` (a hipStream_t; NULL = default stream). One
- * workgroup of the plan's block_size threads integrates one column for all
- * n_steps steps (persistent launch). All problems of a plan share one
- * block_size.
+ * workgroup of block_size/parts threads (parts workgroups, for a split
+ * problem) integrates one column for all n_steps steps, largest problems
+ * first. A plan with any split problem launches cooperatively; check
+ * jja_plan_status after its runs.
  */
 int jja_evolve(jja_plan* plan, const jja_problem_run* runs, int32_t n_steps, int64_t step0,
                void* stream);
