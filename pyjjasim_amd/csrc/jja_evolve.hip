@@ -773,9 +773,11 @@ __device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_prob
   zs.sync();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
-    face_phase<BS>(P, &R.f, u, zs, w, t);
-    zs.barrier();
-    solve(P, zs);
+    if (P.Nf > 0) {  // a circuit without faces has nothing to solve (K is 0 x 0)
+      face_phase<BS>(P, &R.f, u, zs, w, t);
+      zs.barrier();
+      solve(P, zs);
+    }
     // finish step t and prepare step t+1 in one pass
     junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     zs.sync();
@@ -947,9 +949,11 @@ __device__ __forceinline__ void adjoint_steps(const DevProblem& P, const jja_adj
   if (A.prime) adjoint_prime(P, A, C, zs, k_hi);
   zs.sync();
   for (int64_t k = k_hi; k >= k_lo; --k) {
-    face_phase<BS>(P, nullptr, C.u, zs, w, k);
-    zs.barrier();
-    solve(P, zs);
+    if (P.Nf > 0) {
+      face_phase<BS>(P, nullptr, C.u, zs, w, k);
+      zs.barrier();
+      solve(P, zs);
+    }
     for (int32_t p = zs.vt; p < P.Nf; p += zs.vbs) gf[p] = gf[p] - kTwoPi * zs.rd(p);
     adjoint_junctions(P, A, C, zs, k, w);
     zs.sync();
@@ -1597,9 +1601,11 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
   // ordered after earlier work on the stream
   JJA_HIP(hipMemcpyAsync(plan->d_runs, runs, sizeof(jja_problem_run) * np, hipMemcpyHostToDevice, stream));
   if (plan->parts > 1) {
-    // every problem of a split plan has the global face store (validated)
-    const std::vector<WorkItem> split = split_work(plan, work_g);
-    if (int rc = reset_counters(plan, work_g.size(), stream)) return rc;
+    // every problem of a split plan has the global face store (validated; a
+    // circuit without faces has none and needs none): all columns, one launch
+    work.insert(work.end(), work_g.begin(), work_g.end());
+    const std::vector<WorkItem> split = split_work(plan, work);
+    if (int rc = reset_counters(plan, work.size(), stream)) return rc;
     if (int rc = upload_work(plan, split, stream)) return rc;
     return launch_evolve_split(plan, static_cast<int32_t>(split.size()), n_steps, step0, stream);
   }
@@ -1646,8 +1652,9 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   JJA_HIP(hipMemcpyAsync(plan->d_adj, runs, sizeof(jja_adjoint_run) * np, hipMemcpyHostToDevice, stream));
   if (plan->parts > 1) {
-    const std::vector<WorkItem> split = split_work(plan, work_g);
-    if (int rc = reset_counters(plan, work_g.size(), stream)) return rc;
+    work.insert(work.end(), work_g.begin(), work_g.end());
+    const std::vector<WorkItem> split = split_work(plan, work);
+    if (int rc = reset_counters(plan, work.size(), stream)) return rc;
     if (int rc = upload_work(plan, split, stream)) return rc;
     return launch_adjoint_split(plan, static_cast<int32_t>(split.size()), k_hi, k_lo, stream);
   }

@@ -60,6 +60,8 @@ class DeviceSource:
             sep = hasattr(value, "amplitude") and hasattr(value, "base")
             base = value.base[:, :, None] if sep else value
             dev, (s_i, s_w, s_t) = source_layout(base, n, W, Nt)
+            if dev.size == 0:                  # no elements (no faces): a valid base, never read
+                dev = np.zeros(1)
             self.tensor = torch.from_numpy(dev).cuda()
             self.src.base = self.tensor.data_ptr()
             self.src.s_i, self.src.s_w, self.src.s_t = s_i, s_w, s_t
@@ -80,7 +82,8 @@ class DeviceSource:
         buf = np.empty((n_steps, self.W, self.n))
         for k in range(n_steps):
             buf[k] = np.broadcast_to(self.value(t0 + k), (self.n, self.W)).T
-        self.tensor = torch.from_numpy(buf.reshape(-1)).cuda()
+        flat = buf.reshape(-1) if buf.size else np.zeros(1)   # no elements: a valid base
+        self.tensor = torch.from_numpy(flat).cuda()
         self.src.base = self.tensor.data_ptr()
         self.src.s_i, self.src.s_w, self.src.s_t = 1, self.n, self.n * self.W
         self.src.t0 = t0

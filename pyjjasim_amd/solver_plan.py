@@ -379,6 +379,9 @@ class CircuitPlan:
 
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.column_threads, self.solve_plan, self.Nf, self.Nj
+        if Nf == 0:
+            self._build_faceless()
+            return
         # the face vector and stage scratch of a column: in the workgroup's LDS
         # when they fit (16-bit indices), else in a global per-column array
         store = 8 * (Nf + sp.max_stage_rows)
@@ -485,6 +488,25 @@ class CircuitPlan:
         self.slice_rows = np.concatenate((self.slice_rows, face_rows)).astype(np.int32)
         self._face_rec = self._slice_array(self.face_slices)
         self._solve_rec = self._slice_array(self.solve_slices)
+        self._wave_ptr = np.ascontiguousarray(self.wave_ptr, dtype=np.int32)
+
+    def _build_faceless(self):
+        """A circuit without faces (a tree): K is 0 x 0 and the step is
+        theta = -x / c0; no face records, no solve program."""
+        self.global_store = self.parts > 1
+        self.index_bytes = 4 if self.global_store else 2
+        self.junc_pos_code = -np.ones(2 * self.Nj, dtype=np.int32)
+        self.face_of_pos = np.zeros(0, np.int32)
+        self.face_code_sliced = np.zeros(0, np.int32)
+        self.face_slices, self.solve_slices = [], []
+        self.slice_rows = np.zeros(0, np.int32)
+        self.ent_val = np.zeros(0)
+        self.ent_off = np.zeros(0, np.uint32 if self.global_store else np.uint16)
+        self.face_copy_rows = 0
+        self.n_slices = self.n_solve_exec = 0
+        self.wave_ptr = np.zeros(self.column_threads // WAVE + 1, dtype=np.int64)
+        self._face_rec = self._slice_array([])
+        self._solve_rec = self._slice_array([])
         self._wave_ptr = np.ascontiguousarray(self.wave_ptr, dtype=np.int32)
 
     def _build_wave_program(self, phases, target_iter):

@@ -149,3 +149,20 @@ def test_gpu_large_array_global_store_matches_oracle():
     assert np.abs(th - th_o[:, :, 2:]).max() <= 1e-9 * scale
     n_o = te_oracle.vortex(c.get_cycle_matrix(), th_o[:, :, 2:])
     np.testing.assert_array_equal(r.get_vortex_configuration(), n_o)
+
+
+@pytest.mark.parametrize("parts", [1, 2])
+def test_gpu_faceless_circuit_matches_oracle(parts):
+    """A circuit without faces (a tree: K is 0 x 0, theta = -x / c0) against the
+    oracle, with one and with two workgroups per column."""
+    print(f"[gpu parity] faceless circuit, parts={parts}", flush=True)
+    g = pa.EmbeddedGraph(np.array([0., 1., 2., 1.]), np.array([0., 0., 0., 1.]),
+                         np.array([0, 1, 1]), np.array([1, 2, 3]))
+    c = pa.Circuit(g)
+    assert c.face_count() == 0
+    Is = np.array([0.5, 1.2, 0.3])[:, None, None] * np.array([1.0, 1.5])[None, :, None]
+    kw = dict(time_step=0.05, time_step_count=200, current_sources=Is, store_time_steps=[10, 199])
+    res = pa.TimeEvolutionProblem(c, block_size=256, parts=parts, **kw).compute()
+    ref = to.evolve(c, **oracle_kwargs(kw))
+    for key in ("theta", "current", "voltage"):
+        assert_close_per_step(getattr(res, "get_" + key)(), ref[key], what=key)
