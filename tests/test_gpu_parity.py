@@ -166,3 +166,33 @@ def test_gpu_faceless_circuit_matches_oracle(parts):
     ref = to.evolve(c, **oracle_kwargs(kw))
     for key in ("theta", "current", "voltage"):
         assert_close_per_step(getattr(res, "get_" + key)(), ref[key], what=key)
+
+
+@pytest.mark.parametrize("Nt", [1, 2, 3])
+def test_gpu_shortest_horizons_match_oracle(Nt):
+    """One, two and three steps, with every step stored (the prefix of initial
+    conditions and the first steps of the stencil)."""
+    print(f"[gpu parity] Nt={Nt}", flush=True)
+    c = pa.SquareArray(5, 4)
+    c.set_capacitance(0.7)
+    Is = c.current_base(0.0)[:, None, None] * np.array([0.6, 1.4])[None, :, None]
+    kw = dict(time_step=0.05, time_step_count=Nt, external_flux=0.2, current_sources=Is)
+    res = pa.TimeEvolutionProblem(c, **kw).compute()
+    ref = to.evolve(c, **oracle_kwargs(kw))
+    for key in ("theta", "current", "voltage"):
+        assert_close_per_step(getattr(res, "get_" + key)(), ref[key], what=key)
+    np.testing.assert_array_equal(res.get_vortex_configuration(),
+                                  to.vortex(c.get_cycle_matrix(), ref["theta"]))
+
+
+def test_gpu_single_junction_matches_oracle():
+    """The smallest circuit: two nodes and one junction (RSJ)."""
+    print("[gpu parity] single junction", flush=True)
+    g1 = pa.EmbeddedGraph(np.array([0., 1.]), np.array([0., 0.]), np.array([0]), np.array([1]))
+    c = pa.Circuit(g1)
+    Is = np.array([[[1.3]]])
+    kw = dict(time_step=0.05, time_step_count=400, current_sources=Is, store_time_steps=[100, 399])
+    res = pa.TimeEvolutionProblem(c, **kw).compute()
+    ref = to.evolve(c, **oracle_kwargs(kw))
+    for key in ("theta", "current", "voltage"):
+        assert_close_per_step(getattr(res, "get_" + key)(), ref[key], what=key)
