@@ -414,8 +414,8 @@ def main():
                     help="only run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(columns and problems are independent; see the sharding notes in DESIGN.md)",
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(columns and problems are independent and would shard over GPUs with no exchange)",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -18,8 +18,8 @@ reverse. The step's linear part M = D A^T K^-1 A D - D is symmetric, so the
 backward solve uses the forward's factor and solve program unchanged.
 
 Reference scheme: time_evolution.py:523-580 (the step), :547-558 (x), :560-570
-(the solve); the adjoint is this build's addition (the reference has no
-gradients).
+(the solve); the reference itself has no gradients, so the adjoint has no
+counterpart there.
 """
 from __future__ import annotations
 
