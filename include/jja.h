@@ -169,6 +169,14 @@ int jja_plan_destroy(jja_plan* plan);
  * library's own way to spread one large circuit over many compute units.
  */
 int jja_plan_status(jja_plan* plan);
+/*
+ * Polls a column barrier of the plan's split launches makes before it gives up
+ * (default 2^21, each poll sleeping briefly: far longer than any phase of a
+ * step). A small value forces the give-up path, which jja_plan_status then
+ * reports; every workgroup still runs to the end of its launch. No
+ * counterpart in the reference.
+ */
+int jja_plan_set_wait_limit(jja_plan* plan, uint32_t polls);
 /* Dynamic LDS bytes one column of problem p needs (0 if it cannot run). */
 int64_t jja_plan_lds_bytes(const jja_plan* plan, int32_t p);
 /* Doubles per column of the global face store problem p needs (0: it uses LDS). */

@@ -236,9 +236,9 @@ struct FaceStore<true> {
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
-// Waits of one column barrier before a workgroup gives up (each wait sleeps
-// and polls; far longer than any phase of a step takes).
-constexpr uint32_t kSpinLimit = 1u << 21;
+// Default polls of one column barrier before a workgroup gives up (each poll
+// sleeps briefly; far longer than any phase of a step takes).
+constexpr uint32_t kWaitLimit = 1u << 21;
 // s_waitcnt immediate (gfx9 encoding) that waits for vmcnt 0 only: expcnt and
 // lgkmcnt fields at their maxima
 constexpr int kWaitVmem = 0x0F70;
@@ -249,6 +249,7 @@ struct SplitStore {
   int32_t vt, vbs;
   uint32_t* ctr;         // the column's arrival counter, zero at launch
   uint32_t parts;
+  uint32_t wait_limit;   // polls before a wait gives up
   int32_t* err;          // launch status: set when a wait gave up
   int32_t* stopped;      // this workgroup's flag (LDS): it no longer waits
   mutable uint32_t gen;  // arrivals to wait for (thread 0)
@@ -276,7 +277,7 @@ struct SplitStore {
       __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t spins = 0;
       while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
-        if (++spins > kSpinLimit) {
+        if (++spins > wait_limit) {
           *stopped = 1;
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
@@ -804,13 +805,15 @@ __device__ __forceinline__ void evolve_column(const DevProblem& P, const jja_pro
 // kernel has a workgroup barrier.
 __device__ __forceinline__ SplitStore split_store(const DevProblem& P, GAS double* zbuf, int64_t w,
                                                   int32_t part, uint32_t* ctrs, const WorkItem& wi,
-                                                  int32_t* err, int32_t* stopped, int BS) {
+                                                  int32_t* err, uint32_t wait_limit, int32_t* stopped,
+                                                  int BS) {
   SplitStore zs;
   zs.p = zbuf + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
   zs.vt = part * BS + static_cast<int32_t>(threadIdx.x);
   zs.vbs = P.parts * BS;
   zs.ctr = ctrs + wi.slot;
   zs.parts = static_cast<uint32_t>(P.parts);
+  zs.wait_limit = wait_limit;
   zs.err = err;
   zs.stopped = stopped;
   zs.gen = 0;
@@ -822,7 +825,7 @@ __global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __re
                                                           const jja_problem_run* __restrict__ runs,
                                                           const WorkItem* __restrict__ work,
                                                           int32_t n_steps, int64_t step0, uint32_t* ctrs,
-                                                          int32_t* err) {
+                                                          int32_t* err, uint32_t wait_limit) {
   __shared__ int32_t stopped;
   if (threadIdx.x == 0) stopped = 0;
   __syncthreads();
@@ -831,7 +834,7 @@ __global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __re
   const jja_problem_run& R = runs[wi.problem];
   const int64_t w = wi.column / P.parts;
   const int32_t part = wi.column % P.parts;
-  const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, &stopped, BS);
+  const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
   evolve_steps<BS, SW>(P, R, w, zs, n_steps, step0);
 }
 
@@ -984,7 +987,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __restrict__ probs,
                                                            const jja_adjoint_run* __restrict__ runs,
                                                            const WorkItem* __restrict__ work, int64_t k_hi,
-                                                           int64_t k_lo, uint32_t* ctrs, int32_t* err) {
+                                                           int64_t k_lo, uint32_t* ctrs, int32_t* err,
+                                                           uint32_t wait_limit) {
   __shared__ int32_t stopped;
   if (threadIdx.x == 0) stopped = 0;
   __syncthreads();
@@ -993,7 +997,7 @@ __global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __r
   const jja_adjoint_run& A = runs[wi.problem];
   const int64_t w = wi.column / P.parts;
   const int32_t part = wi.column % P.parts;
-  const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, &stopped, BS);
+  const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
   adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
 }
 
@@ -1049,6 +1053,7 @@ struct Plan {
   size_t ctr_cap = 0;
   int32_t* d_err = nullptr;          // split launches: set when a column barrier gave up
   int32_t n_cu = 0;                  // compute units of the device
+  uint32_t wait_limit = kWaitLimit;  // split launches: polls before a column barrier gives up
 };
 
 template <typename T>
@@ -1367,7 +1372,7 @@ int launch_evolve_split_bs(Plan* plan, int32_t n, int32_t n_steps, int64_t step0
   const jja_problem_run* runs = plan->d_runs;
   const WorkItem* work = plan->d_work;
   return launch_cooperative(plan, reinterpret_cast<const void*>(evolve_split_kernel<BS, 3>), BS, n, stream,
-                            probs, runs, work, n_steps, step0, plan->d_ctrs, plan->d_err);
+                            probs, runs, work, n_steps, step0, plan->d_ctrs, plan->d_err, plan->wait_limit);
 }
 
 int launch_evolve_split(Plan* plan, int32_t n, int32_t n_steps, int64_t step0, hipStream_t stream) {
@@ -1386,7 +1391,7 @@ int launch_adjoint_split_bs(Plan* plan, int32_t n, int64_t k_hi, int64_t k_lo, h
   const jja_adjoint_run* runs = plan->d_adj;
   const WorkItem* work = plan->d_work;
   return launch_cooperative(plan, reinterpret_cast<const void*>(adjoint_split_kernel<BS>), BS, n, stream,
-                            probs, runs, work, k_hi, k_lo, plan->d_ctrs, plan->d_err);
+                            probs, runs, work, k_hi, k_lo, plan->d_ctrs, plan->d_err, plan->wait_limit);
 }
 
 int launch_adjoint_split(Plan* plan, int32_t n, int64_t k_hi, int64_t k_lo, hipStream_t stream) {
@@ -1551,6 +1556,13 @@ int jja_plan_status(jja_plan* p) {
   if (!e) return JJA_OK;
   JJA_HIP(hipMemset(plan->d_err, 0, sizeof(int32_t)));
   return fail(JJA_EHIP, "a column barrier of a split launch gave up waiting; its results are invalid");
+}
+
+int jja_plan_set_wait_limit(jja_plan* p, uint32_t polls) {
+  Plan* plan = reinterpret_cast<Plan*>(p);
+  if (!plan) return fail(JJA_EINVAL, "jja_plan_set_wait_limit: invalid arguments");
+  plan->wait_limit = polls;
+  return JJA_OK;
 }
 
 int jja_plan_destroy(jja_plan* p) {
