@@ -1299,31 +1299,45 @@ void order_work(const Plan* plan, std::vector<WorkItem>& work) {
 
 // Work items of a split launch, from one item per column: every part of every
 // column (its problem's part count), each column with its own barrier counter
-// (slot), largest problems first. When all columns have the same part count and
-// the column count is a multiple of 8, the parts of a column sit at block
-// indices equal modulo 8: workgroups go to the eight XCDs in turn by block
-// index, so the parts of a column then share one L2.
+// (slot). All workgroups of a cooperative launch are resident at once, so the
+// order only decides placement: workgroups go to the eight XCDs in turn by
+// block index (block b on XCD b mod 8), and the columns are packed onto the
+// XCDs (largest part count first, each to the XCD with the most free blocks)
+// so that the parts of a column share one L2. If a column does not fit, the
+// items stay in column order.
 std::vector<WorkItem> split_work(const Plan* plan, std::vector<WorkItem> cols) {
   order_work(plan, cols);
   constexpr size_t kXcd = 8;
   const size_t nc = cols.size();
-  bool uniform = true;
   size_t total = 0;
-  for (const WorkItem& c : cols) {
-    const size_t G = static_cast<size_t>(plan->problems[c.problem].parts);
-    uniform = uniform && G == static_cast<size_t>(plan->problems[cols[0].problem].parts);
-    total += G;
+  for (const WorkItem& c : cols) total += static_cast<size_t>(plan->problems[c.problem].parts);
+  std::vector<WorkItem> out(total);
+  // free block indices of each XCD, in ascending order
+  std::vector<std::vector<size_t>> slots(kXcd);
+  for (size_t b = 0; b < total; ++b) slots[b % kXcd].push_back(b);
+  std::vector<size_t> next(kXcd, 0);
+  std::vector<size_t> order(nc);
+  for (size_t c = 0; c < nc; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return plan->problems[cols[a].problem].parts > plan->problems[cols[b].problem].parts;
+  });
+  bool packed = true;
+  for (size_t c : order) {
+    const size_t G = static_cast<size_t>(plan->problems[cols[c].problem].parts);
+    size_t best = kXcd;
+    for (size_t x = 0; x < kXcd; ++x)
+      if (slots[x].size() - next[x] >= G && (best == kXcd || slots[x].size() - next[x] > slots[best].size() - next[best]))
+        best = x;
+    if (best == kXcd) {
+      packed = false;
+      break;
+    }
+    for (size_t g = 0; g < G; ++g)
+      out[slots[best][next[best]++]] = {cols[c].problem, static_cast<int32_t>(cols[c].column * G + g),
+                                        static_cast<int32_t>(c)};
   }
-  std::vector<WorkItem> out;
-  out.reserve(total);
-  if (uniform && nc % kXcd == 0) {
-    const size_t G = static_cast<size_t>(plan->problems[cols[0].problem].parts);
-    out.resize(total);
-    for (size_t c = 0; c < nc; ++c)
-      for (size_t g = 0; g < G; ++g)
-        out[((c / kXcd) * G + g) * kXcd + c % kXcd] = {
-            cols[c].problem, static_cast<int32_t>(cols[c].column * G + g), static_cast<int32_t>(c)};
-  } else {
+  if (!packed) {
+    out.clear();
     for (size_t c = 0; c < nc; ++c) {
       const int32_t G = plan->problems[cols[c].problem].parts;
       for (int32_t g = 0; g < G; ++g)
