@@ -57,8 +57,9 @@ def balanced_parts(hosts, columns, n_cu, max_parts=16):
 
 
 def _plan_one(args):
-    circuit, dt, sw, leaf, bs, store = args
-    return CircuitPlan(circuit, dt, sw, leaf_size=leaf, block_size=bs, face_store=store)
+    circuit, dt, sw, leaf, bs, store, local = args
+    return CircuitPlan(circuit, dt, sw, leaf_size=leaf, block_size=bs, face_store=store,
+                       local_stages=local)
 
 
 def build_plans(problems, block_size, workers=None):
@@ -68,7 +69,7 @@ def build_plans(problems, block_size, workers=None):
     only run host code; they are started fresh (spawn), never forked from a
     process that may hold a device context."""
     jobs = [(p.circuit, p._dt(), p.stencil_width, p.leaf_size, block_size,
-             getattr(p, "face_store", "auto")) for p in problems]
+             getattr(p, "face_store", "auto"), getattr(p, "local_stages", "auto")) for p in problems]
     size = sum(p.circuit._Nj() for p in problems)
     if workers is None:
         workers = min(16, len(os.sched_getaffinity(0)), len(jobs)) if size > 200_000 else 1

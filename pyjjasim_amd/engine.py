@@ -154,7 +154,8 @@ class Engine:
             self.block_size = problem.block_size or default_block_size(Nj, Nf)
             self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
                                     block_size=self.block_size,
-                                    face_store=getattr(problem, "face_store", "auto"))
+                                    face_store=getattr(problem, "face_store", "auto"),
+                                    local_stages=getattr(problem, "local_stages", "auto"))
             parts = getattr(problem, "parts", None)
             if parts is None:
                 parts = auto_parts(self.host, W, sw)
@@ -360,7 +361,8 @@ class BatchEngine:
                                                 for p in problems)
             self.hosts = [CircuitPlan(p.circuit, p._dt(), sw, leaf_size=p.leaf_size,
                                       block_size=self.block_size,
-                                      face_store=getattr(p, "face_store", "auto")) for p in problems]
+                                      face_store=getattr(p, "face_store", "auto"),
+                                      local_stages=getattr(p, "local_stages", "auto")) for p in problems]
         descs = (_native.ProblemDesc * len(problems))(*[h.desc() for h in self.hosts])
         self._descs = descs
         self.handle = C.c_void_p()

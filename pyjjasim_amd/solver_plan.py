@@ -21,6 +21,7 @@ import heapq
 import numpy as np
 import scipy.linalg
 import scipy.sparse
+import scipy.sparse.csgraph
 
 from . import _native
 from .ordering import nested_dissection
@@ -209,6 +210,31 @@ class SolvePlan:
         assert x.shape[0] == n
         return x[:, 0] if squeeze else x
 
+    def row_work(self):
+        """Factor entries each row's records stream per solve (both passes)."""
+        a = np.arange(self.n) - self.row_block.astype(np.int64)
+        return np.diff(self.lo_ptr) + np.diff(self.up_ptr) + (a + 1) + (self.row_bsize - a)
+
+    def subtrees(self, n_low):
+        """Group label of every row of the lowest ``n_low`` stages: blocks of those
+        stages joined by a factor entry between their rows share a label. Below
+        stage n_low the solve reads and writes only rows of the row's own group
+        (its lower entries point into the group; its upper entries point into the
+        group or into higher stages), so groups can be solved independently."""
+        R = int(self.stage_ptr[n_low])
+        sizes = self.blocks[:, 1] - self.blocks[:, 0]
+        nb = int(np.count_nonzero(self.blocks[:, 2] < n_low))
+        bid = np.repeat(np.arange(len(self.blocks)), sizes)
+        if not hasattr(self, "_block_pairs"):
+            rr = np.repeat(np.arange(self.n), np.diff(self.lo_ptr))
+            key = np.unique(bid[rr] * np.int64(len(self.blocks)) + bid[self.lo_col])
+            self._block_pairs = np.stack([key // len(self.blocks), key % len(self.blocks)], axis=1)
+        p = self._block_pairs
+        p = p[(p[:, 0] < nb) & (p[:, 1] < nb)]
+        g = scipy.sparse.coo_matrix((np.ones(len(p)), (p[:, 0], p[:, 1])), shape=(nb, nb))
+        _, label = scipy.sparse.csgraph.connected_components(g, directed=False)
+        return label[bid[:R]]
+
     def factor_bytes(self):
         """Bytes of factor data one column streams per step (both passes)."""
         return int(12 * (len(self.lo_val) + len(self.up_val)) + 8 * (len(self.inv_cm) + len(self.inv_rm))
@@ -318,18 +344,28 @@ class CircuitPlan:
     """All host arrays of one circuit for ``jja_plan_create``, laid out for
     workgroups of ``block_size`` threads; with ``parts`` > 1 a column is worked
     by that many workgroups (the layouts are built for parts * block_size
-    threads, and the face store is global)."""
+    threads, and the face store is global).
+
+    ``local_stages``: the lowest stages are solved wave-locally: their blocks
+    fall into independent subtrees, each wave solves whole subtrees, and waves
+    meet once per pass for those stages instead of twice per stage. "auto"
+    chooses the count from a cost estimate for split columns and 0 (every stage
+    in step) for one workgroup per column. Results do not depend on it: a
+    row's summation order depends only on the row."""
 
     def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8,
-                 face_store="auto", parts=1):
+                 face_store="auto", parts=1, local_stages="auto"):
         if face_store not in ("auto", "lds", "global"):
             raise ValueError("face_store must be 'auto', 'lds' or 'global'")
         if parts < 1 or parts & (parts - 1) or parts > 32:
             raise ValueError("parts must be a power of two from 1 to 32")
         if parts > 1 and face_store == "lds":
             raise ValueError("a column split over workgroups needs the global face store")
+        if local_stages != "auto" and (int(local_stages) != local_stages or local_stages < 0):
+            raise ValueError("local_stages must be 'auto' or a count >= 0")
         self.face_store = face_store
         self.parts = parts
+        self.local_option = local_stages
         if stencil_width not in STENCILS:
             raise ValueError(f"stencil width must be 3, 4 or 5 (equals {stencil_width})")
         self.Nj, self.Nf = circuit._Nj(), circuit._Nf()
@@ -355,12 +391,16 @@ class CircuitPlan:
         self.target_iter = target_iter
         self._build_layouts(target_iter)
 
-    def relayout(self, block_size, parts=None, face_store=None):
+    def relayout(self, block_size, parts=None, face_store=None, local_stages=None):
         """The same circuit, factor and schedule laid out for another workgroup
-        size, part count or face store (the factorisation depends on none of
-        them)."""
+        size, part count, face store or wave-local stage count (the
+        factorisation depends on none of them)."""
         other = copy.copy(self)
         other.block_size = block_size
+        if local_stages is not None:
+            if local_stages != "auto" and (int(local_stages) != local_stages or local_stages < 0):
+                raise ValueError("local_stages must be 'auto' or a count >= 0")
+            other.local_option = local_stages
         if face_store is not None:
             if face_store not in ("auto", "lds", "global"):
                 raise ValueError("face_store must be 'auto', 'lds' or 'global'")
@@ -377,20 +417,82 @@ class CircuitPlan:
         """Threads that work on one column: the layouts are built for this many."""
         return self.block_size * self.parts
 
+    # What one column barrier between workgroups costs, in factor entries one
+    # wave streams meanwhile (about 14 us on MI355X for SquareArray(512,512) at
+    # 16 parts, where a wave streams about 0.7 entries/ns).
+    COLUMN_BARRIER_ENTRIES = 6000
+
+    def _subtree_waves(self, NW, n_low):
+        """Subtrees of the lowest n_low stages spread over NW waves, heaviest
+        first, NW at a time: each round gives its heaviest subtree to the
+        least-loaded wave, the next to the next least-loaded, and so on.
+        Returns (factor entries per wave, owner wave of every row of those
+        stages)."""
+        sp = self.solve_plan
+        label = sp.subtrees(n_low)
+        if not hasattr(sp, "_row_work"):
+            sp._row_work = sp.row_work()
+        gw = np.bincount(label, weights=sp._row_work[:len(label)])
+        order = np.lexsort((np.arange(len(gw)), -gw))
+        loads = np.zeros(NW)
+        wave_of = np.zeros(len(gw), dtype=np.int64)
+        for i in range(0, len(order), NW):
+            chunk = order[i:i + NW]
+            waves = np.argsort(loads, kind="stable")[:len(chunk)]
+            wave_of[chunk] = waves
+            loads[waves] += gw[chunk]
+        return loads, wave_of[label]
+
+    def _choose_local(self, NW, max_rows):
+        """Wave-local stage count with the lowest estimated cost: the slowest
+        wave's excess over an even share of the wave-local work, minus the
+        column barriers saved (4 per stage, less the leaves' skipped phase,
+        plus the two that close the runs). max_rows bounds the scratch rows.
+
+        With one workgroup per column the count is 0: a workgroup barrier costs
+        less than the partly filled records that per-wave chains add (on
+        SquareArray(100,100) every count from 2 to 7 was slower, by 4-14%)."""
+        if self.parts == 1:
+            return 0
+        sp = self.solve_plan
+        B = self.COLUMN_BARRIER_ENTRIES
+        best, best_cost = 0, 0.0
+        for k in range(1, sp.n_stages + 1):
+            if max_rows is not None and max(sp.max_stage_rows, int(sp.stage_ptr[k])) > max_rows:
+                break
+            load, _ = self._subtree_waves(NW, k)
+            cost = (load.max() - load.sum() / NW) - B * (4 * k - 3)
+            if cost < best_cost:
+                best, best_cost = k, cost
+        return best
+
+    def _wave_owner(self, NW, n_low):
+        """Owner wave of every row of the lowest n_low stages (None for none)."""
+        return self._subtree_waves(NW, n_low)[1] if n_low else None
+
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.column_threads, self.solve_plan, self.Nf, self.Nj
         if Nf == 0:
             self._build_faceless()
             return
         # the face vector and stage scratch of a column: in the workgroup's LDS
-        # when they fit (16-bit indices), else in a global per-column array
-        store = 8 * (Nf + sp.max_stage_rows)
-        fits = store <= LDS_BYTES and Nf + sp.max_stage_rows <= 65536
+        # when they fit (16-bit indices), else in a global per-column array.
+        # Wave-local stages keep every scratch row of those stages at once.
+        NW = BS // WAVE
+        fixed = None if self.local_option == "auto" else min(int(self.local_option), sp.n_stages)
+        scratch = sp.max_stage_rows if fixed is None else max(sp.max_stage_rows, int(sp.stage_ptr[fixed]))
+        lds_rows = min(LDS_BYTES // 8, 65536) - Nf
+        fits = scratch <= lds_rows
         if self.face_store == "lds" and not fits:
-            raise ValueError(f"circuit with {Nf} faces needs {store} bytes of face store per column; "
-                             f"more than one workgroup's LDS ({LDS_BYTES})")
+            raise ValueError(f"circuit with {Nf} faces needs {8 * (Nf + scratch)} bytes of face store per "
+                             f"column; more than one workgroup's LDS ({LDS_BYTES})")
         self.global_store = self.face_store == "global" or not fits or self.parts > 1
         self.index_bytes = 4 if self.global_store else 2
+        if fixed is None:
+            fixed = self._choose_local(NW, None if self.global_store else lds_rows)
+        self.local_stages = fixed
+        self.scratch_rows = max(sp.max_stage_rows, int(sp.stage_ptr[fixed]))
+        owner = self._wave_owner(NW, fixed)
         pos = sp.face_pos.astype(np.int64)
         # junctions: face positions of the (up to two) faces, ascending face index
         jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
@@ -437,8 +539,12 @@ class CircuitPlan:
         # solve program: forward stages ascending, backward stages descending
         rb, bsz, boff = sp.row_block.astype(np.int64), sp.row_bsize.astype(np.int64), sp.row_boff
         per_stage = []
+        n_loc = self.local_stages
         for s in range(sp.n_stages):
             r0, r1 = int(sp.stage_ptr[s]), int(sp.stage_ptr[s + 1])
+            # scratch row of row r: t[r - t0]; wave-local stages keep all their
+            # rows' scratch at once (t0 = 0), the others reuse it per stage
+            t0 = 0 if s < n_loc else r0
             rows = np.arange(r0, r1)
             a = rows - rb[rows]
             B = bsz[rows]
@@ -449,7 +555,7 @@ class CircuitPlan:
             rr = np.repeat(np.arange(r1 - r0), cnt)
             c = np.arange(ptr[-1]) - np.repeat(ptr[:-1], cnt)
             ar = a[rr]
-            parts.append((ptr, (rb[rows][rr] - r0) + c, sp.inv_rm[boff[rows][rr] + ar * (ar + 1) // 2 + c]))
+            parts.append((ptr, (rb[rows][rr] - t0) + c, sp.inv_rm[boff[rows][rr] + ar * (ar + 1) // 2 + c]))
             parts.append((sp.up_ptr[r0:r1 + 1] - sp.up_ptr[r0], sp.up_col[sp.up_ptr[r0]:sp.up_ptr[r1]],
                           sp.up_val[sp.up_ptr[r0]:sp.up_ptr[r1]]))
             cnt = B - a
@@ -457,9 +563,9 @@ class CircuitPlan:
             rr = np.repeat(np.arange(r1 - r0), cnt)
             c = np.arange(ptr[-1]) - np.repeat(ptr[:-1], cnt)
             ar, Br = a[rr], B[rr]
-            parts.append((ptr, (rb[rows][rr] - r0) + ar + c,
+            parts.append((ptr, (rb[rows][rr] - t0) + ar + c,
                           sp.inv_cm[boff[rows][rr] + ar * Br - ar * (ar - 1) // 2 + c]))
-            per_stage.append((r0, rows, parts))
+            per_stage.append((t0, rows, parts))
         prog = []
         for s in range(sp.n_stages):
             prog += [(s, 0), (s, 1)]
@@ -468,17 +574,19 @@ class CircuitPlan:
         phases = []
         self.face_copy_rows = 0
         for s, kind in prog:
-            r0, rows, parts = per_stage[s]
+            t0, rows, parts = per_stage[s]
             pptr, pidx, pval = parts[kind]
-            if (s, kind) == (0, 0) and r0 == 0 and pptr[-1] == pptr[0]:
+            if (s, kind) == (0, 0) and pptr[-1] == pptr[0]:
                 # leaves have no off-block entries: their scratch is b itself,
                 # written by the face phase
                 self.face_copy_rows = len(rows)
                 continue
-            # LDS indices in doubles: z at its position, stage scratch t at Nf + row
+            # LDS indices in doubles: z at its position, stage scratch t at Nf + row - t0
             lds_base = Nf if kind % 2 else 0
-            phases.append((r0, rows, pptr, lds_base + np.asarray(pidx, dtype=np.int64), pval,
-                           _native.SLICE_BLOCK if kind % 2 else 0))
+            local = s < n_loc
+            end = local and (s, kind) in ((n_loc - 1, 1), (0, 3))
+            phases.append((t0, rows, pptr, lds_base + np.asarray(pidx, dtype=np.int64), pval,
+                           _native.SLICE_BLOCK if kind % 2 else 0, owner[rows] if local else None, end))
         self._build_wave_program(phases, target_iter)
         self.face_slices = [(b, 0, rb, r0, n, fl) for (b, rb, r0, n, fl) in self.face_slices]
         # face rows follow the solve rows (record row bases were taken before)
@@ -503,6 +611,7 @@ class CircuitPlan:
         self.ent_val = np.zeros(0)
         self.ent_off = np.zeros(0, np.uint32 if self.global_store else np.uint16)
         self.face_copy_rows = 0
+        self.local_stages = self.scratch_rows = 0
         self.n_slices = self.n_solve_exec = 0
         self.wave_ptr = np.zeros(self.column_threads // WAVE + 1, dtype=np.int64)
         self._face_rec = self._slice_array([])
@@ -520,6 +629,11 @@ class CircuitPlan:
         record of its chain). Chains go to the least-loaded wave (longest first);
         the last record of every wave in a phase carries BARRIER (a wave without
         work gets a no-op record), so all waves meet at every phase boundary.
+        A phase with an owner wave per row (a wave-local stage) is cut into chains
+        of one owner's rows, each chain goes to its owner, and no barrier follows
+        it: the owner's later records see its writes because one wave's LDS and
+        vector-memory accesses are performed in order. The last phase of a run
+        of wave-local phases ends with a barrier in every wave.
         Record data: value `it` of lane l at base + (it/2)*128 + 2l + it%2, LDS
         index `it` (in doubles) of lane l at ibase + 8l + it, row of lane l at
         rowbase + l/G. Block 0 of every array is zero (the no-op record); record
@@ -532,31 +646,41 @@ class CircuitPlan:
         n_rec, nr = 1, WAVE
         noop = (0, 0, 0, 0, 0, 1)
         progs = [[] for _ in range(NW)]
-        for (r0, rows, ptr, idx, val, kflag) in phases:
+        region = [None] * NW    # where each wave's current wave-local run began
+        for (r0, rows, ptr, idx, val, kflag, owner, end) in phases:
             ptr = np.asarray(ptr, dtype=np.int64)
             rows = np.asarray(rows, dtype=np.int64)
             lens = np.diff(ptr)
             need = np.maximum(1, -(-lens // target_iter))
             Gsel = np.minimum(WAVE, 2 ** np.ceil(np.log2(need)).astype(np.int64))
-            chains = []   # (g, rows, records, first record, rowbase, iterations)
+            key_all = np.zeros(len(rows), dtype=np.int64) if owner is None else np.asarray(owner, np.int64)
+            chains = []   # (g, rows, records, first record, rowbase, iterations, owner)
             for g in sorted(set(Gsel.tolist())):
                 sel = np.nonzero(Gsel == g)[0]
-                sel = sel[np.argsort(-lens[sel], kind="stable")]
+                sel = sel[np.lexsort((-lens[sel], key_all[sel]))]   # by owner, longest first
+                key = key_all[sel]
                 R = WAVE // g
-                n_ch = -(-len(sel) // R)
-                pos = np.arange(len(sel))
-                n_it = -(-lens[sel[::R]] // g)             # the chain's first row is its longest
+                starts = np.concatenate(([0], np.nonzero(np.diff(key))[0] + 1))
+                gsize = np.diff(np.concatenate((starts, [len(sel)])))
+                gid = np.repeat(np.arange(len(starts)), gsize)
+                inpos = np.arange(len(sel)) - starts[gid]
+                gch = -(-gsize // R)
+                chain_of = (np.cumsum(gch) - gch)[gid] + inpos // R
+                slot = inpos % R
+                n_ch = int(gch.sum())
+                head = slot == 0                           # a chain's first row is its longest
+                n_it = -(-lens[sel[head]] // g)
                 nch = np.maximum(1, -(-n_it // MAX_ITER))
                 rec0 = np.cumsum(nch) - nch
                 v = np.zeros((int(nch.sum()), MAX_ITER, WAVE))
                 o = np.zeros(v.shape, dtype=np.int64)
                 L = lens[sel]
-                who = np.repeat(pos, L)
+                who = np.repeat(np.arange(len(sel)), L)
                 e = np.arange(int(L.sum())) - np.repeat(np.cumsum(L) - L, L)
                 src = np.repeat(ptr[sel], L) + e
                 k = e // g
-                rec = rec0[who // R] + k // MAX_ITER
-                ln = (who % R) * g + e % g
+                rec = rec0[chain_of[who]] + k // MAX_ITER
+                ln = slot[who] * g + e % g
                 v[rec, k % MAX_ITER, ln] = val[src]
                 o[rec, k % MAX_ITER, ln] = idx[src]
                 # record layouts: value it of lane l at (it/2)*128 + 2l + it%2,
@@ -564,37 +688,55 @@ class CircuitPlan:
                 vals.append(v.reshape(-1, MAX_ITER // 2, 2, WAVE).transpose(0, 1, 3, 2).reshape(-1))
                 offs.append(o.transpose(0, 2, 1).reshape(-1))
                 rws = -np.ones(n_ch * R, dtype=np.int64)
-                rws[:len(sel)] = rows[sel]
+                rws[chain_of * R + slot] = rows[sel]
                 rows_out.append(rws)
-                nrow = np.minimum(R, len(sel) - np.arange(n_ch) * R)
+                nrow = np.bincount(chain_of, minlength=n_ch)
+                cown = key[head]
                 for c in range(n_ch):
                     chains.append((g, int(nrow[c]), int(nch[c]), n_rec + int(rec0[c]), nr + c * R,
-                                   int(n_it[c])))
+                                   int(n_it[c]), int(cown[c])))
                 n_rec += int(nch.sum())
                 nr += n_ch * R
             order = sorted(range(len(chains)), key=lambda c: -chains[c][2])
-            # least-loaded wave first, the lowest index among equals
-            heap = [(0, w) for w in range(NW)]
             assign = [[] for _ in range(NW)]
-            for c in order:
-                load, w = heapq.heappop(heap)
-                assign[w].append(c)
-                heapq.heappush(heap, (load + chains[c][2], w))
+            if owner is None:
+                # least-loaded wave first, the lowest index among equals
+                heap = [(0, w) for w in range(NW)]
+                for c in order:
+                    load, w = heapq.heappop(heap)
+                    assign[w].append(c)
+                    heapq.heappush(heap, (load + chains[c][2], w))
+            else:
+                for c in order:
+                    assign[chains[c][6]].append(c)
             for w in range(NW):
                 recs = []
                 for c in assign[w]:
-                    g, nrow, nch, first, rowbase, n_it = chains[c]
+                    g, nrow, nch, first, rowbase, n_it, _ = chains[c]
                     for ch in range(nch):
                         flags = g | kflag | (nrow << 16)
                         if ch == nch - 1:
                             flags |= _native.SLICE_FINISH
                         b = 512 * (first + ch)
                         recs.append((b, b, rowbase, r0, min(MAX_ITER, n_it - ch * MAX_ITER), flags))
-                if not recs:
+                if owner is not None:
+                    # wave-local: the wave's records follow each other with no
+                    # barrier until the run of wave-local phases ends
+                    if region[w] is None:
+                        region[w] = len(progs[w])
+                    progs[w] += recs
+                    if not end:
+                        continue
+                    if len(progs[w]) == region[w]:
+                        progs[w].append(noop)
+                    recs, progs[w] = [progs[w][-1]], progs[w][:-1]
+                    region[w] = None
+                elif not recs:
                     recs.append(noop)
                 b_, ib_, rb_, r0_, it_, fl_ = recs[-1]
                 recs[-1] = (b_, ib_, rb_, r0_, it_, fl_ | _native.SLICE_BARRIER)
                 progs[w] += recs
+        assert all(r is None for r in region), "a wave-local run without its closing barrier"
         solve, wave_ptr, n_exec = [], [], 0
         for w in range(NW):
             recs = progs[w]
@@ -661,14 +803,14 @@ class CircuitPlan:
         d.ent_val = P(self.ent_val, C.c_double)
         d.n_index = len(self.ent_off)
         d.ent_off = self.ent_off.ctypes.data
-        d.max_stage_rows = sp.max_stage_rows
+        d.max_stage_rows = self.scratch_rows
         d.face_copy_rows = self.face_copy_rows
         d.index_bytes = self.index_bytes
         return d
 
     def lds_bytes(self):
         """Bytes of the per-column face store (LDS or global)."""
-        return 8 * (self.Nf + self.solve_plan.max_stage_rows)
+        return 8 * (self.Nf + self.scratch_rows)
 
     # ---------------------------------------------------- host emulation
     def factor_bytes(self):
@@ -681,11 +823,15 @@ class CircuitPlan:
         """Exactly the kernel's face phase and per-wave solve programs, on the host:
         u (Nj,) = x/c0 - theta_s  ->  J (Nf,) = K^-1 (A u)  (f = 0). Lanes
         accumulate with FMA-free numpy adds of products, so sums agree with the
-        kernel to rounding, not bit for bit."""
+        kernel to rounding, not bit for bit.
+
+        Between two barriers each wave runs its records in order and sees its own
+        writes at once; any face-store element that one wave writes and another
+        wave reads or writes between the same two barriers raises AssertionError
+        (a race on the GPU). Reads with a zero factor value are left out: they
+        add an exact zero whatever they read."""
         BS, Nf = self.column_threads, self.Nf
-        sp = self.solve_plan
-        zs = np.zeros(Nf)
-        ts = np.zeros(max(sp.max_stage_rows, 1))
+        zs = np.zeros(Nf + max(self.scratch_rows, 1))
         tids = np.arange(BS)
         acc = np.zeros(BS)
         for (base, _, rowbase, r0, n_it, flags) in self.face_slices:
@@ -698,7 +844,7 @@ class CircuitPlan:
                 ok = pos >= 0
                 zs[pos[ok]] = acc[ok]
                 cp = ok & (pos < self.face_copy_rows)
-                ts[pos[cp]] = acc[cp]
+                zs[Nf + pos[cp]] = acc[cp]
                 acc = np.zeros(BS)
         # split every wave's program at its barriers; phase p of all waves runs
         # before phase p+1 (waves meet at each barrier)
@@ -716,27 +862,43 @@ class CircuitPlan:
         n_ph = len(phases[0])
         assert all(len(p) == n_ph for p in phases), "waves disagree on the barrier count"
         lane = np.arange(WAVE)
+        n = len(zs)
         for ph in range(n_ph):
-            lds = np.concatenate((zs, ts))
-            writes = []
+            views, wrote, read = [], [], []
             for w in range(NW):
+                mem = zs.copy()
+                wmask = np.zeros(n, dtype=bool)
+                rmask = np.zeros(n, dtype=bool)
                 acc = np.zeros(WAVE)
                 for (vbase, ibase, rowbase, r0, n_it, flags) in phases[w][ph]:
                     G = flags & 0xFF
                     for it in range(MAX_ITER):
                         v = self.ent_val[vbase + (it // 2) * 128 + 2 * lane + it % 2]
                         off = self.ent_off[ibase + 8 * lane + it].astype(np.int64)
-                        acc = acc + v * lds[off]
+                        rmask[off[v != 0]] = True
+                        acc = acc + v * mem[off]
                     if flags & _native.SLICE_FINISH:
                         red = acc.reshape(WAVE // G, G).sum(axis=1)
                         rows = self.slice_rows[rowbase + np.arange(WAVE // G)]
                         live = np.arange(WAVE // G) < (flags >> 16)
                         ok = live & (rows >= 0)
-                        writes.append((bool(flags & _native.SLICE_BLOCK), rows[ok], r0, red[ok]))
+                        rows, red = rows[ok].astype(np.int64), red[ok]
+                        if flags & _native.SLICE_BLOCK:
+                            mem[rows] = red
+                            wmask[rows] = True
+                        else:
+                            rmask[rows] = True
+                            mem[Nf + rows - r0] = mem[rows] - red
+                            wmask[Nf + rows - r0] = True
                         acc = np.zeros(WAVE)
-            for block, rows, r0, red in writes:
-                if block:
-                    zs[rows] = red
-                else:
-                    ts[rows - r0] = zs[rows] - red
-        return zs[sp.face_pos]
+                views.append(mem)
+                wrote.append(wmask)
+                read.append(rmask)
+            writers = np.sum(wrote, axis=0)
+            assert writers.max(initial=0) <= 1, "two waves write one face-store element between barriers"
+            for w in range(NW):
+                assert not np.any((writers - wrote[w])[read[w]]), \
+                    f"wave {w} reads an element another wave writes between the same barriers"
+            for w in range(NW):
+                zs[wrote[w]] = views[w][wrote[w]]
+        return zs[self.solve_plan.face_pos]

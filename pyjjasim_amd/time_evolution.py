@@ -28,6 +28,12 @@ Extensions (keyword-only, defaults keep reference behaviour):
                    None (default) picks the largest power of two up to 16 that
                    keeps all columns' workgroups resident on the device; 1 keeps
                    one workgroup per column. Results do not depend on it.
+  local_stages     Lowest stages of the factor schedule solved wave-locally (each
+                   wave solves whole independent subtrees, with no barrier between
+                   those stages): "auto" (default) chooses by a cost estimate
+                   for columns split over workgroups and 0 otherwise; 0 solves
+                   every stage with all waves in step. Results do not depend
+                   on it.
 
 There is no CPU fallback: without a HIP device ``compute()`` raises.
 """
@@ -116,7 +122,7 @@ class TimeEvolutionProblem:
                  store_voltage=True, store_current=True, config_at_minus_1=None,
                  config_at_minus_2=None, config_at_minus_3=None, config_at_minus_4=None,
                  stencil_width=3, *, noise="philox", seed=None, steps_per_launch=None,
-                 block_size=None, leaf_size=16, face_store="auto", parts=None):
+                 block_size=None, leaf_size=16, face_store="auto", parts=None, local_stages="auto"):
         self.circuit = circuit
         self.time_step = time_step
         self.time_step_count = time_step_count
@@ -171,6 +177,7 @@ class TimeEvolutionProblem:
         self.leaf_size = leaf_size
         self.face_store = face_store
         self.parts = parts
+        self.local_stages = local_stages
 
     # ------------------------------------------------------------- getters
     def get_problem_count(self):

@@ -1,6 +1,7 @@
 """A column split over several workgroups (parts) that meet at column barriers
-in global memory gives exactly the results of one workgroup per column: the
-per-row summation order, the Philox stream and every expression are the same."""
+in global memory gives exactly the results of one workgroup per column, and so
+does a solve whose lowest stages run wave-locally: the per-row summation order,
+the Philox stream and every expression are the same."""
 import numpy as np
 import pytest
 
@@ -18,13 +19,14 @@ def _circuit():
     return c
 
 
-def _run(c, parts, T, Nt=150):
+def _run(c, parts, T, Nt=150, store="global", local="auto"):
     W = 3
     Is = c.current_base(0.0)[:, None, None] * np.array([0.4, 0.9, 1.3])[None, :, None]
     prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
                                    current_sources=Is, temperature=T, seed=5,
                                    store_time_steps=[Nt // 2, Nt - 1], block_size=256,
-                                   face_store="global", parts=parts, steps_per_launch=60)
+                                   face_store=store, parts=parts, steps_per_launch=60,
+                                   local_stages=local)
     return prob.compute()
 
 
@@ -41,6 +43,22 @@ def test_gpu_split_columns_bit_identical(T):
         np.testing.assert_array_equal(r.get_vortex_configuration(), ref.get_vortex_configuration())
 
 
+@pytest.mark.parametrize("store,parts,local", [("lds", 1, 3), ("lds", 1, 99), ("global", 1, 99),
+                                               ("global", 4, 0), ("global", 4, 2)])
+def test_gpu_wave_local_stages_bit_identical(store, parts, local):
+    """Lowest stages solved wave-locally (each wave whole subtrees, no barrier
+    between those stages), in LDS, in global memory and split over
+    workgroups, give the results of every stage solved in step."""
+    c = _circuit()
+    print("[local] reference: one part, every stage in step", flush=True)
+    ref = _run(c, 1, 0.02, local=0)
+    print(f"[local] {store} store, {parts} parts, local_stages={local}", flush=True)
+    r = _run(c, parts, 0.02, store=store, local=local)
+    np.testing.assert_array_equal(r.get_theta(), ref.get_theta())
+    np.testing.assert_array_equal(r.get_voltage(), ref.get_voltage())
+    np.testing.assert_array_equal(r.get_vortex_configuration(), ref.get_vortex_configuration())
+
+
 def test_gpu_split_adjoint_bit_identical():
     torch = pytest.importorskip("torch")
     c = _circuit()
@@ -54,14 +72,18 @@ def test_gpu_split_adjoint_bit_identical():
     store[[Nt // 2, Nt - 1]] = True
     G = torch.tensor(rng.standard_normal((Nj, W, 2)), dtype=torch.float64, device="cuda")
     grads = []
-    for parts in (1, 4):
-        print(f"[split] adjoint, {parts} parts", flush=True)
+    # one part in step, four parts (wave-local lowest stages by default), one
+    # part with every stage wave-local
+    for parts, local in ((1, 0), (4, "auto"), (1, 99)):
+        print(f"[split] adjoint, {parts} parts, local_stages={local}", flush=True)
         evo = pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W,
                                          temperature=0.01, store_time_steps=store, checkpoint_every=30,
-                                         block_size=256, face_store="global", parts=parts, seed=3)
+                                         block_size=256, face_store="global", parts=parts, seed=3,
+                                         local_stages=local)
         leaves = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
         th = evo(*leaves)
         (th * G).sum().backward()
         grads.append([th.detach().cpu().numpy()] + [t.grad.cpu().numpy() for t in leaves])
-    for a, b in zip(*grads):
-        np.testing.assert_array_equal(a, b)
+    for other in grads[1:]:
+        for a, b in zip(grads[0], other):
+            np.testing.assert_array_equal(a, b)

@@ -136,7 +136,7 @@ def test_native_validation_rejects_bad_programs():
         return CircuitPlan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128)
     plan = fresh()
     k = int(plan.wave_ptr[0])
-    plan.ent_off[plan.solve_slices[k][1] + 3] = plan.Nf + plan.solve_plan.max_stage_rows
+    plan.ent_off[plan.solve_slices[k][1] + 3] = plan.Nf + plan.scratch_rows
     assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
     assert b"offset" in lib.jja_last_error()
     plan = fresh()
@@ -274,3 +274,83 @@ def test_mixed_part_plans_validate():
     d = CircuitPlan(_circuits()["hc7"], 0.05, block_size=128, face_store="global")
     descs = (_native.ProblemDesc * 2)(a.desc(), d.desc())
     assert _native.lib().jja_plan_validate(2, descs) == _native.JJA_EINVAL
+
+
+@pytest.mark.parametrize("store", ["lds", "global"])
+@pytest.mark.parametrize("BS,parts", [(64, 1), (256, 1), (1024, 1), (256, 2)])
+@pytest.mark.parametrize("name", ["sq10", "hc7", "tri8", "sqL", "het", "squid"])
+def test_wave_local_stages_emulation(name, BS, parts, store):
+    """Lowest stages solved wave-locally (whole subtrees per wave, no barrier
+    between those stages): the plans validate, and the host emulation, which
+    runs each wave's records in order and fails on any element one wave writes
+    and another touches between the same barriers, solves K J = A u."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    if parts > 1 and store == "lds":
+        pytest.skip("split columns use the global face store")
+    c = _circuits()[name]
+    u = np.random.default_rng(6).standard_normal(c.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(CircuitPlan(c, 0.05, leaf_size=8).K),
+                                       c.get_cycle_matrix() @ u)
+    base = CircuitPlan(c, 0.05, leaf_size=8, block_size=BS, parts=parts, face_store=store, local_stages=0)
+    for k in (1, 2, 99):
+        plan = base.relayout(BS, local_stages=k)
+        assert plan.local_stages == min(k, plan.solve_plan.n_stages)
+        assert plan.scratch_rows == max(plan.solve_plan.max_stage_rows,
+                                        int(plan.solve_plan.stage_ptr[plan.local_stages]))
+        assert _native.lib().jja_plan_validate(1, C.byref(plan.desc())) == 0, _native.lib().jja_last_error()
+        np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12,
+                                   atol=1e-12 * abs(want).max())
+
+
+def test_wave_local_barrier_count():
+    """k wave-local stages replace their 4k - 1 phase barriers (the leaves'
+    first phase is skipped) by one per pass."""
+    c = pa.TriangularArray(14, 14)
+    base = CircuitPlan(c, 0.05, leaf_size=8, block_size=256, local_stages=0)
+
+    def barriers(plan):
+        return sum(1 for r in plan.wave_program(0) if r[5] & 0x400)
+    n0 = barriers(base)
+    assert n0 == 4 * base.solve_plan.n_stages - 1
+    for k in (1, 3):
+        assert barriers(base.relayout(256, local_stages=k)) == n0 - (4 * k - 1) + 2
+
+
+class _MisownedPlan(CircuitPlan):
+    """Planted mistake: the leaves of one subtree go to another wave than their
+    separators, so two waves share rows between barriers."""
+
+    def _wave_owner(self, NW, n_low):
+        owner = super()._wave_owner(NW, n_low)
+        if owner is not None:
+            leaves = int(self.solve_plan.stage_ptr[1])
+            moved = owner[:leaves] == owner[0]
+            owner[:leaves][moved] = (owner[0] + 1) % NW
+        return owner
+
+
+def test_wave_local_race_is_detected():
+    c = _circuits()["tri8"]
+    plan = _MisownedPlan(c, 0.05, leaf_size=8, block_size=256, local_stages=3)
+    u = np.random.default_rng(7).standard_normal(c.junction_count())
+    with pytest.raises(AssertionError, match="another wave"):
+        plan.emulate_face_and_solve(u)
+
+
+def test_local_stage_choice_and_lds_fit():
+    """auto: none with one workgroup per column, some for split columns. A
+    count whose scratch does not fit LDS moves the store to global memory, or
+    is refused when LDS is forced."""
+    c = _circuits()["tri8"]
+    assert CircuitPlan(c, 0.05, leaf_size=8, block_size=256).local_stages == 0
+    assert CircuitPlan(c, 0.05, leaf_size=8, block_size=256, parts=2).local_stages > 0
+    with pytest.raises(ValueError, match="local_stages"):
+        CircuitPlan(c, 0.05, local_stages=-1)
+    big = pa.SquareArray(103, 103)          # 10404 faces: LDS holds the store only in step
+    p0 = CircuitPlan(big, 0.05, block_size=1024)
+    assert not p0.global_store and p0.local_stages == 0
+    pall = p0.relayout(1024, local_stages=99)
+    assert pall.global_store and pall.scratch_rows > p0.scratch_rows
+    with pytest.raises(ValueError, match="LDS"):
+        p0.relayout(1024, face_store="lds", local_stages=99)
