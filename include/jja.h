@@ -177,6 +177,14 @@ int jja_plan_status(jja_plan* plan);
  * counterpart in the reference.
  */
 int jja_plan_set_wait_limit(jja_plan* plan, uint32_t polls);
+/*
+ * Workgroups of the plan's split launches (since the previous call, which
+ * resets the count; waits for the device) whose column had every part on one
+ * XCD: their column barriers skip the L2 write-back. A
+ * diagnostic of the launch layout; returns the count or a negative JJA_E*
+ * code. No counterpart in the reference.
+ */
+int64_t jja_plan_one_xcd_parts(jja_plan* plan);
 /* Dynamic LDS bytes one column of problem p needs (0 if it cannot run). */
 int64_t jja_plan_lds_bytes(const jja_plan* plan, int32_t p);
 /* Doubles per column of the global face store problem p needs (0: it uses LDS). */

@@ -243,28 +243,41 @@ constexpr uint32_t kWaitLimit = 1u << 21;
 // lgkmcnt fields at their maxima
 constexpr int kWaitVmem = 0x0F70;
 
+// The XCD (accelerator complex die) the calling wave runs on: hardware
+// register XCC_ID, bits 3:0.
+__device__ __forceinline__ uint32_t xcd_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;
+}
+
 struct SplitStore {
   static constexpr bool kGlobal = true;
   GAS double* p;
   int32_t vt, vbs;
   uint32_t* ctr;         // the column's arrival counter, zero at launch
+  uint32_t* xcds;        // bit x set by each part running on XCD x, zero at launch
   uint32_t parts;
   uint32_t wait_limit;   // polls before a wait gives up
-  int32_t* err;          // launch status: set when a wait gave up
+  int32_t* err;          // [0] set when a wait gave up; [1] counts one-XCD workgroups
   int32_t* stopped;      // this workgroup's flag (LDS): it no longer waits
   mutable uint32_t gen;  // arrivals to wait for (thread 0)
+  mutable bool one_xcd;  // every part of the column shares this XCD's L2 (thread 0)
   __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
   __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
   // Every wave waits for its own memory operations (stores reach the L2,
   // which the L1 writes through to), the workgroup meets, and thread 0 alone
-  // releases at device scope (one L2 write-back per workgroup), arrives, polls
-  // the counter with relaxed loads (an acquire per poll would invalidate the
-  // L2 under the working workgroups) and acquires once (one L1 and L2
-  // invalidation, which covers the compute unit's other waves); then the
-  // workgroup meets again. A wait that gives up sets err and stops this
-  // workgroup's waits for the rest of the launch (its results are then
-  // invalid, and the host reports the error).
+  // arrives on the column's counter, polls it with relaxed loads (an acquire
+  // per poll would invalidate caches under the working workgroups) and
+  // acquires once at device scope (the invalidation covers the compute unit's
+  // other waves); then the workgroup meets again.
+  //   Parts on several XCDs (separate L2s) release at device scope before the
+  // arrival: an L2 write-back. Parts on one XCD skip it: their stores already
+  // meet in that XCD's L2. The parts publish their XCDs at launch
+  // (split_store); each workgroup reads the column's set after its first
+  // barrier, which always releases.
+  //   A wait that gives up sets err and stops this workgroup's waits for the
+  // rest of the launch (its results are then invalid, and the host reports
+  // the error).
   __device__ __forceinline__ void sync() const {
     if (parts == 1) {  // a column of one workgroup (in a launch with split columns)
       __syncthreads();
@@ -274,7 +287,10 @@ struct SplitStore {
     __syncthreads();
     if (threadIdx.x == 0 && !*stopped) {
       gen += parts;
-      __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (one_xcd)
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       uint32_t spins = 0;
       while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
         if (++spins > wait_limit) {
@@ -285,6 +301,11 @@ struct SplitStore {
         __builtin_amdgcn_s_sleep(2);
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // after the first barrier every part has published its XCD
+      if (gen == parts && !*stopped) {
+        one_xcd = __builtin_popcount(__hip_atomic_load(xcds, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 1;
+        if (one_xcd) __hip_atomic_fetch_add(err + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     __syncthreads();
   }
@@ -811,12 +832,16 @@ __device__ __forceinline__ SplitStore split_store(const DevProblem& P, GAS doubl
   zs.p = zbuf + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
   zs.vt = part * BS + static_cast<int32_t>(threadIdx.x);
   zs.vbs = P.parts * BS;
-  zs.ctr = ctrs + wi.slot;
+  zs.ctr = ctrs + 2 * static_cast<int64_t>(wi.slot);
+  zs.xcds = zs.ctr + 1;
   zs.parts = static_cast<uint32_t>(P.parts);
   zs.wait_limit = wait_limit;
   zs.err = err;
   zs.stopped = stopped;
   zs.gen = 0;
+  zs.one_xcd = false;
+  if (threadIdx.x == 0 && P.parts > 1)
+    __hip_atomic_fetch_or(zs.xcds, 1u << xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return zs;
 }
 
@@ -1049,9 +1074,10 @@ struct Plan {
   int32_t stencil_width = 3;
   jja_adjoint_run* d_adj = nullptr;  // allocated at the first jja_adjoint call
   int32_t parts = 1;                 // workgroups per column (block_size is one workgroup's)
-  uint32_t* d_ctrs = nullptr;        // split launches: one barrier counter per column
+  uint32_t* d_ctrs = nullptr;        // split launches: barrier words per column (counter, XCD set)
   size_t ctr_cap = 0;
-  int32_t* d_err = nullptr;          // split launches: set when a column barrier gave up
+  int32_t* d_err = nullptr;          // split launches: [0] a column barrier gave up,
+                                     // [1] workgroups whose column ran on one XCD
   int32_t n_cu = 0;                  // compute units of the device
   uint32_t wait_limit = kWaitLimit;  // split launches: polls before a column barrier gives up
 };
@@ -1347,8 +1373,10 @@ std::vector<WorkItem> split_work(const Plan* plan, std::vector<WorkItem> cols) {
   return out;
 }
 
-// Barrier counters of n columns, zeroed on the stream before a split launch.
+// Barrier words of n columns (arrival counter, XCD set), zeroed on the stream
+// before a split launch.
 int reset_counters(Plan* plan, size_t n, hipStream_t stream) {
+  n *= 2;
   if (n > plan->ctr_cap) {
     if (plan->d_ctrs) {
       auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_ctrs));
@@ -1546,7 +1574,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
   }
   if (rc == JJA_OK && plan->parts > 1) {
     void* d = nullptr;
-    if (hipMalloc(&d, sizeof(int32_t)) != hipSuccess || hipMemset(d, 0, sizeof(int32_t)) != hipSuccess) {
+    if (hipMalloc(&d, 2 * sizeof(int32_t)) != hipSuccess || hipMemset(d, 0, 2 * sizeof(int32_t)) != hipSuccess) {
       rc = fail(JJA_ENOMEM, "jja_plan_create: device allocation failed");
     } else {
       plan->allocations.push_back(d);
@@ -1570,6 +1598,16 @@ int jja_plan_status(jja_plan* p) {
   if (!e) return JJA_OK;
   JJA_HIP(hipMemset(plan->d_err, 0, sizeof(int32_t)));
   return fail(JJA_EHIP, "a column barrier of a split launch gave up waiting; its results are invalid");
+}
+
+int64_t jja_plan_one_xcd_parts(jja_plan* p) {
+  Plan* plan = reinterpret_cast<Plan*>(p);
+  if (!plan) return fail(JJA_EINVAL, "jja_plan_one_xcd_parts: invalid arguments");
+  if (!plan->d_err) return 0;
+  int32_t n = 0;
+  JJA_HIP(hipMemcpy(&n, plan->d_err + 1, sizeof(n), hipMemcpyDeviceToHost));
+  JJA_HIP(hipMemset(plan->d_err + 1, 0, sizeof(int32_t)));
+  return n;
 }
 
 int jja_plan_set_wait_limit(jja_plan* p, uint32_t polls) {

@@ -43,6 +43,45 @@ def test_gpu_split_columns_bit_identical(T):
         np.testing.assert_array_equal(r.get_vortex_configuration(), ref.get_vortex_configuration())
 
 
+def _engine_run(c, parts, W, Nt=150):
+    """theta output and the count of workgroups whose column barriers found
+    every part of their column on one XCD, over the run's launches."""
+    from pyjjasim_amd.engine import Engine
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.3, 1.4, W)[None, :, None]
+    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
+                                   current_sources=Is, temperature=0.02, seed=9,
+                                   store_time_steps=[Nt // 2, Nt - 1], block_size=256,
+                                   face_store="global", parts=parts, steps_per_launch=50)
+    eng = Engine(prob, prob.store_time_steps.copy(), np.zeros(Nt, dtype=bool))
+    try:
+        eng.advance(Nt)
+        th, _, nv = eng.results()
+        n = eng.lib.jja_plan_one_xcd_parts(eng.handle)
+        assert n >= 0
+    finally:
+        eng.close()
+    return th, nv, n
+
+
+@pytest.mark.parametrize("W,parts,packed", [(8, 2, True), (8, 4, True), (16, 4, True), (3, 4, False)])
+def test_gpu_split_one_xcd_barriers_bit_identical(W, parts, packed):
+    """Launches whose workgroups divide evenly over the 8 XCDs place every
+    column's parts on one XCD (block indices equal modulo 8); their column
+    barriers then skip the L2 write-back. W=3 x 4 parts cannot
+    be packed, and its barriers take the device-scope path. Either way the
+    results are those of one workgroup per column."""
+    c = _circuit()
+    print(f"[xcd] W={W}, one part", flush=True)
+    ref_th, ref_nv, n1 = _engine_run(c, 1, W)
+    assert n1 == 0
+    print(f"[xcd] W={W}, {parts} parts", flush=True)
+    th, nv, n = _engine_run(c, parts, W)
+    np.testing.assert_array_equal(th, ref_th)
+    np.testing.assert_array_equal(nv, ref_nv)
+    launches = 3
+    assert n == (launches * W * parts if packed else 0)
+
+
 @pytest.mark.parametrize("store,parts,local", [("lds", 1, 3), ("lds", 1, 99), ("global", 1, 99),
                                                ("global", 4, 0), ("global", 4, 2)])
 def test_gpu_wave_local_stages_bit_identical(store, parts, local):
