@@ -152,6 +152,12 @@ typedef struct jja_problem_desc {
    * index_bytes 4, and jja_evolve / jja_adjoint launch cooperatively (all
    * workgroups of the launch resident at once, or JJA_EUNSUPPORTED). */
   int32_t parts;
+  /* 0, or 1: split columns are worked two at a time (columns 2q and 2q+1 of a
+   * problem share one set of parts workgroups, which stream the solve records
+   * once for both and meet at one column barrier; the last column is alone when
+   * W is odd). Needs parts >= 2 and block_size/parts <= 512; the same value for
+   * every problem of a plan. Results are those of unpaired columns, bit for bit. */
+  int32_t column_pairs;
 } jja_problem_desc;
 
 /*

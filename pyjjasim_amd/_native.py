@@ -48,6 +48,7 @@ class ProblemDesc(C.Structure):
         ("ent_off", C.c_void_p),
         ("max_stage_rows", C.c_int32), ("face_copy_rows", C.c_int32),
         ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("parts", C.c_int32),
+        ("column_pairs", C.c_int32),
     ]
 
 

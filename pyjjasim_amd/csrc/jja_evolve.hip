@@ -445,6 +445,81 @@ __device__ __forceinline__ void solve(const DevProblem& P, const Z& zs) {
   }
 }
 
+// Two columns of one problem through the same record (their face stores za and
+// zb): each column's gathers, sums and writes are exactly those of
+// process_slice, so each column's results are bit-identical to a solve of its
+// own; the record's values and indices are loaded once for both.
+template <class Z>
+__device__ __forceinline__ void process_slice_pair(const Z& za, const Z& zb, int32_t Nf, int lane,
+                                                   const jja_slice& m, const SliceRegs<Z::kGlobal>& r,
+                                                   double& acca, double& accb) {
+  double va[kMaxIter], vb[kMaxIter];
+#pragma unroll
+  for (int i = 0; i < kMaxIter; ++i) {
+    const uint32_t ix = slice_index(r, i);
+    va[i] = za.gather(ix);
+    vb[i] = zb.gather(ix);
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxIter; ++i) {
+    acca = __builtin_fma(r.v[i], va[i], acca);
+    accb = __builtin_fma(r.v[i], vb[i], accb);
+  }
+  if (m.flags & JJA_SLICE_FINISH) {
+    const int G = m.flags & 0xff;
+    acca = group_sum_last(acca, G);
+    accb = group_sum_last(accb, G);
+    const int slot = lane >> __builtin_ctz(static_cast<unsigned>(G));
+    if (slot < (m.flags >> 16) && r.row >= 0 && (lane & (G - 1)) == G - 1) {
+      if (m.flags & JJA_SLICE_BLOCK) {
+        za.wr(r.row, acca);
+        zb.wr(r.row, accb);
+      } else {
+        za.wr(Nf + r.row - m.r0, za.rd(r.row) - acca);
+        zb.wr(Nf + r.row - m.r0, zb.rd(r.row) - accb);
+      }
+    }
+    acca = 0.0;
+    accb = 0.0;
+  }
+  if (m.flags & JJA_SLICE_BARRIER) za.barrier();
+}
+
+// solve() for two columns that share one set of threads and one barrier (za's).
+template <class Z>
+__device__ __forceinline__ void solve_pair(const DevProblem& P, const Z& za, const Z& zb) {
+  constexpr bool GLOBAL = Z::kGlobal;
+  const int lane = za.vt & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(za.vt / kWave);
+  const CAS int32_t* wp = (const CAS int32_t*)(P.wave_ptr) + wave;
+  const int32_t k0 = __builtin_amdgcn_readfirstlane(wp[0]);
+  const int32_t nk = __builtin_amdgcn_readfirstlane(wp[1]) - k0 - 4;
+  const jja_slice* __restrict__ prog = P.solve_slices + k0;
+  jja_slice m0 = load_rec(prog, 0), m1 = load_rec(prog, 1), m2 = load_rec(prog, 2);
+  const __amdgpu_buffer_rsrc_t vres = value_resource(P);
+  SliceRegs<GLOBAL> A, B, C;
+  load_slice(P, vres, m0, lane, A);
+  load_slice(P, vres, m1, lane, B);
+  double acca = 0.0, accb = 0.0;
+  for (int32_t k = 0; k < nk; k += 3) {
+    const jja_slice m3 = load_rec(prog, k + 3);
+    load_slice(P, vres, m2, lane, C);
+    __builtin_amdgcn_sched_barrier(0);
+    process_slice_pair(za, zb, P.Nf, lane, m0, A, acca, accb);
+    const jja_slice m4 = load_rec(prog, k + 4);
+    load_slice(P, vres, m3, lane, A);
+    __builtin_amdgcn_sched_barrier(0);
+    process_slice_pair(za, zb, P.Nf, lane, m1, B, acca, accb);
+    const jja_slice m5 = load_rec(prog, k + 5);
+    load_slice(P, vres, m4, lane, B);
+    __builtin_amdgcn_sched_barrier(0);
+    process_slice_pair(za, zb, P.Nf, lane, m2, C, acca, accb);
+    m0 = m3;
+    m1 = m4;
+    m2 = m5;
+  }
+}
+
 // One input term of one column at one step: element i is p[i * si] * amp.
 // amp is exactly 1 for a term without amplitude; a term that is absent reads
 // an exact +0.0 from the plan's zero page (si = 0), which leaves every sum it
@@ -806,6 +881,45 @@ __device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_prob
   }
 }
 
+// Columns wa and wa+1 (two) or wa alone of one problem, worked by one set of
+// threads: the face phases and junction passes run column by column, the solve
+// streams each record once for both, and za's barriers serve both.
+template <int BS, int SW>
+__device__ __forceinline__ void evolve_steps_pair(const DevProblem& P, const jja_problem_run& R, int64_t wa,
+                                                  bool two, const SplitStore& za, const SplitStore& zb,
+                                                  int32_t n_steps, int64_t step0) {
+  const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
+  const int64_t wb = wa + 1;
+  const size_t ca = static_cast<size_t>(wa) * P.Nj, cb = two ? static_cast<size_t>(wb) * P.Nj : ca;
+  GAS double* __restrict__ tha = gblw(R.theta) + ca;
+  GAS double* __restrict__ thb = gblw(R.theta) + cb;
+  GAS double* __restrict__ thsa = R.theta_s ? gblw(R.theta_s) + ca : nullptr;
+  GAS double* __restrict__ thsb = R.theta_s ? gblw(R.theta_s) + cb : nullptr;
+  GAS double* __restrict__ ua = gblw(R.u) + ca;
+  GAS double* __restrict__ ub = gblw(R.u) + cb;
+  const uint32_t na = static_cast<uint32_t>(R.column_offset + wa), nb = static_cast<uint32_t>(R.column_offset + wb);
+
+  junction_pass<BS, SW>(P, R, tha, thsa, ua, za, wa, slot_stride, step0, false, step0, true, na);
+  if (two) junction_pass<BS, SW>(P, R, thb, thsb, ub, zb, wb, slot_stride, step0, false, step0, true, nb);
+  za.sync();
+  for (int32_t n = 0; n < n_steps; ++n) {
+    const int64_t t = step0 + n;
+    if (P.Nf > 0) {
+      face_phase<BS>(P, &R.f, ua, za, wa, t);
+      if (two) face_phase<BS>(P, &R.f, ub, zb, wb, t);
+      za.barrier();
+      if (two)
+        solve_pair(P, za, zb);
+      else
+        solve(P, za);
+    }
+    const bool prep = n + 1 < n_steps;
+    junction_pass<BS, SW>(P, R, tha, thsa, ua, za, wa, slot_stride, t, true, t + 1, prep, na);
+    if (two) junction_pass<BS, SW>(P, R, thb, thsb, ub, zb, wb, slot_stride, t, true, t + 1, prep, nb);
+    za.sync();
+  }
+}
+
 // Column w of one problem in one workgroup: face vector by solve position,
 // stage scratch after it.
 template <int BS, int SW, bool GLOBAL>
@@ -845,7 +959,9 @@ __device__ __forceinline__ SplitStore split_store(const DevProblem& P, GAS doubl
   return zs;
 }
 
-template <int BS, int SW>
+// PAIR: a work item is a pair of columns (2q, 2q+1; the last one alone when W is
+// odd), its barrier words those of the pair.
+template <int BS, int SW, bool PAIR>
 __global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __restrict__ probs,
                                                           const jja_problem_run* __restrict__ runs,
                                                           const WorkItem* __restrict__ work,
@@ -859,8 +975,17 @@ __global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __re
   const jja_problem_run& R = runs[wi.problem];
   const int64_t w = wi.column / P.parts;
   const int32_t part = wi.column % P.parts;
-  const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
-  evolve_steps<BS, SW>(P, R, w, zs, n_steps, step0);
+  if constexpr (PAIR) {
+    const int64_t wa = 2 * w;
+    const bool two = wa + 1 < R.W;
+    const SplitStore za = split_store(P, gblw(R.zbuf), wa, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    SplitStore zb = za;
+    if (two) zb.p = za.p + (static_cast<size_t>(P.Nf) + P.max_stage_rows);
+    evolve_steps_pair<BS, SW>(P, R, wa, two, za, zb, n_steps, step0);
+  } else {
+    const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    evolve_steps<BS, SW>(P, R, w, zs, n_steps, step0);
+  }
 }
 
 // One workgroup per work item (problem, column). STORE: 0 face stores in LDS,
@@ -988,6 +1113,41 @@ __device__ __forceinline__ void adjoint_steps(const DevProblem& P, const jja_adj
   }
 }
 
+// adjoint_steps for columns wa and wa+1 (two) or wa alone, on one set of
+// threads: per-column face phases and junction updates, one shared solve, za's
+// barriers.
+template <int BS>
+__device__ __forceinline__ void adjoint_steps_pair(const DevProblem& P, const jja_adjoint_run& A, int64_t wa,
+                                                   bool two, const SplitStore& za, const SplitStore& zb,
+                                                   int64_t k_hi, int64_t k_lo) {
+  const int64_t wb = two ? wa + 1 : wa;
+  const AdjCol Ca = adjoint_column(P, A, wa), Cb = adjoint_column(P, A, wb);
+  GAS double* __restrict__ gfa = gblw(A.grad_f) + static_cast<size_t>(wa) * P.Nf;
+  GAS double* __restrict__ gfb = gblw(A.grad_f) + static_cast<size_t>(wb) * P.Nf;
+  if (A.prime) {
+    adjoint_prime(P, A, Ca, za, k_hi);
+    if (two) adjoint_prime(P, A, Cb, zb, k_hi);
+  }
+  za.sync();
+  for (int64_t k = k_hi; k >= k_lo; --k) {
+    if (P.Nf > 0) {
+      face_phase<BS>(P, nullptr, Ca.u, za, wa, k);
+      if (two) face_phase<BS>(P, nullptr, Cb.u, zb, wb, k);
+      za.barrier();
+      if (two)
+        solve_pair(P, za, zb);
+      else
+        solve(P, za);
+    }
+    for (int32_t p = za.vt; p < P.Nf; p += za.vbs) gfa[p] = gfa[p] - kTwoPi * za.rd(p);
+    if (two)
+      for (int32_t p = zb.vt; p < P.Nf; p += zb.vbs) gfb[p] = gfb[p] - kTwoPi * zb.rd(p);
+    adjoint_junctions(P, A, Ca, za, k, wa);
+    if (two) adjoint_junctions(P, A, Cb, zb, k, wb);
+    za.sync();
+  }
+}
+
 template <int BS, bool GLOBAL>
 __global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restrict__ probs,
                                                      const jja_adjoint_run* __restrict__ runs,
@@ -1008,7 +1168,7 @@ __global__ __launch_bounds__(BS) void adjoint_kernel(const DevProblem* __restric
   adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
 }
 
-template <int BS>
+template <int BS, bool PAIR>
 __global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __restrict__ probs,
                                                            const jja_adjoint_run* __restrict__ runs,
                                                            const WorkItem* __restrict__ work, int64_t k_hi,
@@ -1022,8 +1182,17 @@ __global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __r
   const jja_adjoint_run& A = runs[wi.problem];
   const int64_t w = wi.column / P.parts;
   const int32_t part = wi.column % P.parts;
-  const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
-  adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
+  if constexpr (PAIR) {
+    const int64_t wa = 2 * w;
+    const bool two = wa + 1 < A.W;
+    const SplitStore za = split_store(P, gblw(A.zbuf), wa, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    SplitStore zb = za;
+    if (two) zb.p = za.p + (static_cast<size_t>(P.Nf) + P.max_stage_rows);
+    adjoint_steps_pair<BS>(P, A, wa, two, za, zb, k_hi, k_lo);
+  } else {
+    const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
+  }
 }
 
 // n[col][f] = -sum_j A[f,j] rint(theta_j / 2pi) (time_evolution.py:734-755): one
@@ -1074,6 +1243,7 @@ struct Plan {
   int32_t stencil_width = 3;
   jja_adjoint_run* d_adj = nullptr;  // allocated at the first jja_adjoint call
   int32_t parts = 1;                 // workgroups per column (block_size is one workgroup's)
+  bool pairs = false;                // split launches work two columns per work item
   uint32_t* d_ctrs = nullptr;        // split launches: barrier words per column (counter, XCD set)
   size_t ctr_cap = 0;
   int32_t* d_err = nullptr;          // split launches: [0] a column barrier gave up,
@@ -1120,6 +1290,8 @@ int check_plan_shapes(int32_t n, const jja_problem_desc* descs) {
       return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size/parts");
     if (split && d.index_bytes != 4)
       return fail(JJA_EINVAL, "jja_plan_create: with a split column, every problem needs the global face store");
+    if ((d.column_pairs != 0) != (descs[0].column_pairs != 0))
+      return fail(JJA_EINVAL, "jja_plan_create: column_pairs must be the same for every problem of a plan");
   }
   return JJA_OK;
 }
@@ -1137,6 +1309,9 @@ int validate(const jja_problem_desc& d) {
     return fail(JJA_EINVAL, "jja_plan_create: block_size/parts must be 64, 128, 256, 512 or 1024");
   if (parts > 1 && d.index_bytes != 4)
     return fail(JJA_EINVAL, "jja_plan_create: a column split over workgroups needs the global face store");
+  if (d.column_pairs != 0 && (d.column_pairs != 1 || parts < 2 || wg > 512))
+    return fail(JJA_EINVAL, "jja_plan_create: column_pairs needs split columns (parts >= 2) and workgroups "
+                            "of at most 512 threads");
   if (!d.Ic || !d.Rv || !d.coef || !d.junc_code)
     return fail(JJA_EINVAL, "jja_plan_create: missing junction arrays");
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
@@ -1373,8 +1548,12 @@ std::vector<WorkItem> split_work(const Plan* plan, std::vector<WorkItem> cols) {
   return out;
 }
 
-// Barrier words of n columns (arrival counter, XCD set), zeroed on the stream
-// before a split launch.
+// Work items of a problem with W columns: one per column, or one per pair of
+// columns (the last one alone when W is odd) in a plan with column pairs.
+int32_t items_of(const Plan* plan, int32_t W) { return plan->pairs ? (W + 1) / 2 : W; }
+
+// Barrier words of n work items (arrival counter, XCD set), zeroed on the
+// stream before a split launch.
 int reset_counters(Plan* plan, size_t n, hipStream_t stream) {
   n *= 2;
   if (n > plan->ctr_cap) {
@@ -1413,8 +1592,10 @@ int launch_evolve_split_bs(Plan* plan, int32_t n, int32_t n_steps, int64_t step0
   const DevProblem* probs = plan->d_problems;
   const jja_problem_run* runs = plan->d_runs;
   const WorkItem* work = plan->d_work;
-  return launch_cooperative(plan, reinterpret_cast<const void*>(evolve_split_kernel<BS, 3>), BS, n, stream,
-                            probs, runs, work, n_steps, step0, plan->d_ctrs, plan->d_err, plan->wait_limit);
+  const void* kern = plan->pairs ? reinterpret_cast<const void*>(evolve_split_kernel<BS, 3, BS <= 512>)
+                                 : reinterpret_cast<const void*>(evolve_split_kernel<BS, 3, false>);
+  return launch_cooperative(plan, kern, BS, n, stream, probs, runs, work, n_steps, step0, plan->d_ctrs,
+                            plan->d_err, plan->wait_limit);
 }
 
 int launch_evolve_split(Plan* plan, int32_t n, int32_t n_steps, int64_t step0, hipStream_t stream) {
@@ -1432,8 +1613,10 @@ int launch_adjoint_split_bs(Plan* plan, int32_t n, int64_t k_hi, int64_t k_lo, h
   const DevProblem* probs = plan->d_problems;
   const jja_adjoint_run* runs = plan->d_adj;
   const WorkItem* work = plan->d_work;
-  return launch_cooperative(plan, reinterpret_cast<const void*>(adjoint_split_kernel<BS>), BS, n, stream,
-                            probs, runs, work, k_hi, k_lo, plan->d_ctrs, plan->d_err, plan->wait_limit);
+  const void* kern = plan->pairs ? reinterpret_cast<const void*>(adjoint_split_kernel<BS, BS <= 512>)
+                                 : reinterpret_cast<const void*>(adjoint_split_kernel<BS, false>);
+  return launch_cooperative(plan, kern, BS, n, stream, probs, runs, work, k_hi, k_lo, plan->d_ctrs,
+                            plan->d_err, plan->wait_limit);
 }
 
 int launch_adjoint_split(Plan* plan, int32_t n, int64_t k_hi, int64_t k_lo, hipStream_t stream) {
@@ -1478,6 +1661,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
   Plan* plan = new Plan();
   plan->parts = 1;
   for (int32_t p = 0; p < n_problems; ++p) plan->parts = std::max(plan->parts, parts_of(descs[p]));
+  plan->pairs = descs[0].column_pairs != 0;
   plan->block_size = descs[0].block_size / parts_of(descs[0]);
   plan->stencil_width = descs[0].stencil_width;
   int rc = JJA_OK;
@@ -1658,7 +1842,7 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
                                         " needs " + std::to_string(plan->lds_bytes[i]) +
                                         " bytes of LDS; limit " + std::to_string(plan->lds_limit));
     if (!global) lds = std::max(lds, plan->lds_bytes[i]);
-    for (int32_t w = 0; w < r.W; ++w) (global ? work_g : work).push_back({i, w});
+    for (int32_t w = 0; w < items_of(plan, r.W); ++w) (global ? work_g : work).push_back({i, w});
   }
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
   // pageable source: the call returns once the data is staged, and the copy is
@@ -1705,7 +1889,7 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
     if (!global && plan->lds_bytes[i] > plan->lds_limit)
       return fail(JJA_EUNSUPPORTED, "jja_adjoint: face vector of problem " + std::to_string(i) + " exceeds LDS");
     if (!global) lds = std::max(lds, plan->lds_bytes[i]);
-    for (int32_t w = 0; w < r.W; ++w) (global ? work_g : work).push_back({i, w});
+    for (int32_t w = 0; w < items_of(plan, r.W); ++w) (global ? work_g : work).push_back({i, w});
   }
   if (!plan->d_adj) {
     void* d = nullptr;

@@ -161,6 +161,9 @@ class Engine:
                 parts = auto_parts(self.host, W, sw)
             if parts > 1:
                 self.host = self.host.relayout(self.block_size, parts)
+            if getattr(problem, "column_pairs", False):
+                self.host = self.host.paired()
+                self.block_size = self.host.block_size
             self._desc = self.host.desc()
             self.handle = C.c_void_p()
             _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))

@@ -366,6 +366,7 @@ class CircuitPlan:
         self.face_store = face_store
         self.parts = parts
         self.local_option = local_stages
+        self.column_pairs = False
         if stencil_width not in STENCILS:
             raise ValueError(f"stencil width must be 3, 4 or 5 (equals {stencil_width})")
         self.Nj, self.Nf = circuit._Nj(), circuit._Nf()
@@ -397,6 +398,7 @@ class CircuitPlan:
         factorisation depends on none of them)."""
         other = copy.copy(self)
         other.block_size = block_size
+        other.column_pairs = False
         if local_stages is not None:
             if local_stages != "auto" and (int(local_stages) != local_stages or local_stages < 0):
                 raise ValueError("local_stages must be 'auto' or a count >= 0")
@@ -410,6 +412,18 @@ class CircuitPlan:
         if other.parts > 1 and other.face_store == "lds":
             raise ValueError("a column split over workgroups needs the global face store")
         other._build_layouts(self.target_iter)
+        return other
+
+    def paired(self):
+        """This split plan for columns worked two at a time: workgroups of half
+        the size, twice the parts, so the same threads per column and the same
+        record programs (nothing is rebuilt)."""
+        if self.parts < 2 or self.block_size not in (512, 1024) or 2 * self.parts > 32:
+            raise ValueError("column pairs need a split plan (parts 2..16) with workgroups of 512 or 1024 threads")
+        other = copy.copy(self)
+        other.block_size = self.block_size // 2
+        other.parts = 2 * self.parts
+        other.column_pairs = True
         return other
 
     @property
@@ -785,6 +799,7 @@ class CircuitPlan:
         d.n_junctions, d.n_faces = self.Nj, self.Nf
         d.stencil_width, d.block_size = self.stencil_width, self.column_threads
         d.parts = self.parts
+        d.column_pairs = int(getattr(self, "column_pairs", False))
         d.Ic = P(self.Ic, C.c_double)
         d.Rv = P(self.Rv, C.c_double)
         d.coef = P(self.coef, C.c_double)

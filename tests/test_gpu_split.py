@@ -98,6 +98,31 @@ def test_gpu_wave_local_stages_bit_identical(store, parts, local):
     np.testing.assert_array_equal(r.get_vortex_configuration(), ref.get_vortex_configuration())
 
 
+def _run_pairs(c, W, pairs, Nt=120):
+    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.3, 1.4, W)[None, :, None]
+    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
+                                   current_sources=Is, temperature=0.02, seed=11,
+                                   store_time_steps=[Nt // 2, Nt - 1], block_size=512,
+                                   face_store="global", parts=2, steps_per_launch=50,
+                                   column_pairs=pairs)
+    return prob.compute()
+
+
+@pytest.mark.parametrize("W", [2, 3, 8])
+def test_gpu_column_pairs_bit_identical(W):
+    """Split columns worked two at a time (one record stream, one barrier for
+    both; the last column alone when W is odd) give the results of unpaired
+    split columns."""
+    c = _circuit()
+    print(f"[pairs] W={W}, unpaired", flush=True)
+    ref = _run_pairs(c, W, False)
+    print(f"[pairs] W={W}, paired", flush=True)
+    r = _run_pairs(c, W, True)
+    np.testing.assert_array_equal(r.get_theta(), ref.get_theta())
+    np.testing.assert_array_equal(r.get_voltage(), ref.get_voltage())
+    np.testing.assert_array_equal(r.get_vortex_configuration(), ref.get_vortex_configuration())
+
+
 def test_gpu_split_adjoint_bit_identical():
     torch = pytest.importorskip("torch")
     c = _circuit()
@@ -112,13 +137,15 @@ def test_gpu_split_adjoint_bit_identical():
     G = torch.tensor(rng.standard_normal((Nj, W, 2)), dtype=torch.float64, device="cuda")
     grads = []
     # one part in step, four parts (wave-local lowest stages by default), one
-    # part with every stage wave-local
-    for parts, local in ((1, 0), (4, "auto"), (1, 99)):
-        print(f"[split] adjoint, {parts} parts, local_stages={local}", flush=True)
+    # part with every stage wave-local, and columns paired (512-thread
+    # workgroups halved, 2 parts doubled)
+    for parts, local, bs, pairs in ((1, 0, 256, False), (4, "auto", 256, False), (1, 99, 256, False),
+                                    (2, "auto", 512, True)):
+        print(f"[split] adjoint, {parts} parts, local_stages={local}, pairs={pairs}", flush=True)
         evo = pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W,
                                          temperature=0.01, store_time_steps=store, checkpoint_every=30,
-                                         block_size=256, face_store="global", parts=parts, seed=3,
-                                         local_stages=local)
+                                         block_size=bs, face_store="global", parts=parts, seed=3,
+                                         local_stages=local, column_pairs=pairs)
         leaves = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
         th = evo(*leaves)
         (th * G).sum().backward()

@@ -354,3 +354,30 @@ def test_local_stage_choice_and_lds_fit():
     assert pall.global_store and pall.scratch_rows > p0.scratch_rows
     with pytest.raises(ValueError, match="LDS"):
         p0.relayout(1024, face_store="lds", local_stages=99)
+
+
+def test_column_pair_plans_validate():
+    """A split plan worked two columns at a time keeps its programs (half the
+    workgroup size, twice the parts); the library accepts it, and refuses
+    pairs without split columns, an unknown value or plans that mix them."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    c = _circuits()["tri8"]
+    split = CircuitPlan(c, 0.05, leaf_size=8, block_size=512, parts=2)
+    pair = split.paired()
+    assert (pair.block_size, pair.parts, pair.column_threads) == (256, 4, split.column_threads)
+    assert pair.solve_slices is split.solve_slices
+    d = pair.desc()
+    assert d.column_pairs == 1
+    assert _native.lib().jja_plan_validate(1, C.byref(d)) == 0, _native.lib().jja_last_error()
+    d.column_pairs = 2
+    assert _native.lib().jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
+    one = CircuitPlan(c, 0.05, leaf_size=8, block_size=256, face_store="global").desc()
+    one.column_pairs = 1                          # pairs need split columns
+    assert _native.lib().jja_plan_validate(1, C.byref(one)) == _native.JJA_EINVAL
+    mixed = (_native.ProblemDesc * 2)(pair.desc(), CircuitPlan(c, 0.05, leaf_size=8, block_size=256,
+                                                                  parts=4).desc())
+    assert _native.lib().jja_plan_validate(2, mixed) == _native.JJA_EINVAL
+    with pytest.raises(ValueError, match="column pairs"):
+        CircuitPlan(c, 0.05, leaf_size=8, block_size=256).paired()
+    assert not split.relayout(512, 2).column_pairs
