@@ -73,7 +73,11 @@ const char* jja_factor_last_error(void);
  *     Face codes pad with -1.
  * Solve records (one wavefront of 64 lanes each): every record holds 8 entries
  *     per lane, zero-padded. Value `it` of lane l at base + (it/2)*128 + 2*l +
- *     it%2 (16-byte loads); index `it` of lane l at ibase + 8*l + it; the row
+ *     it%2 (16-byte loads); index `it` of lane l at ibase + 8*l + it, except
+ *     in a block-phase record (flag BLOCK) with 4-byte indices, whose lane
+ *     reads consecutive scratch rows: lane l has one word at ent_off slot
+ *     ibase + l, base | len << 28, and entry `it` is base + it*G for it < len,
+ *     else 0; the row
  *     of lane l at slice_rows[rowbase + l/G]. An index addresses the column's
  *     face store in doubles: z at its position, the stage scratch t at
  *     n_faces + stage row. Wave w of the workgroup runs records solve_wave_ptr[w] ..

@@ -329,6 +329,21 @@ __device__ __forceinline__ uint32_t slice_index(const SliceRegs<GLOBAL>& r, int 
     return (r.idx[i / 2] >> (16 * (i % 2))) & 0xffffu;
 }
 
+// Face-store index of entry i of a record. In a global store a block-phase
+// record's lane reads consecutive scratch rows: its word (idx[0]) is
+// base | len << 28, and entry i is base + i*G for i < len, else 0 (a padding
+// entry: its value is zero). LDS stores keep explicit indices: there the
+// decode costs more than the index bytes it saves.
+template <bool GLOBAL>
+__device__ __forceinline__ uint32_t record_index(const SliceRegs<GLOBAL>& r, int32_t flags, int i) {
+  if (GLOBAL && (flags & JJA_SLICE_BLOCK)) {
+    const uint32_t w = r.idx[0];
+    const uint32_t G = static_cast<uint32_t>(flags & 0xff);
+    return static_cast<uint32_t>(i) < (w >> 28) ? (w & 0x0FFFFFFFu) + static_cast<uint32_t>(i) * G : 0u;
+  }
+  return slice_index(r, i);
+}
+
 // Issue the loads of one solve record: always the same vector loads, each a
 // scalar base (from the record) plus a per-lane offset that does not change
 // between records, so no address arithmetic runs per record.
@@ -356,7 +371,10 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, __amdgpu_buffer_
     r.v[2 * q] = d.x;
     r.v[2 * q + 1] = d.y;
   }
-  if constexpr (GLOBAL) {
+  if (GLOBAL && (m.flags & JJA_SLICE_BLOCK)) {
+    // implicit indices: one word per lane
+    r.idx[0] = gbl(static_cast<const uint32_t*>(P.ent_off) + m.ibase)[lane];
+  } else if constexpr (GLOBAL) {
     const GAS vuint4* __restrict__ op =
         gbl(reinterpret_cast<const vuint4*>(static_cast<const uint32_t*>(P.ent_off) + m.ibase)) + 2 * lane;
 #pragma unroll
@@ -384,7 +402,7 @@ __device__ __forceinline__ void process_slice(const Z& zs, int32_t Nf, int lane,
   // their gathers read one broadcast address and contribute exact zeros
   double zv[kMaxIter];
 #pragma unroll
-  for (int i = 0; i < kMaxIter; ++i) zv[i] = zs.gather(slice_index(r, i));
+  for (int i = 0; i < kMaxIter; ++i) zv[i] = zs.gather(record_index(r, m.flags, i));
 #pragma unroll
   for (int i = 0; i < kMaxIter; ++i) acc = __builtin_fma(r.v[i], zv[i], acc);
   if (m.flags & JJA_SLICE_FINISH) {
@@ -456,7 +474,7 @@ __device__ __forceinline__ void process_slice_pair(const Z& za, const Z& zb, int
   double va[kMaxIter], vb[kMaxIter];
 #pragma unroll
   for (int i = 0; i < kMaxIter; ++i) {
-    const uint32_t ix = slice_index(r, i);
+    const uint32_t ix = record_index(r, m.flags, i);
     va[i] = za.gather(ix);
     vb[i] = zb.gather(ix);
   }
@@ -1393,10 +1411,20 @@ int validate(const jja_problem_desc& d) {
         if (r < 0 || r >= Nf || (!block && (r - m.r0 < 0 || r - m.r0 >= ms)))
           return fail(JJA_EINVAL, "solve slice row out of range");
       }
-      for (int64_t e = m.ibase; e < m.ibase + kMaxIter * kWave; ++e) {
-        const int64_t ix = d.index_bytes == 2 ? static_cast<const uint16_t*>(d.ent_off)[e]
-                                              : static_cast<const uint32_t*>(d.ent_off)[e];
-        if (ix >= lds_end) return fail(JJA_EINVAL, "solve LDS offset out of range");
+      if (block && d.index_bytes == 4) {
+        // one word per lane: base | len << 28; entries base + i*G, i < len
+        for (int32_t l = 0; l < kWave; ++l) {
+          const uint32_t wd = static_cast<const uint32_t*>(d.ent_off)[m.ibase + l];
+          const int64_t len = wd >> 28, base = wd & 0x0FFFFFFFu;
+          if (len > kMaxIter || (len > 0 && base + (len - 1) * G >= lds_end))
+            return fail(JJA_EINVAL, "solve LDS offset out of range");
+        }
+      } else {
+        for (int64_t e = m.ibase; e < m.ibase + kMaxIter * kWave; ++e) {
+          const int64_t ix = d.index_bytes == 2 ? static_cast<const uint16_t*>(d.ent_off)[e]
+                                                : static_cast<const uint32_t*>(d.ent_off)[e];
+          if (ix >= lds_end) return fail(JJA_EINVAL, "solve LDS offset out of range");
+        }
       }
     }
     if (barriers < 0) barriers = nb;

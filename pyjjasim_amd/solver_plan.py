@@ -696,11 +696,14 @@ class CircuitPlan:
                 rec = rec0[chain_of[who]] + k // MAX_ITER
                 ln = slot[who] * g + e % g
                 v[rec, k % MAX_ITER, ln] = val[src]
-                o[rec, k % MAX_ITER, ln] = idx[src]
-                # record layouts: value it of lane l at (it/2)*128 + 2l + it%2,
-                # index it of lane l at 8l + it
+                # record layouts: value it of lane l at (it/2)*128 + 2l + it%2
                 vals.append(v.reshape(-1, MAX_ITER // 2, 2, WAVE).transpose(0, 1, 3, 2).reshape(-1))
-                offs.append(o.transpose(0, 2, 1).reshape(-1))
+                if kflag & _native.SLICE_BLOCK and self.index_bytes == 4:
+                    offs.append(self._block_index_words(rec, k % MAX_ITER, ln, idx[src], g, v.shape[0]))
+                else:
+                    # index it of lane l at 8l + it
+                    o[rec, k % MAX_ITER, ln] = idx[src]
+                    offs.append(o.transpose(0, 2, 1).reshape(-1))
                 rws = -np.ones(n_ch * R, dtype=np.int64)
                 rws[chain_of * R + slot] = rows[sel]
                 rows_out.append(rws)
@@ -766,6 +769,39 @@ class CircuitPlan:
         self.ent_off = np.concatenate(offs).astype(np.uint32 if self.global_store else np.uint16)
         self.slice_rows = np.concatenate(rows_out).astype(np.int32)
         self.n_slices = len(solve)
+
+    def _block_index_words(self, rec, it, lane, ix, g, n_rec):
+        """Index data of block-phase records of a global face store. A lane's
+        entries in one record address consecutive scratch rows, so its 8
+        indices are implicit: one word per lane (record slot l), base | len <<
+        28, and entry it reads base + it*g for it < len, else index 0 (a
+        position the face phase always writes; the value there is zero). The
+        kernel loads 256 bytes of indices instead of 2 KB. LDS stores keep
+        explicit indices (there the decode cost more than it saved)."""
+        base = np.zeros((n_rec, WAVE), dtype=np.int64)
+        cnt = np.zeros((n_rec, WAVE), dtype=np.int64)
+        first = it == 0
+        base[rec[first], lane[first]] = ix[first]
+        np.add.at(cnt, (rec, lane), 1)
+        if not np.array_equal(ix, base[rec, lane] + it * g) or cnt.max(initial=0) > MAX_ITER:
+            raise AssertionError("block-phase entries of a lane are not consecutive scratch rows")
+        if base.max(initial=0) >= 1 << 28:
+            raise ValueError("face store too large for implicit block indices (2^28 doubles)")
+        out = np.zeros((n_rec, MAX_ITER * WAVE), dtype=np.int64)
+        out[:, :WAVE] = base | (cnt << 28)
+        return out.reshape(-1)
+
+    def record_indices(self, ibase, flags):
+        """Face-store index of entry it of every lane of a solve record, (8, 64):
+        read from the index array, or decoded from the lane words of a
+        global-store block-phase record (_block_index_words)."""
+        lane = np.arange(WAVE)
+        if not (flags & _native.SLICE_BLOCK and self.index_bytes == 4):
+            return np.stack([self.ent_off[ibase + 8 * lane + it].astype(np.int64) for it in range(MAX_ITER)])
+        words = self.ent_off[ibase + lane].astype(np.int64)
+        base, cnt = words & ((1 << 28) - 1), words >> 28
+        it = np.arange(MAX_ITER)[:, None]
+        return np.where(it < cnt[None, :], base[None, :] + it * (flags & 0xFF), 0)
 
     def wave_program(self, w):
         """Executed records of wave w (the four lookahead records stripped)."""
@@ -887,9 +923,10 @@ class CircuitPlan:
                 acc = np.zeros(WAVE)
                 for (vbase, ibase, rowbase, r0, n_it, flags) in phases[w][ph]:
                     G = flags & 0xFF
+                    offs = self.record_indices(ibase, flags)
                     for it in range(MAX_ITER):
                         v = self.ent_val[vbase + (it // 2) * 128 + 2 * lane + it % 2]
-                        off = self.ent_off[ibase + 8 * lane + it].astype(np.int64)
+                        off = offs[it]
                         rmask[off[v != 0]] = True
                         acc = acc + v * mem[off]
                     if flags & _native.SLICE_FINISH:

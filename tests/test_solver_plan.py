@@ -135,8 +135,15 @@ def test_native_validation_rejects_bad_programs():
     def fresh():
         return CircuitPlan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128)
     plan = fresh()
-    k = int(plan.wave_ptr[0])
+    k = next(i for i in range(len(plan.solve_slices)) if plan.solve_slices[i][4] > 0
+             and not plan.solve_slices[i][5] & _native.SLICE_BLOCK)
     plan.ent_off[plan.solve_slices[k][1] + 3] = plan.Nf + plan.scratch_rows
+    assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
+    assert b"offset" in lib.jja_last_error()
+    # a global store's block-phase lane word (base | len << 28) past the store
+    plan = CircuitPlan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128, face_store="global")
+    k = next(i for i in range(len(plan.solve_slices)) if plan.solve_slices[i][5] & _native.SLICE_BLOCK)
+    plan.ent_off[plan.solve_slices[k][1]] = (plan.Nf + plan.scratch_rows) | (1 << 28)
     assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
     assert b"offset" in lib.jja_last_error()
     plan = fresh()
