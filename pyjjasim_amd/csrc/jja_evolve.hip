@@ -909,12 +909,13 @@ __device__ __forceinline__ void evolve_steps_pair(const DevProblem& P, const jja
   const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
   const int64_t wb = wa + 1;
   const size_t ca = static_cast<size_t>(wa) * P.Nj, cb = two ? static_cast<size_t>(wb) * P.Nj : ca;
-  GAS double* __restrict__ tha = gblw(R.theta) + ca;
-  GAS double* __restrict__ thb = gblw(R.theta) + cb;
-  GAS double* __restrict__ thsa = R.theta_s ? gblw(R.theta_s) + ca : nullptr;
-  GAS double* __restrict__ thsb = R.theta_s ? gblw(R.theta_s) + cb : nullptr;
-  GAS double* __restrict__ ua = gblw(R.u) + ca;
-  GAS double* __restrict__ ub = gblw(R.u) + cb;
+  // column b's arrays alias column a's when it is alone (and are then unused)
+  GAS double* tha = gblw(R.theta) + ca;
+  GAS double* thb = gblw(R.theta) + cb;
+  GAS double* thsa = R.theta_s ? gblw(R.theta_s) + ca : nullptr;
+  GAS double* thsb = R.theta_s ? gblw(R.theta_s) + cb : nullptr;
+  GAS double* ua = gblw(R.u) + ca;
+  GAS double* ub = gblw(R.u) + cb;
   const uint32_t na = static_cast<uint32_t>(R.column_offset + wa), nb = static_cast<uint32_t>(R.column_offset + wb);
 
   junction_pass<BS, SW>(P, R, tha, thsa, ua, za, wa, slot_stride, step0, false, step0, true, na);
@@ -1140,8 +1141,9 @@ __device__ __forceinline__ void adjoint_steps_pair(const DevProblem& P, const jj
                                                    int64_t k_hi, int64_t k_lo) {
   const int64_t wb = two ? wa + 1 : wa;
   const AdjCol Ca = adjoint_column(P, A, wa), Cb = adjoint_column(P, A, wb);
-  GAS double* __restrict__ gfa = gblw(A.grad_f) + static_cast<size_t>(wa) * P.Nf;
-  GAS double* __restrict__ gfb = gblw(A.grad_f) + static_cast<size_t>(wb) * P.Nf;
+  // column b's gradient aliases column a's when it is alone (and is then unused)
+  GAS double* gfa = gblw(A.grad_f) + static_cast<size_t>(wa) * P.Nf;
+  GAS double* gfb = gblw(A.grad_f) + static_cast<size_t>(wb) * P.Nf;
   if (A.prime) {
     adjoint_prime(P, A, Ca, za, k_hi);
     if (two) adjoint_prime(P, A, Cb, zb, k_hi);
