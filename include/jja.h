@@ -251,7 +251,9 @@ typedef struct jja_problem_run {
  * workgroup of block_size/parts threads (parts workgroups, for a split
  * problem) integrates one column for all n_steps steps, largest problems
  * first. A plan with any split problem launches cooperatively; check
- * jja_plan_status after its runs.
+ * jja_plan_status after its runs. Steps step0 .. step0 + n_steps - 1 must lie
+ * within what every run's sources, noise and slot maps cover: those arrays
+ * carry no length here, so the caller keeps launches inside the horizon.
  */
 int jja_evolve(jja_plan* plan, const jja_problem_run* runs, int32_t n_steps, int64_t step0,
                void* stream);

@@ -161,7 +161,10 @@ class TimeEvolutionBatch:
         returns (block_size, parts, hosts, record)."""
         torch = _torch()
         base = build_plans(self.problems, self.block_sizes[0], self.workers)
-        steps = min(self.tune_steps, self.problems[0]._Nt())
+        # a warm-up step and the timed steps, within the problems' horizon
+        steps = min(self.tune_steps, self.problems[0]._Nt() - 1)
+        if steps < 1:
+            return self.block_sizes[0], 1, base, {}
         record = {}
         best = None
         for bs, parts in self._candidates():
@@ -183,8 +186,12 @@ class TimeEvolutionBatch:
 
     def _launch_length(self, eng):
         """Shortest launch length whose measured fixed cost per launch stays below
-        1% of the launch (at least 100 steps, at most 4000)."""
+        1% of the launch (at least 100 steps, at most 4000). The measurement takes
+        24 steps; a shorter horizon runs in one launch."""
+        if eng.Nt < 24:
+            return max(1, eng.Nt)
         torch = _torch()
+        eng.launch(1)                 # the engine's first launch also sets it up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(3):

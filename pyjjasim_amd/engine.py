@@ -268,7 +268,10 @@ class Engine:
         return run
 
     def launch(self, n_steps, stream=None):
-        """One persistent launch of n_steps steps starting at the current step."""
+        """One persistent launch of n_steps steps starting at the current step
+        (within time_step_count: the sources and output-slot maps end there)."""
+        if n_steps < 0 or self.t + n_steps > self.Nt:
+            raise ValueError("launch beyond time_step_count")
         torch = self.torch
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
@@ -385,6 +388,8 @@ class BatchEngine:
         self.host_chunks = any(e.host_chunks for e in self.engines)
 
     def launch(self, n_steps, stream=None):
+        if n_steps < 0 or self.t + n_steps > self.Nt:
+            raise ValueError("launch beyond time_step_count")
         torch = self.torch
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream

@@ -94,3 +94,26 @@ def test_gpu_batch_balanced_parts_match_individual_runs():
     batch = tb.compute()
     for p, rb in zip(probs, batch):
         _same(rb, p.compute())
+
+
+def test_gpu_batch_autotune_short_horizon():
+    """Autotuning a batch whose horizon is shorter than the tuning runs keeps
+    every trial launch within time_step_count and gives each problem's own
+    results; a launch past the horizon is refused."""
+    print("[batch] short-horizon autotune", flush=True)
+    probs = []
+    for n, f in ((6, 0.1), (8, 0.2)):
+        c = pa.SquareArray(n, n)
+        Is = c.current_base(0.0)[:, None, None] * np.array([0.5, 1.1])[None, :, None]
+        probs.append(pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=12, external_flux=f,
+                                             current_sources=Is, temperature=0.01, seed=4))
+    tb = pa.TimeEvolutionBatch(probs, autotune=True, block_sizes=(64, 128), tune_steps=40)
+    for r, p in zip(tb.compute(), probs):
+        _same(r, p.compute())
+    eng = tb.make_engine()
+    try:
+        eng.advance(10)
+        with pytest.raises(ValueError, match="beyond"):
+            eng.launch(3)
+    finally:
+        eng.close()
