@@ -98,11 +98,15 @@ const char* jja_factor_last_error(void);
  *     block     (flag BLOCK set):   z[row]      = sum val * t[idx]
  * where the block entries are rows (forward) or columns (backward) of the
  * inverse of the row's dense diagonal block and idx is a stage-local row.
- * Flag BARRIER ends a phase: the workgroup synchronises after the slice.
+ * Flag BARRIER ends a phase: the workgroup synchronises after the slice (all
+ * parts of a split column: a column barrier). Flag WG_BARRIER ends a phase of
+ * a split column whose rows are read and written by one workgroup only: that
+ * workgroup synchronises. Every wave meets the same barriers in the same order.
  */
 #define JJA_SLICE_BLOCK 0x100
 #define JJA_SLICE_FINISH 0x200
 #define JJA_SLICE_BARRIER 0x400
+#define JJA_SLICE_WG_BARRIER 0x800
 
 typedef struct jja_slice {
   int64_t base;    /* first value (face slices: first code) */

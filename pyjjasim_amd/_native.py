@@ -32,6 +32,7 @@ class Slice(C.Structure):
 SLICE_BLOCK = 0x100
 SLICE_FINISH = 0x200
 SLICE_BARRIER = 0x400
+SLICE_WG_BARRIER = 0x800
 
 
 class ProblemDesc(C.Structure):

@@ -221,6 +221,7 @@ struct FaceStore<false> {
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
   // orders LDS traffic only: global loads in flight stay in flight
   __device__ __forceinline__ void barrier() const { lds_barrier(); }
+  __device__ __forceinline__ void wg_barrier() const { lds_barrier(); }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -233,6 +234,7 @@ struct FaceStore<true> {
   __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
   __device__ __forceinline__ void barrier() const { __syncthreads(); }
+  __device__ __forceinline__ void wg_barrier() const { __syncthreads(); }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -310,6 +312,9 @@ struct SplitStore {
     __syncthreads();
   }
   __device__ __forceinline__ void barrier() const { sync(); }
+  // the waves of this workgroup only (a part-local phase: its rows' readers
+  // and writers are all in this workgroup)
+  __device__ __forceinline__ void wg_barrier() const { __syncthreads(); }
 };
 
 // Entries of one solve record for one lane, held in registers: 8 values and
@@ -417,7 +422,10 @@ __device__ __forceinline__ void process_slice(const Z& zs, int32_t Nf, int lane,
     }
     acc = 0.0;
   }
-  if (m.flags & JJA_SLICE_BARRIER) zs.barrier();
+  if (m.flags & JJA_SLICE_BARRIER)
+    zs.barrier();
+  else if (m.flags & JJA_SLICE_WG_BARRIER)
+    zs.wg_barrier();
 }
 
 // The whole forward + backward solve. Each wave walks its own record program,
@@ -500,7 +508,10 @@ __device__ __forceinline__ void process_slice_pair(const Z& za, const Z& zb, int
     acca = 0.0;
     accb = 0.0;
   }
-  if (m.flags & JJA_SLICE_BARRIER) za.barrier();
+  if (m.flags & JJA_SLICE_BARRIER)
+    za.barrier();
+  else if (m.flags & JJA_SLICE_WG_BARRIER)
+    za.wg_barrier();
 }
 
 // solve() for two columns that share one set of threads and one barrier (za's).
@@ -1390,22 +1401,29 @@ int validate(const jja_problem_desc& d) {
     return fail(JJA_EINVAL, "face_copy_rows out of range");
   const int64_t lds_end = static_cast<int64_t>(Nf) + ms;
   int32_t barriers = -1;
+  std::string seq0;  // column ('c') and workgroup ('w') barriers of wave 0, in order
   for (int32_t w = 0; w < NW; ++w) {
     if (wp[w + 1] - wp[w] < 4 || (wp[w + 1] - wp[w] - 4) % 3)
       return fail(JJA_EINVAL, "a wave program needs a record count divisible by 3 and 4 lookahead records");
     int32_t nb = 0;
+    std::string seq;
     for (int32_t k = wp[w]; k < wp[w + 1]; ++k) {
       const jja_slice& m = d.solve_slices[k];
       const int32_t G = m.flags & 0xff, rows = m.flags >> 16;
       if (!pow2_le(G, kWave)) return fail(JJA_EINVAL, "solve record lanes per row must be a power of two <= 64");
-      if (m.n_iter < 0 || m.n_iter > kMaxIter || rows < 0 || rows > kWave / G || (m.flags & 0xf800))
+      if (m.n_iter < 0 || m.n_iter > kMaxIter || rows < 0 || rows > kWave / G || (m.flags & 0xf000))
         return fail(JJA_EINVAL, "solve record fields out of range");
       if (m.base < 0 || (m.base % 2) || m.base + kMaxIter * kWave > d.n_entries || m.ibase < 0 ||
           (m.ibase % (16 / d.index_bytes)) || m.ibase + kMaxIter * kWave > d.n_index || m.rowbase < 0 ||
           m.rowbase + kWave / G > d.n_slice_rows)
         return fail(JJA_EINVAL, "solve record data out of range or misaligned");
       if (k >= wp[w + 1] - 4) continue;  // lookahead only: loaded, never executed
-      if (m.flags & JJA_SLICE_BARRIER) ++nb;
+      if (m.flags & JJA_SLICE_BARRIER) {
+        ++nb;
+        seq.push_back('c');
+      } else if (m.flags & JJA_SLICE_WG_BARRIER) {
+        seq.push_back('w');
+      }
       const bool block = m.flags & JJA_SLICE_BLOCK;
       for (int32_t i = 0; i < rows; ++i) {
         const int32_t r = d.slice_rows[m.rowbase + i];
@@ -1431,6 +1449,12 @@ int validate(const jja_problem_desc& d) {
     }
     if (barriers < 0) barriers = nb;
     if (nb != barriers) return fail(JJA_EINVAL, "wave programs disagree on their barrier count");
+    // every wave meets the same barriers in the same order (a workgroup
+    // barrier is a __syncthreads of its workgroup, and so is part of a column one)
+    if (w == 0)
+      seq0 = seq;
+    else if (seq != seq0)
+      return fail(JJA_EINVAL, "wave programs disagree on their barrier sequence");
   }
   return JJA_OK;
 }

@@ -484,6 +484,72 @@ class CircuitPlan:
         """Owner wave of every row of the lowest n_low stages (None for none)."""
         return self._subtree_waves(NW, n_low)[1] if n_low else None
 
+    @staticmethod
+    def _spread(weights, n):
+        """Items heaviest first, n at a time, onto the least-loaded bins."""
+        order = np.lexsort((np.arange(len(weights)), -weights))
+        loads = np.zeros(n)
+        bin_of = np.zeros(len(weights), dtype=np.int64)
+        for i in range(0, len(order), n):
+            chunk = order[i:i + n]
+            bins = np.argsort(loads, kind="stable")[:len(chunk)]
+            bin_of[chunk] = bins
+            loads[bins] += weights[chunk]
+        return loads, bin_of
+
+    def _choose_part_local(self, NW, n_wave):
+        """Stage count below which whole subtrees go to one workgroup (part) of
+        a split column, so that those stages' phases end in workgroup barriers
+        instead of column barriers: the one with the lowest estimated cost (the
+        slowest part's excess over an even share, per wave, minus 4 column
+        barriers per stage beyond the wave-local ones). 0 for unsplit columns."""
+        if self.parts == 1 or self.local_option == 0:
+            return 0
+        sp = self.solve_plan
+        if not hasattr(sp, "_row_work"):
+            sp._row_work = sp.row_work()
+        per_part = NW // self.parts
+        B = self.COLUMN_BARRIER_ENTRIES
+        best, best_cost = 0, 0.0
+        for k in range(n_wave + 1, sp.n_stages + 1):
+            label = sp.subtrees(k)
+            gw = np.bincount(label, weights=sp._row_work[:len(label)])
+            if len(gw) < self.parts:
+                break
+            load, _ = self._spread(gw, self.parts)
+            cost = (load.max() - load.sum() / self.parts) / per_part - B * 4 * (k - n_wave)
+            if cost < best_cost:
+                best, best_cost = k, cost
+        return best
+
+    def _owners(self, NW, n_wave, n_part):
+        """Owner wave of every row of the lowest n_wave stages and owner part of
+        every row of the lowest n_part stages (None where none). Subtrees at
+        level n_part go to parts; within a part, its subtrees at level n_wave go
+        to the part's waves, so a wave's rows lie inside its part's."""
+        if not n_part:
+            return self._wave_owner(NW, n_wave), None
+        sp = self.solve_plan
+        if not hasattr(sp, "_row_work"):
+            sp._row_work = sp.row_work()
+        per_part = NW // self.parts
+        label2 = sp.subtrees(n_part)
+        gw2 = np.bincount(label2, weights=sp._row_work[:len(label2)])
+        part_of = self._spread(gw2, self.parts)[1][label2]
+        if not n_wave:
+            return None, part_of
+        label1 = sp.subtrees(n_wave)
+        R1 = len(label1)
+        gw1 = np.bincount(label1, weights=sp._row_work[:R1])
+        grp_part = np.zeros(len(gw1), dtype=np.int64)
+        grp_part[label1] = part_of[:R1]
+        wave_of_grp = np.zeros(len(gw1), dtype=np.int64)
+        for q in range(self.parts):
+            gs = np.nonzero(grp_part == q)[0]
+            if len(gs):
+                wave_of_grp[gs] = q * per_part + self._spread(gw1[gs], per_part)[1]
+        return wave_of_grp[label1], part_of
+
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.column_threads, self.solve_plan, self.Nf, self.Nj
         if Nf == 0:
@@ -505,8 +571,9 @@ class CircuitPlan:
         if fixed is None:
             fixed = self._choose_local(NW, None if self.global_store else lds_rows)
         self.local_stages = fixed
-        self.scratch_rows = max(sp.max_stage_rows, int(sp.stage_ptr[fixed]))
-        owner = self._wave_owner(NW, fixed)
+        self.part_stages = self._choose_part_local(NW, fixed)
+        self.scratch_rows = max(sp.max_stage_rows, int(sp.stage_ptr[max(fixed, self.part_stages)]))
+        owner, part_owner = self._owners(NW, fixed, self.part_stages)
         pos = sp.face_pos.astype(np.int64)
         # junctions: face positions of the (up to two) faces, ascending face index
         jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
@@ -554,11 +621,12 @@ class CircuitPlan:
         rb, bsz, boff = sp.row_block.astype(np.int64), sp.row_bsize.astype(np.int64), sp.row_boff
         per_stage = []
         n_loc = self.local_stages
+        n_part = self.part_stages
         for s in range(sp.n_stages):
             r0, r1 = int(sp.stage_ptr[s]), int(sp.stage_ptr[s + 1])
             # scratch row of row r: t[r - t0]; wave-local stages keep all their
             # rows' scratch at once (t0 = 0), the others reuse it per stage
-            t0 = 0 if s < n_loc else r0
+            t0 = 0 if s < max(n_loc, n_part) else r0
             rows = np.arange(r0, r1)
             a = rows - rb[rows]
             B = bsz[rows]
@@ -597,10 +665,23 @@ class CircuitPlan:
                 continue
             # LDS indices in doubles: z at its position, stage scratch t at Nf + row - t0
             lds_base = Nf if kind % 2 else 0
-            local = s < n_loc
-            end = local and (s, kind) in ((n_loc - 1, 1), (0, 3))
+            if s < n_loc:
+                # a wave-local run ends at the last forward stage (then part-local
+                # stages, if any, follow within each workgroup) and at the end of
+                # the solve
+                mode, own = "wave", owner[rows]
+                if (s, kind) == (n_loc - 1, 1):
+                    end = "wg" if n_part > n_loc else "col"
+                else:
+                    end = "col" if (s, kind) == (0, 3) else None
+            elif s < n_part:
+                mode, own = "part", part_owner[rows]
+                last = (s, kind) == (n_part - 1, 1) or ((s, kind) == (0, 3))
+                end = "col" if last else "wg"
+            else:
+                mode, own, end = "col", None, "col"
             phases.append((t0, rows, pptr, lds_base + np.asarray(pidx, dtype=np.int64), pval,
-                           _native.SLICE_BLOCK if kind % 2 else 0, owner[rows] if local else None, end))
+                           _native.SLICE_BLOCK if kind % 2 else 0, own, mode, end))
         self._build_wave_program(phases, target_iter)
         self.face_slices = [(b, 0, rb, r0, n, fl) for (b, rb, r0, n, fl) in self.face_slices]
         # face rows follow the solve rows (record row bases were taken before)
@@ -625,7 +706,7 @@ class CircuitPlan:
         self.ent_val = np.zeros(0)
         self.ent_off = np.zeros(0, np.uint32 if self.global_store else np.uint16)
         self.face_copy_rows = 0
-        self.local_stages = self.scratch_rows = 0
+        self.local_stages = self.part_stages = self.scratch_rows = 0
         self.n_slices = self.n_solve_exec = 0
         self.wave_ptr = np.zeros(self.column_threads // WAVE + 1, dtype=np.int64)
         self._face_rec = self._slice_array([])
@@ -650,7 +731,10 @@ class CircuitPlan:
         of one owner's rows, each chain goes to its owner, and no barrier follows
         it: the owner's later records see its writes because one wave's LDS and
         vector-memory accesses are performed in order. The last phase of a run
-        of wave-local phases ends with a barrier in every wave.
+        of wave-local phases ends with a barrier in every wave. A phase with an
+        owner part per row (a part-local stage of a split column) spreads each
+        part's chains over that part's waves and ends with a workgroup barrier
+        (WG_BARRIER); a column barrier (BARRIER) closes the run.
         Record data: value `it` of lane l at base + (it/2)*128 + 2l + it%2, LDS
         index `it` (in doubles) of lane l at ibase + 8l + it, row of lane l at
         rowbase + l/G. Block 0 of every array is zero (the no-op record); record
@@ -664,7 +748,8 @@ class CircuitPlan:
         noop = (0, 0, 0, 0, 0, 1)
         progs = [[] for _ in range(NW)]
         region = [None] * NW    # where each wave's current wave-local run began
-        for (r0, rows, ptr, idx, val, kflag, owner, end) in phases:
+        per_part = NW // self.parts
+        for (r0, rows, ptr, idx, val, kflag, owner, mode, end) in phases:
             ptr = np.asarray(ptr, dtype=np.int64)
             rows = np.asarray(rows, dtype=np.int64)
             lens = np.diff(ptr)
@@ -726,16 +811,21 @@ class CircuitPlan:
                 nr += n_ch * R
             order = sorted(range(len(chains)), key=lambda c: -chains[c][2])
             assign = [[] for _ in range(NW)]
-            if owner is None:
-                # least-loaded wave first, the lowest index among equals
-                heap = [(0, w) for w in range(NW)]
-                for c in order:
-                    load, w = heapq.heappop(heap)
-                    assign[w].append(c)
-                    heapq.heappush(heap, (load + chains[c][2], w))
-            else:
+            if mode == "wave":
                 for c in order:
                     assign[chains[c][6]].append(c)
+            else:
+                # least-loaded wave first, the lowest index among equals: over all
+                # waves, or over the waves of the chain's owner part
+                groups = [range(NW)] if mode == "col" else \
+                    [range(q * per_part, (q + 1) * per_part) for q in range(self.parts)]
+                heaps = [[(0, w) for w in ws] for ws in groups]
+                for c in order:
+                    h = heaps[0 if mode == "col" else chains[c][6]]
+                    load, w = heapq.heappop(h)
+                    assign[w].append(c)
+                    heapq.heappush(h, (load + chains[c][2], w))
+            bar = _native.SLICE_BARRIER if end == "col" else _native.SLICE_WG_BARRIER
             for w in range(NW):
                 recs = []
                 for c in assign[w]:
@@ -746,7 +836,7 @@ class CircuitPlan:
                             flags |= _native.SLICE_FINISH
                         b = 512 * (first + ch)
                         recs.append((b, b, rowbase, r0, min(MAX_ITER, n_it - ch * MAX_ITER), flags))
-                if owner is not None:
+                if mode == "wave":
                     # wave-local: the wave's records follow each other with no
                     # barrier until the run of wave-local phases ends
                     if region[w] is None:
@@ -761,7 +851,7 @@ class CircuitPlan:
                 elif not recs:
                     recs.append(noop)
                 b_, ib_, rb_, r0_, it_, fl_ = recs[-1]
-                recs[-1] = (b_, ib_, rb_, r0_, it_, fl_ | _native.SLICE_BARRIER)
+                recs[-1] = (b_, ib_, rb_, r0_, it_, fl_ | bar)
                 progs[w] += recs
         assert all(r is None for r in region), "a wave-local run without its closing barrier"
         solve, wave_ptr, n_exec = [], [], 0
@@ -889,8 +979,10 @@ class CircuitPlan:
         Between two barriers each wave runs its records in order and sees its own
         writes at once; any face-store element that one wave writes and another
         wave reads or writes between the same two barriers raises AssertionError
-        (a race on the GPU). Reads with a zero factor value are left out: they
-        add an exact zero whatever they read."""
+        (a race on the GPU). A workgroup barrier (WG_BARRIER) joins only the
+        waves of one part: within a column phase, an element that one part writes
+        and another part touches raises too. Reads with a zero factor value are
+        left out: they add an exact zero whatever they read."""
         BS, Nf = self.column_threads, self.Nf
         zs = np.zeros(Nf + max(self.scratch_rows, 1))
         tids = np.arange(BS)
@@ -907,60 +999,89 @@ class CircuitPlan:
                 cp = ok & (pos < self.face_copy_rows)
                 zs[Nf + pos[cp]] = acc[cp]
                 acc = np.zeros(BS)
-        # split every wave's program at its barriers; phase p of all waves runs
-        # before phase p+1 (waves meet at each barrier)
+        # split every wave's program at its column barriers into phases, and a
+        # phase at its workgroup barriers into sub-phases: phase p of all waves
+        # runs before phase p+1; within a phase, the waves of one part meet at
+        # each workgroup barrier, and different parts never meet
         NW = BS // WAVE
+        per_part = NW // self.parts
+        WGB, COLB = _native.SLICE_WG_BARRIER, _native.SLICE_BARRIER
         phases = []
         for w in range(NW):
-            cur, mine = [], []
+            cur, sub, mine = [], [], []
             for rec in self.wave_program(w):
                 cur.append(rec)
-                if rec[5] & _native.SLICE_BARRIER:
-                    mine.append(cur)
+                if rec[5] & (WGB | COLB):
+                    sub.append(cur)
                     cur = []
+                    if rec[5] & COLB:
+                        mine.append(sub)
+                        sub = []
+            assert not sub, "work after the last barrier"
             assert not cur or all(r[5] & 0xFFFF0000 == 0 for r in cur), "work after the last barrier"
             phases.append(mine)
         n_ph = len(phases[0])
         assert all(len(p) == n_ph for p in phases), "waves disagree on the barrier count"
         lane = np.arange(WAVE)
         n = len(zs)
-        for ph in range(n_ph):
-            views, wrote, read = [], [], []
-            for w in range(NW):
-                mem = zs.copy()
-                wmask = np.zeros(n, dtype=bool)
-                rmask = np.zeros(n, dtype=bool)
-                acc = np.zeros(WAVE)
-                for (vbase, ibase, rowbase, r0, n_it, flags) in phases[w][ph]:
-                    G = flags & 0xFF
-                    offs = self.record_indices(ibase, flags)
-                    for it in range(MAX_ITER):
-                        v = self.ent_val[vbase + (it // 2) * 128 + 2 * lane + it % 2]
-                        off = offs[it]
-                        rmask[off[v != 0]] = True
-                        acc = acc + v * mem[off]
-                    if flags & _native.SLICE_FINISH:
-                        red = acc.reshape(WAVE // G, G).sum(axis=1)
-                        rows = self.slice_rows[rowbase + np.arange(WAVE // G)]
-                        live = np.arange(WAVE // G) < (flags >> 16)
-                        ok = live & (rows >= 0)
-                        rows, red = rows[ok].astype(np.int64), red[ok]
-                        if flags & _native.SLICE_BLOCK:
-                            mem[rows] = red
-                            wmask[rows] = True
-                        else:
-                            rmask[rows] = True
-                            mem[Nf + rows - r0] = mem[rows] - red
-                            wmask[Nf + rows - r0] = True
-                        acc = np.zeros(WAVE)
-                views.append(mem)
-                wrote.append(wmask)
-                read.append(rmask)
+
+        def run_wave(recs, mem):
+            """One wave's records in order; returns its view and write/read masks."""
+            mem = mem.copy()
+            wmask = np.zeros(n, dtype=bool)
+            rmask = np.zeros(n, dtype=bool)
+            acc = np.zeros(WAVE)
+            for (vbase, ibase, rowbase, r0, n_it, flags) in recs:
+                G = flags & 0xFF
+                offs = self.record_indices(ibase, flags)
+                for it in range(MAX_ITER):
+                    v = self.ent_val[vbase + (it // 2) * 128 + 2 * lane + it % 2]
+                    off = offs[it]
+                    rmask[off[v != 0]] = True
+                    acc = acc + v * mem[off]
+                if flags & _native.SLICE_FINISH:
+                    red = acc.reshape(WAVE // G, G).sum(axis=1)
+                    rows = self.slice_rows[rowbase + np.arange(WAVE // G)]
+                    live = np.arange(WAVE // G) < (flags >> 16)
+                    ok = live & (rows >= 0)
+                    rows, red = rows[ok].astype(np.int64), red[ok]
+                    if flags & _native.SLICE_BLOCK:
+                        mem[rows] = red
+                        wmask[rows] = True
+                    else:
+                        rmask[rows] = True
+                        mem[Nf + rows - r0] = mem[rows] - red
+                        wmask[Nf + rows - r0] = True
+                    acc = np.zeros(WAVE)
+            return mem, wmask, rmask
+
+        def check(wrote, read, what):
             writers = np.sum(wrote, axis=0)
-            assert writers.max(initial=0) <= 1, "two waves write one face-store element between barriers"
-            for w in range(NW):
-                assert not np.any((writers - wrote[w])[read[w]]), \
-                    f"wave {w} reads an element another wave writes between the same barriers"
-            for w in range(NW):
-                zs[wrote[w]] = views[w][wrote[w]]
+            assert writers.max(initial=0) <= 1, f"two {what}s write one face-store element between barriers"
+            for i in range(len(wrote)):
+                assert not np.any((writers - wrote[i])[read[i]]), \
+                    f"{what} {i} reads an element another {what} writes between the same barriers"
+
+        for ph in range(n_ph):
+            part_mem, part_w, part_r = [], [], []
+            for q in range(self.parts):
+                waves = range(q * per_part, (q + 1) * per_part)
+                n_sub = len(phases[waves[0]][ph])
+                assert all(len(phases[w][ph]) == n_sub for w in waves), "waves of a part disagree on barriers"
+                mem = zs.copy()
+                tw = np.zeros(n, dtype=bool)
+                tr = np.zeros(n, dtype=bool)
+                for j in range(n_sub):
+                    out = [run_wave(phases[w][ph][j], mem) for w in waves]
+                    check([o[1] for o in out], [o[2] for o in out], "wave")
+                    for view, wm, rm in out:
+                        mem[wm] = view[wm]
+                        tw |= wm
+                        tr |= rm
+                part_mem.append(mem)
+                part_w.append(tw)
+                part_r.append(tr)
+            check(part_w, part_r, "part")
+            for mem, wm in zip(part_mem, part_w):
+                zs[wm] = mem[wm]
         return zs[self.solve_plan.face_pos]

@@ -304,7 +304,8 @@ def test_wave_local_stages_emulation(name, BS, parts, store):
         plan = base.relayout(BS, local_stages=k)
         assert plan.local_stages == min(k, plan.solve_plan.n_stages)
         assert plan.scratch_rows == max(plan.solve_plan.max_stage_rows,
-                                        int(plan.solve_plan.stage_ptr[plan.local_stages]))
+                                        int(plan.solve_plan.stage_ptr[max(plan.local_stages,
+                                                                          plan.part_stages)]))
         assert _native.lib().jja_plan_validate(1, C.byref(plan.desc())) == 0, _native.lib().jja_last_error()
         np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12,
                                    atol=1e-12 * abs(want).max())
@@ -388,3 +389,65 @@ def test_column_pair_plans_validate():
     with pytest.raises(ValueError, match="column pairs"):
         CircuitPlan(c, 0.05, leaf_size=8, block_size=256).paired()
     assert not split.relayout(512, 2).column_pairs
+
+
+def _part_local_plan(cls=CircuitPlan):
+    c = pa.SquareArray(64, 64)
+    plan = cls(c, 0.05, block_size=512, parts=2)
+    assert plan.part_stages > plan.local_stages > 0
+    return c, plan
+
+
+def test_part_local_stages_emulation_and_validation():
+    """A split column whose middle stages are part-local (whole subtrees per
+    workgroup, workgroup barriers between their phases): the plan validates,
+    every wave meets the same barrier sequence, and the emulation (workgroup
+    barriers join only a part's waves) solves K J = A u."""
+    import ctypes as C
+    from pyjjasim_amd import _native
+    c, plan = _part_local_plan()
+    assert _native.lib().jja_plan_validate(1, C.byref(plan.desc())) == 0, _native.lib().jja_last_error()
+    kinds = [[r[5] & (_native.SLICE_BARRIER | _native.SLICE_WG_BARRIER) for r in plan.wave_program(w)
+              if r[5] & (_native.SLICE_BARRIER | _native.SLICE_WG_BARRIER)]
+             for w in range(plan.column_threads // 64)]
+    assert all(k == kinds[0] for k in kinds) and _native.SLICE_WG_BARRIER in kinds[0]
+    u = np.random.default_rng(8).standard_normal(c.junction_count())
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
+    np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
+    # every stage in step: no workgroup barriers, the same solution
+    flat = plan.relayout(512, local_stages=0)
+    assert flat.part_stages == 0
+    np.testing.assert_allclose(flat.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
+
+
+class _MisownedPartPlan(CircuitPlan):
+    """Planted mistake: the part-local rows of the lowest part-local stage that
+    part 0 owns go to part 1, away from their wave-local descendants."""
+
+    def _owners(self, NW, n_wave, n_part):
+        wave_of, part_of = super()._owners(NW, n_wave, n_part)
+        if part_of is not None:
+            r0 = int(self.solve_plan.stage_ptr[n_wave])
+            r1 = int(self.solve_plan.stage_ptr[n_wave + 1])
+            part_of = part_of.copy()
+            sel = np.arange(r0, r1)[part_of[r0:r1] == 0]
+            part_of[sel] = 1
+        return wave_of, part_of
+
+
+def test_part_local_race_is_detected():
+    c, plan = _part_local_plan(_MisownedPartPlan)
+    u = np.random.default_rng(9).standard_normal(c.junction_count())
+    with pytest.raises(AssertionError, match="another (part|wave)"):
+        plan.emulate_face_and_solve(u)
+
+
+def test_native_validation_rejects_barrier_sequences_that_differ():
+    import ctypes as C
+    from pyjjasim_amd import _native
+    _, plan = _part_local_plan()
+    w1 = [i for i in range(int(plan.wave_ptr[1]), int(plan.wave_ptr[2]) - 4)
+          if plan._solve_rec[i].flags & _native.SLICE_WG_BARRIER]
+    rec = plan._solve_rec[w1[0]]
+    rec.flags = (rec.flags & ~_native.SLICE_WG_BARRIER) | _native.SLICE_BARRIER
+    assert _native.lib().jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
