@@ -350,8 +350,11 @@ class CircuitPlan:
     fall into independent subtrees, each wave solves whole subtrees, and waves
     meet once per pass for those stages instead of twice per stage. "auto"
     chooses the count from a cost estimate for split columns and 0 (every stage
-    in step) for one workgroup per column. Results do not depend on it: a
-    row's summation order depends only on the row."""
+    in step) for one workgroup per column. Above them, a split column's stages
+    with at least one subtree per part are part-local (``part_stages``): each
+    workgroup solves its own subtrees between workgroup barriers; 0 turns both
+    off. Results do not depend on either: a row's summation order depends only
+    on the row."""
 
     def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8,
                  face_store="auto", parts=1, local_stages="auto"):

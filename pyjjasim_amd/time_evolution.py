@@ -30,10 +30,12 @@ Extensions (keyword-only, defaults keep reference behaviour):
                    one workgroup per column. Results do not depend on it.
   local_stages     Lowest stages of the factor schedule solved wave-locally (each
                    wave solves whole independent subtrees, with no barrier between
-                   those stages): "auto" (default) chooses by a cost estimate
-                   for columns split over workgroups and 0 otherwise; 0 solves
-                   every stage with all waves in step. Results do not depend
-                   on it.
+                   those stages; for a split column, the stages above them are
+                   solved part-locally, each workgroup with its own subtrees and
+                   workgroup barriers): "auto" (default) chooses by a cost
+                   estimate for columns split over workgroups and 0 otherwise; 0
+                   solves every stage with all waves in step. Results do not
+                   depend on it.
   column_pairs     True: split columns are worked two at a time by one set of
                    workgroups (half the workgroup size, twice the parts), which
                    stream the solve program once for both. Needs a split column
