@@ -153,3 +153,30 @@ def test_gpu_split_adjoint_bit_identical():
     for other in grads[1:]:
         for a, b in zip(grads[0], other):
             np.testing.assert_array_equal(a, b)
+
+
+def test_gpu_part_local_stages_bit_identical():
+    """A split column whose middle factor stages are part-local (each workgroup
+    solves its own subtrees between workgroup barriers) gives the results of
+    one workgroup per column and of every stage solved in step."""
+    from pyjjasim_amd.solver_plan import CircuitPlan
+    c = pa.SquareArray(64, 64)
+    c.set_capacitance(0.9)
+    assert CircuitPlan(c, 0.05, block_size=512, parts=2).part_stages > 0
+    W, Nt = 2, 60
+    Is = c.current_base(0.0)[:, None, None] * np.array([0.6, 1.2])[None, :, None]
+
+    def run(parts, local):
+        prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.13,
+                                       current_sources=Is, temperature=0.02, seed=13,
+                                       store_time_steps=[Nt // 2, Nt - 1], block_size=512,
+                                       face_store="global", parts=parts, steps_per_launch=25,
+                                       local_stages=local)
+        return prob.compute()
+    print("[part-local] one part", flush=True)
+    ref = run(1, 0)
+    for parts, local in ((2, "auto"), (2, 0)):
+        print(f"[part-local] {parts} parts, local_stages={local}", flush=True)
+        r = run(parts, local)
+        np.testing.assert_array_equal(r.get_theta(), ref.get_theta())
+        np.testing.assert_array_equal(r.get_vortex_configuration(), ref.get_vortex_configuration())
