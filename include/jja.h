@@ -107,6 +107,10 @@ const char* jja_factor_last_error(void);
 #define JJA_SLICE_FINISH 0x200
 #define JJA_SLICE_BARRIER 0x400
 #define JJA_SLICE_WG_BARRIER 0x800
+/* an off-block record of a global store (4-byte indices) whose lanes each span
+ * fewer than 2^16 positions: slot ibase + l holds lane l's base, slots ibase +
+ * 64 + 4l + q hold its entries 2q and 2q+1 as 16-bit offsets from the base */
+#define JJA_SLICE_NARROW 0x1000
 
 typedef struct jja_slice {
   int64_t base;    /* first value (face slices: first code) */

@@ -33,6 +33,7 @@ SLICE_BLOCK = 0x100
 SLICE_FINISH = 0x200
 SLICE_BARRIER = 0x400
 SLICE_WG_BARRIER = 0x800
+SLICE_NARROW = 0x1000
 
 
 class ProblemDesc(C.Structure):

@@ -346,6 +346,10 @@ __device__ __forceinline__ uint32_t record_index(const SliceRegs<GLOBAL>& r, int
     const uint32_t G = static_cast<uint32_t>(flags & 0xff);
     return static_cast<uint32_t>(i) < (w >> 28) ? (w & 0x0FFFFFFFu) + static_cast<uint32_t>(i) * G : 0u;
   }
+  // a NARROW off-block record of a global store: lane base (idx[0]) plus a
+  // 16-bit offset (two per word in idx[1..4])
+  if (GLOBAL && (flags & JJA_SLICE_NARROW))
+    return r.idx[0] + ((r.idx[1 + i / 2] >> (16 * (i % 2))) & 0xffffu);
   return slice_index(r, i);
 }
 
@@ -379,6 +383,15 @@ __device__ __forceinline__ void load_slice(const DevProblem& P, __amdgpu_buffer_
   if (GLOBAL && (m.flags & JJA_SLICE_BLOCK)) {
     // implicit indices: one word per lane
     r.idx[0] = gbl(static_cast<const uint32_t*>(P.ent_off) + m.ibase)[lane];
+  } else if (GLOBAL && (m.flags & JJA_SLICE_NARROW)) {
+    // lane base, then 8 16-bit offsets in one 16-byte load
+    const uint32_t* ip = static_cast<const uint32_t*>(P.ent_off) + m.ibase;
+    r.idx[0] = gbl(ip)[lane];
+    const vuint4 o = gbl(reinterpret_cast<const vuint4*>(ip + kWave))[lane];
+    r.idx[1] = o.x;
+    r.idx[2] = o.y;
+    r.idx[3] = o.z;
+    r.idx[4] = o.w;
   } else if constexpr (GLOBAL) {
     const GAS vuint4* __restrict__ op =
         gbl(reinterpret_cast<const vuint4*>(static_cast<const uint32_t*>(P.ent_off) + m.ibase)) + 2 * lane;
@@ -1411,8 +1424,10 @@ int validate(const jja_problem_desc& d) {
       const jja_slice& m = d.solve_slices[k];
       const int32_t G = m.flags & 0xff, rows = m.flags >> 16;
       if (!pow2_le(G, kWave)) return fail(JJA_EINVAL, "solve record lanes per row must be a power of two <= 64");
-      if (m.n_iter < 0 || m.n_iter > kMaxIter || rows < 0 || rows > kWave / G || (m.flags & 0xf000))
+      if (m.n_iter < 0 || m.n_iter > kMaxIter || rows < 0 || rows > kWave / G || (m.flags & 0xe000))
         return fail(JJA_EINVAL, "solve record fields out of range");
+      if ((m.flags & JJA_SLICE_NARROW) && (d.index_bytes != 4 || (m.flags & JJA_SLICE_BLOCK)))
+        return fail(JJA_EINVAL, "NARROW solve records need 4-byte indices and an off-block phase");
       if (m.base < 0 || (m.base % 2) || m.base + kMaxIter * kWave > d.n_entries || m.ibase < 0 ||
           (m.ibase % (16 / d.index_bytes)) || m.ibase + kMaxIter * kWave > d.n_index || m.rowbase < 0 ||
           m.rowbase + kWave / G > d.n_slice_rows)
@@ -1431,7 +1446,18 @@ int validate(const jja_problem_desc& d) {
         if (r < 0 || r >= Nf || (!block && (r - m.r0 < 0 || r - m.r0 >= ms)))
           return fail(JJA_EINVAL, "solve slice row out of range");
       }
-      if (block && d.index_bytes == 4) {
+      if (m.flags & JJA_SLICE_NARROW) {
+        // lane base, then two 16-bit offsets per word
+        const uint32_t* ip = static_cast<const uint32_t*>(d.ent_off) + m.ibase;
+        for (int32_t l = 0; l < kWave; ++l) {
+          uint32_t dmax = 0;
+          for (int32_t q = 0; q < kMaxIter / 2; ++q) {
+            const uint32_t wd = ip[kWave + 4 * l + q];
+            dmax = std::max(dmax, std::max(wd & 0xffffu, wd >> 16));
+          }
+          if (static_cast<int64_t>(ip[l]) + dmax >= lds_end) return fail(JJA_EINVAL, "solve LDS offset out of range");
+        }
+      } else if (block && d.index_bytes == 4) {
         // one word per lane: base | len << 28; entries base + i*G, i < len
         for (int32_t l = 0; l < kWave; ++l) {
           const uint32_t wd = static_cast<const uint32_t*>(d.ent_off)[m.ibase + l];

@@ -746,6 +746,7 @@ class CircuitPlan:
         NW = self.column_threads // WAVE
         vals = [np.zeros(512)]
         offs = [np.zeros(512, dtype=np.int64)]
+        narrow = [np.zeros(1, dtype=bool)]     # per record: 16-bit index deltas
         rows_out = [-np.ones(WAVE, dtype=np.int64)]
         n_rec, nr = 1, WAVE
         noop = (0, 0, 0, 0, 0, 1)
@@ -798,10 +799,17 @@ class CircuitPlan:
                 vals.append(v.reshape(-1, MAX_ITER // 2, 2, WAVE).transpose(0, 1, 3, 2).reshape(-1))
                 if kflag & _native.SLICE_BLOCK and self.index_bytes == 4:
                     offs.append(self._block_index_words(rec, k % MAX_ITER, ln, idx[src], g, v.shape[0]))
+                    narrow.append(np.zeros(v.shape[0], dtype=bool))
                 else:
                     # index it of lane l at 8l + it
                     o[rec, k % MAX_ITER, ln] = idx[src]
-                    offs.append(o.transpose(0, 2, 1).reshape(-1))
+                    if self.index_bytes == 4:
+                        words, nar = self._narrow_index_words(o, rec, k % MAX_ITER, ln)
+                        offs.append(words)
+                        narrow.append(nar)
+                    else:
+                        offs.append(o.transpose(0, 2, 1).reshape(-1))
+                        narrow.append(np.zeros(v.shape[0], dtype=bool))
                 rws = -np.ones(n_ch * R, dtype=np.int64)
                 rws[chain_of * R + slot] = rows[sel]
                 rows_out.append(rws)
@@ -829,6 +837,7 @@ class CircuitPlan:
                     assign[w].append(c)
                     heapq.heappush(h, (load + chains[c][2], w))
             bar = _native.SLICE_BARRIER if end == "col" else _native.SLICE_WG_BARRIER
+            nar_all = np.concatenate(narrow)
             for w in range(NW):
                 recs = []
                 for c in assign[w]:
@@ -837,6 +846,8 @@ class CircuitPlan:
                         flags = g | kflag | (nrow << 16)
                         if ch == nch - 1:
                             flags |= _native.SLICE_FINISH
+                        if nar_all[first + ch]:
+                            flags |= _native.SLICE_NARROW
                         b = 512 * (first + ch)
                         recs.append((b, b, rowbase, r0, min(MAX_ITER, n_it - ch * MAX_ITER), flags))
                 if mode == "wave":
@@ -873,6 +884,35 @@ class CircuitPlan:
         self.slice_rows = np.concatenate(rows_out).astype(np.int32)
         self.n_slices = len(solve)
 
+    @staticmethod
+    def _narrow_index_words(o, rec, it, lane):
+        """Index data of off-block records of a global face store. A lane's
+        entries are sorted columns of one row; when every lane of a record spans
+        fewer than 2^16 positions the record is NARROW: slot l holds lane l's
+        base (its smallest index), and slots 64 + 4l + q hold its entries 2q and
+        2q+1 as 16-bit offsets from the base (low, high half); padding entries
+        read the base (their value is zero). 1.25 KB of indices per record
+        instead of 2 KB. Other records keep index it of lane l at slot 8l + it.
+        Returns (words, narrow per record)."""
+        n_rec = o.shape[0]
+        real = np.zeros(o.shape, dtype=bool)
+        real[rec, it, lane] = True
+        big = np.iinfo(np.int64).max
+        lo = np.where(real, o, big).min(axis=1)              # (n_rec, WAVE)
+        hi = np.where(real, o, -1).max(axis=1)
+        has = hi >= 0
+        base = np.where(has, lo, 0)
+        nar = np.all(~has | (hi - lo < 1 << 16), axis=1)
+        out = o.transpose(0, 2, 1).reshape(n_rec, -1).copy()  # wide layout
+        if nar.any():
+            d = np.where(real, o - base[:, None, :], 0)[nar]   # (n_nar, 8, WAVE)
+            words = np.zeros((int(nar.sum()), MAX_ITER * WAVE), dtype=np.int64)
+            words[:, :WAVE] = base[nar]
+            packed = d[:, 0::2, :] | (d[:, 1::2, :] << 16)     # (n_nar, 4, WAVE)
+            words[:, WAVE:WAVE + 4 * WAVE] = packed.transpose(0, 2, 1).reshape(len(packed), -1)
+            out[nar] = words
+        return out.reshape(-1), nar
+
     def _block_index_words(self, rec, it, lane, ix, g, n_rec):
         """Index data of block-phase records of a global face store. A lane's
         entries in one record address consecutive scratch rows, so its 8
@@ -899,6 +939,14 @@ class CircuitPlan:
         read from the index array, or decoded from the lane words of a
         global-store block-phase record (_block_index_words)."""
         lane = np.arange(WAVE)
+        if flags & _native.SLICE_NARROW:
+            base = self.ent_off[ibase + lane].astype(np.int64)
+            out = np.empty((MAX_ITER, WAVE), dtype=np.int64)
+            for q in range(MAX_ITER // 2):
+                w = self.ent_off[ibase + WAVE + 4 * lane + q].astype(np.int64)
+                out[2 * q] = base + (w & 0xFFFF)
+                out[2 * q + 1] = base + (w >> 16)
+            return out
         if not (flags & _native.SLICE_BLOCK and self.index_bytes == 4):
             return np.stack([self.ent_off[ibase + 8 * lane + it].astype(np.int64) for it in range(MAX_ITER)])
         words = self.ent_off[ibase + lane].astype(np.int64)
