@@ -40,24 +40,37 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = "batched_iv_sweep_square100_W256"
 
 
-def make_problem(pa, n, W, Nt, T=0.01, seed=12345, steps_per_launch=None, block_size=None):
+def bias_amplitudes(W):
+    """Bias points of the IV sweep, in units of current_base(0)."""
+    return np.linspace(0.0, 2.0, W)
+
+
+def make_problem(pa, n, W, Nt, T=0.01, seed=12345, steps_per_launch=None, block_size=None,
+                 amplitudes=None, **engine_options):
+    """The IV sweep: the bias is current_base(0) times one amplitude per column,
+    passed as a SeparableSource (the device forms the product, bit for bit the
+    reference's current_base[:, None, None] * linspace[None, :, None])."""
     c = pa.SquareArray(n, n)
     rng = np.random.default_rng(20261202)
     c.set_critical_current(np.exp(0.1 * rng.standard_normal(c.junction_count())))
     c.set_capacitance(1.0)
-    Is = c.current_base(0.0)[:, None, None] * np.linspace(0.0, 2.0, W)[None, :, None]
+    amp = bias_amplitudes(W) if amplitudes is None else np.asarray(amplitudes, dtype=np.float64)
+    Is = pa.SeparableSource(c.current_base(0.0), amp[:, None])
     prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.25,
                                    current_sources=Is, temperature=T,
                                    store_time_steps=[Nt // 3, Nt - 1], store_current=False,
                                    store_voltage=False, seed=seed,
-                                   steps_per_launch=steps_per_launch, block_size=block_size)
+                                   steps_per_launch=steps_per_launch, block_size=block_size,
+                                   **engine_options)
     return c, prob
 
 
-def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events=True):
+def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events=True, amplitudes=None,
+            **engine_options):
     from pyjjasim_amd.engine import Engine
     Nt = warmup + steps
-    c, prob = make_problem(pa, n, W, Nt, steps_per_launch=steps_per_launch, block_size=block_size)
+    c, prob = make_problem(pa, n, W, Nt, steps_per_launch=steps_per_launch, block_size=block_size,
+                           amplitudes=amplitudes, **engine_options)
     th_mask = prob.store_time_steps.copy()
     I_mask = np.zeros(Nt, dtype=bool)
     eng = Engine(prob, th_mask, I_mask)
@@ -81,9 +94,12 @@ def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events
     wall = time.perf_counter() - t_start
     launch_ms = [(k, e0.elapsed_time(e1)) for k, e0, e1 in launches]
     th, _, nv = eng.results()
+    sp = eng.host.solve_plan
     info = dict(Nj=c.junction_count(), Nf=c.face_count(), bytes_per_step=eng.bytes_per_step(),
-                nnz_factor=len(eng.host.solve_plan.lo_val) + len(eng.host.solve_plan.inv_cm),
-                n_stages=eng.host.solve_plan.n_stages, block_size=eng.block_size,
+                bytes_per_stored_step=8 * c.junction_count() * W,
+                stored_in_window=int(th_mask[warmup:].sum()),
+                nnz_factor=int(len(sp.lo_val) + len(sp.inv_cm)), nnz_L=int(sp.nnz_L),
+                n_stages=sp.n_stages, block_size=eng.block_size, parts=eng.host.parts,
                 steps_per_launch=eng.steps_per_launch)
     eng.close()
     return wall, launch_ms, th, nv, c, info
@@ -152,7 +168,7 @@ def cpu_baseline_all_cores(n, W, steps, max_workers=16):
                        f"{len(jobs)} single-threaded processes; slowest worker {max(times):.1f} s")
 
 
-def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=5, fd_steps=200):
+def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=50, fd_steps=200, h=1e-4):
     """Differentiation workload: SquareArray(n, n), trainable per-junction Ic
     (log-normal, sigma 0.2, seed A), loss = mean over W bias points (0.5..2.0 Ic)
     of (V_w - V*_w)^2 with V_w the time-averaged voltage of the biased junctions
@@ -160,9 +176,10 @@ def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=5, fd_ste
 
     Timed: the forward pass and the backward pass (segment recompute plus the
     adjoint sweep), each after a warm-up. Checked: the gradient against central
-    differences (h = 1e-6) on a few junctions over the first ``fd_steps`` steps;
-    over the full horizon the dynamics are chaotic and central differences do not
-    converge as h shrinks, so no difference quotient can check it there."""
+    differences on ``n_fd`` seeded junctions over the first ``fd_steps`` steps,
+    Richardson-extrapolated from steps h and h/2 (truncation error O(h^4));
+    over the full horizon the dynamics are chaotic and central differences do
+    not converge as h shrinks, so no difference quotient can check it there."""
     c = pa.SquareArray(n, n)
     c.set_capacitance(1.0)
     Nj, Nf = c.junction_count(), c.face_count()
@@ -186,38 +203,49 @@ def adjoint_bench(pa, torch, n=64, W=32, Nt=4000, checkpoint=200, n_fd=5, fd_ste
             return th[biased].mean(0) / (steps * dt)
         with torch.no_grad():
             target = volt(Ic_star)
-        return lambda Ic: ((volt(Ic) - target) ** 2).mean()
+        return (lambda Ic: ((volt(Ic) - target) ** 2).mean()), evo
 
-    loss = make(Nt)
+    loss, evo = make(Nt)
     times = []
-    for rep in range(2):                          # the first pass warms up
-        Ic = Ic0.clone().requires_grad_(True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        L = loss(Ic)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        L.backward()
-        torch.cuda.synchronize()
-        times.append((t1 - t0, time.perf_counter() - t1))
+    try:
+        for rep in range(2):                          # the first pass warms up
+            Ic = Ic0.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            L = loss(Ic)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.backward()
+            torch.cuda.synchronize()
+            times.append((t1 - t0, time.perf_counter() - t1))
+        parts = evo.eng.host.parts
+    finally:
+        evo.close()
     t_fwd, t_bwd = times[-1]
-    loss_fd = make(fd_steps)
-    Ic = Ic0.clone().requires_grad_(True)
-    loss_fd(Ic).backward()
-    g = Ic.grad.detach().cpu().numpy()
-    idx = np.random.default_rng(303).choice(Nj, size=n_fd, replace=False)
-    h, errs = 1e-6, []
-    for j in idx:
-        p, m = Ic0.clone(), Ic0.clone()
-        p[j] += h
-        m[j] -= h
-        with torch.no_grad():
-            fd = (loss_fd(p) - loss_fd(m)).item() / (2 * h)
-        errs.append(abs(g[j] - fd) / max(abs(fd), 1e-3 * np.abs(g).max()))
+    loss_fd, evo_fd = make(fd_steps)
+    try:
+        Ic = Ic0.clone().requires_grad_(True)
+        loss_fd(Ic).backward()
+        g = Ic.grad.detach().cpu().numpy()
+        idx = np.random.default_rng(303).choice(Nj, size=n_fd, replace=False)
+
+        def central(j, step):
+            p, m = Ic0.clone(), Ic0.clone()
+            p[j] += step
+            m[j] -= step
+            with torch.no_grad():
+                return (loss_fd(p) - loss_fd(m)).item() / (2 * step)
+        fd = np.array([(4 * central(j, h / 2) - central(j, h)) / 3 for j in idx])
+    finally:
+        evo_fd.close()
+    gs = g[idx]
+    scale = np.abs(g).max()
     return {"workload": f"SquareArray({n},{n}) W={W} Nt={Nt} checkpoint={checkpoint}, d loss / d Ic",
-            "forward_s": t_fwd, "backward_s": t_bwd, "grad_over_fwd": t_bwd / t_fwd,
-            "fd_check": {"steps": fd_steps, "junctions": int(n_fd), "h": h,
-                         "max_rel_err": float(max(errs))}}
+            "parts": int(parts), "forward_s": t_fwd, "backward_s": t_bwd, "grad_over_fwd": t_bwd / t_fwd,
+            "fd_check": {"steps": fd_steps, "junctions": int(n_fd), "h": h, "richardson": "h, h/2",
+                         "max_abs_err_over_max_grad": float(np.abs(gs - fd).max() / scale),
+                         "max_rel_err": float(np.max(np.abs(gs - fd) / np.abs(fd))),
+                         "max_abs_grad": float(scale)}}
 
 
 def uneven_problems(pa, Nt, n_problems=64, seed=3003):
@@ -344,12 +372,14 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000):
         t2 = time.perf_counter()
     g = f.grad.detach()
     fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
+    face_store = "global" if evo.eng.host.global_store else "lds"
+    n_exec = int(evo.eng.host.n_solve_exec)
+    evo.close()
     return {"metric": "junction-steps/s (large array forward, FP64)", "value": Nj * W / fwd,
             "unit": "junction-steps/s", "steps": steps, "dtype": "f64",
             "config": {"workload": f"large_square{n}_W{W}", "junctions": Nj, "faces": Nf, "W": W,
                        "beta_L": 0.5, "T": 0.05, "checkpoint": min(checkpoint, steps),
-                       "face_store": "global" if evo.eng.host.global_store else "lds",
-                       "solve_records_per_column_step": int(evo.eng.host.n_solve_exec)},
+                       "face_store": face_store, "solve_records_per_column_step": n_exec},
             "forward_ms_per_step": fwd * 1e3, "backward_ms_per_step": bwd * 1e3,
             "grad_over_fwd": bwd / fwd,
             "full_horizon_estimate_s": {"steps": full_steps, "forward": fwd * full_steps,
@@ -358,11 +388,15 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000):
             "host_setup_s": setup_s}
 
 
-def measured_traffic(steps_per_launch):
-    """HBM bytes per evolve_kernel launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel factor), or
-    None when the summary was taken at another launch length."""
-    path = os.path.join(ROOT, "profiles", "evolve_kernel_square100_W256_traffic.txt")
+TRAFFIC_PROFILE = os.path.join("profiles", "evolve_kernel_square100_W256_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per step of evolve_kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE of one dispatch, each scaled by the
+    copy-kernel factor, divided by the dispatch's steps) and the launch length
+    it was taken at; None when the summary is missing."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -372,11 +406,70 @@ def measured_traffic(steps_per_launch):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        if int(vals["steps_per_launch"]) != steps_per_launch:
-            return None
-        return float(vals["hbm_bytes_per_launch"])
+        return float(vals["hbm_bytes_per_step"]), int(vals["steps_per_launch"])
     except (KeyError, ValueError):
         return None
+
+
+def host_info():
+    """The host the CPU baseline ran on: CPU model, usable cores, library
+    versions and thread settings."""
+    import scipy
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return {"cpu_model": model, "usable_cores": len(os.sched_getaffinity(0)), "numpy": np.__version__,
+            "scipy": scipy.__version__, "thread_env": env}
+
+
+def plain_best(pa, torch, n, steps, amplitude=1.0, parts_options=(1, 2, 4, 8, 16)):
+    """The plain loop: one problem (one bias point, W = 1) at a time, in its
+    fastest single-problem layout: one workgroup with the LDS face store, or the
+    column split over 2..16 workgroups with the global face store. Returns the
+    best rate and every layout's ms/step."""
+    per = {}
+    for parts in parts_options:
+        opts = {"parts": parts} if parts > 1 else {"parts": 1}
+        if parts > 1:
+            opts["face_store"] = "global"
+        wall, _, _, _, c, info = run_gpu(pa, torch, n, 1, steps, 100, None, None, events=False,
+                                         amplitudes=[amplitude], **opts)
+        per[f"parts{parts}"] = wall / steps * 1e3
+    best = min(per, key=per.get)
+    Nj = 2 * n * (n - 1)
+    return {"value": Nj / (per[best] * 1e-3), "unit": "junction-steps/s", "layout": best,
+            "ms_per_step_by_layout": per,
+            "note": f"one problem (W=1, bias {amplitude} current_base) per run, {steps} steps, "
+                    "fastest single-problem layout"}
+
+
+def plain_small(pa, torch, Nt=10000):
+    """The reference's plain small case: SquareArray(10,10), Ic = R = 1, C = L = 0,
+    f = 0.1, Is = 0.5 current_base(0), T = 0, dt = 0.05, Nt steps, W = 1, every
+    step stored (theta, current, voltage); the whole compute() call is timed,
+    download of the stored steps included."""
+    c = pa.SquareArray(10, 10)
+    Is = 0.5 * c.current_base(0.0)[:, None, None]
+    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.1,
+                                   current_sources=Is)
+    prob.compute()                                  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = prob.compute()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    Nj = c.junction_count()
+    assert res.get_theta().shape == (Nj, 1, Nt)
+    return {"workload": "plain_small_square10_W1", "steps": Nt, "seconds": wall,
+            "value": Nj * Nt / wall, "unit": "junction-steps/s",
+            "note": "compute() end to end: plan, launches, download of every stored step"}
 
 
 def copy_calibration(torch, mib):
@@ -404,6 +497,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-plain-small", action="store_true")
     ap.add_argument("--workload", choices=["iv_sweep", "uneven_batch", "large"], default="iv_sweep",
                     help="iv_sweep: SquareArray(100,100) W=256 (default); uneven_batch: "
                          "64 problems of uneven size in one launch per chunk; large: "
@@ -442,26 +536,33 @@ def main():
                                                args.steps_per_launch, args.block_size)
     Nj, W, K = info["Nj"], args.W, args.steps
     value = Nj * W * K / wall
-    # dominant kernel: evolve_kernel; one launch advances k steps of the whole batch
+    # dominant kernel: evolve_kernel; one launch advances k steps of the whole
+    # batch. Algorithmic bytes of the timed launches: B_step per step plus the
+    # theta rows stored within them
     tot_steps = sum(k for k, _ in launch_ms)
     tot_ms = sum(ms for _, ms in launch_ms)
-    mean_launch_ms = tot_ms / len(launch_ms)
-    bytes_per_launch = info["bytes_per_step"] * tot_steps / len(launch_ms)
+    n_launch = len(launch_ms)
+    mean_launch_ms = tot_ms / n_launch
+    bytes_timed = info["bytes_per_step"] * tot_steps + info["bytes_per_stored_step"] * info["stored_in_window"]
+    bytes_per_launch = bytes_timed / n_launch
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(info["steps_per_launch"]) if (args.n, W) == (100, 256) else None
+    traffic = traffic_note = None
+    if (args.n, W) == (100, 256):
+        mt = measured_traffic()
+        if mt is not None:
+            per_step, prof_len = mt
+            traffic = per_step * tot_steps / n_launch
+            traffic_note = (f"{TRAFFIC_PROFILE}: {per_step / 1e6:.1f} MB/step measured on a "
+                            f"{prof_len}-step dispatch, times {tot_steps / n_launch:g} steps per launch")
     # integrity check at full size: device vortex counts equal the host count
     A = c.get_cycle_matrix()
     n_host = (-A @ np.round(th[:, :, -1] / (2.0 * np.pi))).astype(int)
     assert np.array_equal(n_host, nv[:, :, -1]), "device vortex counts disagree with theta"
-    # plain: one problem at a time (W = 1) on the same circuit
+    # plain: one problem at a time (W = 1) on the same circuit, best layout
     plain = None
     if args.plain_steps > 0:
-        pwall, pl, _, _, _, _ = run_gpu(pa, torch, args.n, 1, args.plain_steps, 100,
-                                        args.steps_per_launch, args.block_size, events=False)
-        plain_rate = Nj * 1 * args.plain_steps / pwall
-        plain = {"value": plain_rate, "unit": "junction-steps/s",
-                 "note": "one problem (W=1) per run, same circuit and kernel",
-                 "batched_over_plain": value / plain_rate}
+        plain = plain_best(pa, torch, args.n, args.plain_steps)
+        plain["batched_over_plain"] = value / plain["value"]
     out = {
         "metric": "junction-steps/s (batched IV sweep, FP64)",
         "value": value,
@@ -478,18 +579,24 @@ def main():
         "config": {"workload": WORKLOAD, "lattice": f"SquareArray({args.n},{args.n})",
                    "junctions": Nj, "faces": info["Nf"], "W": W, "dt": 0.05, "T": 0.01,
                    "beta_C": 1.0, "f": 0.25, "Nt": args.warmup + K,
+                   "bias": "SeparableSource(current_base(0), linspace(0, 2, W))",
                    "steps_per_launch": info["steps_per_launch"], "block_size": info["block_size"],
-                   "solve_stages": info["n_stages"]},
+                   "parts": info["parts"], "solve_stages": info["n_stages"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": "evolve_kernel", "bytes_per_step": info["bytes_per_step"],
-                     "mean_launch_ms": mean_launch_ms},
+                     "bytes_per_launch": bytes_per_launch, "stored_steps_in_window": info["stored_in_window"],
+                     "nnz_factor_per_pass": info["nnz_factor"], "nnz_L": info["nnz_L"],
+                     "mean_launch_ms": mean_launch_ms, "launches": n_launch},
         "plain": plain,
     }
+    if not args.no_plain_small:
+        out["plain_small"] = plain_small(pa, torch)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(pa, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pa, args.n, W, args.cpu_seconds)
+        out["cpu_baseline"].update(host_info())
         steps = int(out["cpu_baseline"]["sample"].split()[0])
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(args.n, W, steps)
     print(json.dumps(out), flush=True)

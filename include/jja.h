@@ -16,10 +16,15 @@
  *     is available from jja_last_error() (thread-local);
  *   - plan creation copies host arrays; run-time pointers (state, inputs,
  *     outputs) are DEVICE pointers owned by the caller;
- *   - no function allocates device memory after jja_plan_create, except that
- *     jja_evolve grows one plan-owned descriptor buffer when it first sees more
- *     problems or columns;
- *   - a plan may be used from one host thread at a time.
+ *   - no function allocates device or host memory after jja_plan_create: the
+ *     plan's launch descriptors, work lists and barrier words are sized there
+ *     (jja_problem_desc.max_columns);
+ *   - a plan may be used from one host thread at a time, and its launches run
+ *     on one stream at a time: jja_evolve or jja_adjoint on a stream other than
+ *     the one of the plan's previous launch, while that launch has not
+ *     finished, is refused with JJA_EBUSY (the plan's launch descriptors are
+ *     shared by its launches). Different plans run concurrently on different
+ *     streams.
  */
 #ifndef JJA_H
 #define JJA_H
@@ -36,6 +41,7 @@ extern "C" {
 #define JJA_EFACTOR (-3)      /* face matrix not positive definite               */
 #define JJA_EHIP (-4)         /* HIP runtime error; message has hipGetErrorString */
 #define JJA_EUNSUPPORTED (-5) /* problem too large for this kernel variant       */
+#define JJA_EBUSY (-6)        /* plan still running a launch on another stream   */
 
 typedef struct jja_factor jja_factor;
 typedef struct jja_plan jja_plan;
@@ -170,6 +176,11 @@ typedef struct jja_problem_desc {
    * W is odd). Needs parts >= 2 and block_size/parts <= 512; the same value for
    * every problem of a plan. Results are those of unpaired columns, bit for bit. */
   int32_t column_pairs;
+  /* the most columns (W) a run of this problem will have, 1 .. 2^20: plan
+   * creation sizes the launch descriptors, work lists and barrier words for it,
+   * so that jja_evolve and jja_adjoint allocate nothing; a run with more
+   * columns is refused (JJA_EINVAL) */
+  int32_t max_columns;
 } jja_problem_desc;
 
 /*

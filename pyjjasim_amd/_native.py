@@ -18,6 +18,7 @@ JJA_ENOMEM = -2
 JJA_EFACTOR = -3
 JJA_EHIP = -4
 JJA_EUNSUPPORTED = -5
+JJA_EBUSY = -6
 
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
@@ -50,7 +51,7 @@ class ProblemDesc(C.Structure):
         ("ent_off", C.c_void_p),
         ("max_stage_rows", C.c_int32), ("face_copy_rows", C.c_int32),
         ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("parts", C.c_int32),
-        ("column_pairs", C.c_int32),
+        ("column_pairs", C.c_int32), ("max_columns", C.c_int32),
     ]
 
 

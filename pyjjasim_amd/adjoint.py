@@ -93,6 +93,27 @@ class DifferentiableEvolution:
     def __call__(self, Ic, f, Is):
         return _apply(Ic, f, Is, self)
 
+    def close(self):
+        """Free the device plan now (also done when the object is collected)."""
+        eng = getattr(self, "eng", None)
+        if eng is not None:
+            eng.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    def _check_status(self):
+        """Raise NativeError if a column barrier of a split launch gave up (its
+        results are invalid); waits for the launches of a split plan."""
+        eng = self.eng
+        if eng.host.parts > 1:
+            self.torch.cuda.current_stream().synchronize()
+        _native.check(eng.lib.jja_plan_status(eng.handle))
+
     # ------------------------------------------------------------------ forward
     def _set_inputs(self, Ic, f, Is):
         eng = self.eng
@@ -115,6 +136,7 @@ class DifferentiableEvolution:
         for seg in range(self.n_seg):
             ckpt.append(eng.theta.clone())
             eng.launch(min(self.C, self.Nt - eng.t))
+        self._check_status()
         out = eng.th_out[:self.n_th].permute(2, 1, 0).clone()
         return out, (ckpt, self._Ic, self._f, self._Is, self._fsrc, self._isrc)
 
@@ -168,6 +190,7 @@ class DifferentiableEvolution:
                 adj.prime = 1 if seg == self.n_seg - 1 else 0
                 adj.hist_m0 = s0 - 2
                 _native.check(eng.lib.jja_adjoint(eng.handle, C.byref(adj), s1 - 1, s0, stream))
+            self._check_status()
         finally:
             run.th_slot, run.slot_t0, run.th_out = saved
         gf_face = torch.zeros_like(gf)

@@ -11,6 +11,14 @@ several workgroups, and every face store is global. Workgroups are ordered
 largest problem first, so the hardware's in-order dispatch is a longest-first
 list schedule.
 
+Random inputs: with Philox noise each problem uses its own ``seed`` (a problem
+without one draws it from numpy's global RNG when the batch's engine is made,
+problem by problem). With ``noise="reference"`` the launches draw every
+problem's normals from numpy's global RNG, problem by problem within each chunk
+of steps, so a batch of one problem repeats its compute() exactly after the same
+np.random.seed, and a larger batch uses the same stream in that interleaved
+order.
+
 ``autotune=True`` picks the workgroup size, the parts per column and the launch
 length by measuring: each candidate (workgroup size, parts) is timed on a short
 run of the batch (the factors are computed once; only the record layouts
@@ -213,6 +221,10 @@ class TimeEvolutionBatch:
         store masks over the time steps; by default each problem's own."""
         if masks is None:
             masks = [_masks(p)[2:4] for p in self.problems]
+        # tuning launches draw from numpy's global RNG (Philox seeds, reference
+        # noise); the run itself starts from the state the caller left, so its
+        # results equal each problem's own compute() after the same seeding
+        rng_state = np.random.get_state()
         if self.autotune:
             bs, parts, hosts, record = self._tune(masks)
             self.block_size, self.parts = bs, parts
@@ -225,11 +237,13 @@ class TimeEvolutionBatch:
             hosts = build_plans(self.problems, self.block_size, self.workers)
             if self.parts == "balanced" or self.parts > 1:
                 hosts = self._hosts_for(hosts, self.block_size, self.parts)
+        np.random.set_state(rng_state)
         eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=hosts)
         try:
             if self.autotune and not self.steps_per_launch:
                 spl = self._launch_length(eng)
                 eng.close()
+                np.random.set_state(rng_state)
                 eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=eng.hosts)
                 eng.steps_per_launch = spl
                 self.tuning["steps_per_launch"] = spl

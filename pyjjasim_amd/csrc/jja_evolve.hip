@@ -1278,6 +1278,7 @@ struct Plan {
   std::vector<int64_t> lds_bytes;     // per problem; 0 for a global face store
   std::vector<int64_t> zbuf_doubles;  // per problem; 0 for an LDS face store
   std::vector<int64_t> cost;          // per problem: entries one step of a column reads
+  std::vector<int32_t> max_cols;      // per problem: the most columns a run may have
   DevProblem* d_problems = nullptr;
   jja_problem_run* d_runs = nullptr;
   WorkItem* d_work = nullptr;
@@ -1285,7 +1286,7 @@ struct Plan {
   int64_t lds_limit = 0;
   int32_t block_size = 1024;
   int32_t stencil_width = 3;
-  jja_adjoint_run* d_adj = nullptr;  // allocated at the first jja_adjoint call
+  jja_adjoint_run* d_adj = nullptr;
   int32_t parts = 1;                 // workgroups per column (block_size is one workgroup's)
   bool pairs = false;                // split launches work two columns per work item
   uint32_t* d_ctrs = nullptr;        // split launches: barrier words per column (counter, XCD set)
@@ -1294,6 +1295,19 @@ struct Plan {
                                      // [1] workgroups whose column ran on one XCD
   int32_t n_cu = 0;                  // compute units of the device
   uint32_t wait_limit = kWaitLimit;  // split launches: polls before a column barrier gives up
+  // what the device copies of the launch descriptors hold: a launch uploads
+  // only what changed since the previous one
+  std::vector<jja_problem_run> h_runs;
+  std::vector<jja_adjoint_run> h_adj;
+  std::vector<WorkItem> h_work;
+  bool runs_valid = false, adj_valid = false, work_valid = false;
+  // the stream of the previous launch and an event recorded after it
+  hipEvent_t done = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool launched = false;
+  // per kernel: the dynamic LDS limit set on it, and its resident workgroups per CU
+  std::vector<std::pair<const void*, int>> lds_attr;
+  std::vector<std::pair<const void*, int>> per_cu;
 };
 
 template <typename T>
@@ -1311,7 +1325,58 @@ int upload(Plan* plan, const T* host, size_t n, const T** out) {
 
 void free_plan(Plan* plan) {
   for (void* p : plan->allocations) (void)hipFree(p);
+  if (plan->done) (void)hipEventDestroy(plan->done);
   delete plan;
+}
+
+// Launches of a plan share its device descriptors, so they run on one stream
+// at a time: a launch on another stream waits for nothing and is refused while
+// the previous launch has not finished.
+int claim_stream(Plan* plan, hipStream_t stream) {
+  if (!plan->launched || stream == plan->last_stream) return JJA_OK;
+  const hipError_t e = hipEventQuery(plan->done);
+  if (e == hipErrorNotReady)
+    return fail(JJA_EBUSY, "the plan's previous launch, on another stream, has not finished");
+  if (e != hipSuccess) return fail(JJA_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+  return JJA_OK;
+}
+
+int mark_launched(Plan* plan, hipStream_t stream) {
+  JJA_HIP(hipEventRecord(plan->done, stream));
+  plan->last_stream = stream;
+  plan->launched = true;
+  return JJA_OK;
+}
+
+// Copies n host descriptors to the plan's device array when they differ from
+// what it holds (pageable source: staged before the call returns, ordered after
+// earlier work on the stream).
+template <typename T>
+int upload_if_changed(T* dev, const T* host, size_t n, std::vector<T>& shadow, bool& valid, hipStream_t stream) {
+  if (valid && shadow.size() == n && std::memcmp(shadow.data(), host, n * sizeof(T)) == 0) return JJA_OK;
+  JJA_HIP(hipMemcpyAsync(dev, host, n * sizeof(T), hipMemcpyHostToDevice, stream));
+  shadow.assign(host, host + n);
+  valid = true;
+  return JJA_OK;
+}
+
+// The dynamic LDS limit of a kernel, set once per value.
+int set_lds_limit(Plan* plan, const void* kern, int lds) {
+  for (auto& a : plan->lds_attr)
+    if (a.first == kern) {
+      if (a.second >= lds) return JJA_OK;
+      JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      a.second = lds;
+      return JJA_OK;
+    }
+  hipFuncAttributes attr;
+  JJA_HIP(hipFuncGetAttributes(&attr, kern));
+  // the solve reads LDS at the plan's absolute byte offsets: the dynamic LDS
+  // array must start at address 0, i.e. the kernel must have no static LDS
+  if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "kernel with a dynamic LDS face store has static LDS");
+  JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  plan->lds_attr.push_back({kern, lds});
+  return JJA_OK;
 }
 
 bool pow2_le(int32_t g, int32_t limit) { return g >= 1 && g <= limit && (g & (g - 1)) == 0; }
@@ -1356,6 +1421,8 @@ int validate(const jja_problem_desc& d) {
   if (d.column_pairs != 0 && (d.column_pairs != 1 || parts < 2 || wg > 512))
     return fail(JJA_EINVAL, "jja_plan_create: column_pairs needs split columns (parts >= 2) and workgroups "
                             "of at most 512 threads");
+  if (d.max_columns < 1 || d.max_columns > (1 << 20))
+    return fail(JJA_EINVAL, "jja_plan_create: max_columns must be 1 .. 2^20");
   if (!d.Ic || !d.Rv || !d.coef || !d.junc_code)
     return fail(JJA_EINVAL, "jja_plan_create: missing junction arrays");
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
@@ -1489,13 +1556,7 @@ template <int BS, int SW, int STORE>
 int launch_evolve(Plan* plan, const WorkItem* work, int32_t n_work, int64_t lds, int32_t n_steps,
                   int64_t step0, hipStream_t stream) {
   auto* kern = evolve_kernel<BS, SW, STORE>;
-  // the solve reads LDS at the plan's absolute byte offsets: the dynamic LDS
-  // array must start at address 0, i.e. the kernel must have no static LDS
-  hipFuncAttributes attr;
-  JJA_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kern)));
-  if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "evolve kernel has static LDS");
-  JJA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  if (int rc = set_lds_limit(plan, reinterpret_cast<const void*>(kern), static_cast<int>(lds))) return rc;
   hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
                      plan->d_problems, plan->d_runs, work, n_steps, step0);
   JJA_HIP(hipGetLastError());
@@ -1506,11 +1567,7 @@ template <int BS, bool GLOBAL>
 int launch_adjoint(Plan* plan, const WorkItem* work, int32_t n_work, int64_t lds, int64_t k_hi,
                    int64_t k_lo, hipStream_t stream) {
   auto* kern = adjoint_kernel<BS, GLOBAL>;
-  hipFuncAttributes attr;
-  JJA_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kern)));
-  if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "adjoint kernel has static LDS");
-  JJA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  if (int rc = set_lds_limit(plan, reinterpret_cast<const void*>(kern), static_cast<int>(lds))) return rc;
   hipLaunchKernelGGL(kern, dim3(n_work), dim3(BS), static_cast<size_t>(lds), stream,
                      plan->d_problems, plan->d_adj, work, k_hi, k_lo);
   JJA_HIP(hipGetLastError());
@@ -1526,24 +1583,11 @@ int launch_vortex(Plan* plan, int32_t p, const double* theta, int32_t cols, int3
   return JJA_OK;
 }
 
-// Copies the work items of one launch to the device (grows the array as needed).
+// Copies the work items of one launch to the plan's array (sized at plan
+// creation) when they differ from what it holds.
 int upload_work(Plan* plan, const std::vector<WorkItem>& work, hipStream_t stream) {
-  if (work.size() > plan->work_cap) {
-    if (plan->d_work) {
-      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_work));
-      if (it != plan->allocations.end()) plan->allocations.erase(it);
-      JJA_HIP(hipFree(plan->d_work));
-      plan->d_work = nullptr;
-    }
-    void* d = nullptr;
-    JJA_HIP(hipMalloc(&d, work.size() * sizeof(WorkItem)));
-    plan->allocations.push_back(d);
-    plan->d_work = static_cast<WorkItem*>(d);
-    plan->work_cap = work.size();
-  }
-  JJA_HIP(hipMemcpyAsync(plan->d_work, work.data(), sizeof(WorkItem) * work.size(),
-                         hipMemcpyHostToDevice, stream));
-  return JJA_OK;
+  if (work.size() > plan->work_cap) return fail(JJA_EINVAL, "launch has more work items than the plan was sized for");
+  return upload_if_changed(plan->d_work, work.data(), work.size(), plan->h_work, plan->work_valid, stream);
 }
 
 template <int BS, int STORE>
@@ -1633,22 +1677,10 @@ std::vector<WorkItem> split_work(const Plan* plan, std::vector<WorkItem> cols) {
 int32_t items_of(const Plan* plan, int32_t W) { return plan->pairs ? (W + 1) / 2 : W; }
 
 // Barrier words of n work items (arrival counter, XCD set), zeroed on the
-// stream before a split launch.
+// stream before a split launch (the array is sized at plan creation).
 int reset_counters(Plan* plan, size_t n, hipStream_t stream) {
   n *= 2;
-  if (n > plan->ctr_cap) {
-    if (plan->d_ctrs) {
-      auto it = std::find(plan->allocations.begin(), plan->allocations.end(), static_cast<void*>(plan->d_ctrs));
-      if (it != plan->allocations.end()) plan->allocations.erase(it);
-      JJA_HIP(hipFree(plan->d_ctrs));
-      plan->d_ctrs = nullptr;
-    }
-    void* d = nullptr;
-    JJA_HIP(hipMalloc(&d, n * sizeof(uint32_t)));
-    plan->allocations.push_back(d);
-    plan->d_ctrs = static_cast<uint32_t*>(d);
-    plan->ctr_cap = n;
-  }
+  if (n > plan->ctr_cap) return fail(JJA_EINVAL, "launch has more columns than the plan was sized for");
   JJA_HIP(hipMemsetAsync(plan->d_ctrs, 0, n * sizeof(uint32_t), stream));
   return JJA_OK;
 }
@@ -1657,8 +1689,13 @@ int reset_counters(Plan* plan, size_t n, hipStream_t stream) {
 // column barriers need every part of a column running), or an error.
 template <typename... Args>
 int launch_cooperative(Plan* plan, const void* kern, int BS, int32_t n, hipStream_t stream, Args... args) {
-  int per_cu = 0;
-  JJA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, 0));
+  int per_cu = -1;
+  for (const auto& a : plan->per_cu)
+    if (a.first == kern) per_cu = a.second;
+  if (per_cu < 0) {
+    JJA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, 0));
+    plan->per_cu.push_back({kern, per_cu});
+  }
   if (static_cast<int64_t>(per_cu) * plan->n_cu < n)
     return fail(JJA_EUNSUPPORTED, "split launch of " + std::to_string(n) + " workgroups; the device holds " +
                                       std::to_string(static_cast<int64_t>(per_cu) * plan->n_cu) + " at once");
@@ -1821,21 +1858,41 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     plan->lds_bytes.push_back(d.index_bytes == 2 ? store : 0);
     plan->cost.push_back(d.n_entries + d.n_face_entries + 8 * static_cast<int64_t>(d.n_junctions));
     plan->zbuf_doubles.push_back(d.index_bytes == 2 ? 0 : static_cast<int64_t>(d.n_faces) + d.max_stage_rows);
+    plan->max_cols.push_back(d.max_columns);
   }
   if (rc == JJA_OK) {
     const DevProblem* dprob = nullptr;
     rc = upload(plan, plan->problems.data(), plan->problems.size(), &dprob);
     plan->d_problems = const_cast<DevProblem*>(dprob);
   }
-  if (rc == JJA_OK) {
-    void* d = nullptr;
-    if (hipMalloc(&d, sizeof(jja_problem_run) * n_problems) != hipSuccess) {
-      rc = fail(JJA_ENOMEM, "jja_plan_create: device allocation failed");
-    } else {
-      plan->allocations.push_back(d);
-      plan->d_runs = static_cast<jja_problem_run*>(d);
-    }
+  // launch descriptors, work lists and barrier words for the largest runs
+  // the descriptors allow: launches allocate nothing
+  size_t items = 0, blocks = 0;
+  for (size_t p = 0; p < plan->max_cols.size(); ++p) {
+    const size_t it = static_cast<size_t>(items_of(plan, plan->max_cols[p]));
+    items += it;
+    blocks += it * static_cast<size_t>(plan->problems[p].parts);
   }
+  auto alloc = [&](size_t bytes, void** out) {
+    if (rc != JJA_OK) return;
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+      rc = fail(JJA_ENOMEM, "jja_plan_create: device allocation failed");
+      return;
+    }
+    plan->allocations.push_back(d);
+    *out = d;
+  };
+  alloc(sizeof(jja_problem_run) * n_problems, reinterpret_cast<void**>(&plan->d_runs));
+  alloc(sizeof(jja_adjoint_run) * n_problems, reinterpret_cast<void**>(&plan->d_adj));
+  alloc(sizeof(WorkItem) * blocks, reinterpret_cast<void**>(&plan->d_work));
+  plan->work_cap = blocks;
+  if (plan->parts > 1) {
+    alloc(2 * sizeof(uint32_t) * items, reinterpret_cast<void**>(&plan->d_ctrs));
+    plan->ctr_cap = 2 * items;
+  }
+  if (rc == JJA_OK && hipEventCreateWithFlags(&plan->done, hipEventDisableTiming) != hipSuccess)
+    rc = fail(JJA_EHIP, "jja_plan_create: hipEventCreateWithFlags failed");
   if (rc == JJA_OK && plan->parts > 1) {
     void* d = nullptr;
     if (hipMalloc(&d, 2 * sizeof(int32_t)) != hipSuccess || hipMemset(d, 0, 2 * sizeof(int32_t)) != hipSuccess) {
@@ -1910,6 +1967,8 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
   for (int32_t i = 0; i < np; ++i) {
     const jja_problem_run& r = runs[i];
     if (r.W <= 0 || !r.theta || !r.u) return fail(JJA_EINVAL, "jja_evolve: run needs W > 0, theta and u");
+    if (r.W > plan->max_cols[i])
+      return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " has more columns than its max_columns");
     if (r.dt <= 0) return fail(JJA_EINVAL, "jja_evolve: dt must be positive");
     if (!r.Is.base || !r.Vs.base || !r.f.base || !r.T.base)
       return fail(JJA_EINVAL, "jja_evolve: every source needs a base array");
@@ -1925,28 +1984,35 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
     for (int32_t w = 0; w < items_of(plan, r.W); ++w) (global ? work_g : work).push_back({i, w});
   }
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  // pageable source: the call returns once the data is staged, and the copy is
-  // ordered after earlier work on the stream
-  JJA_HIP(hipMemcpyAsync(plan->d_runs, runs, sizeof(jja_problem_run) * np, hipMemcpyHostToDevice, stream));
+  if (int rc = claim_stream(plan, stream)) return rc;
+  if (int rc = upload_if_changed(plan->d_runs, runs, static_cast<size_t>(np), plan->h_runs, plan->runs_valid, stream))
+    return rc;
+  int rc = JJA_OK;
   if (plan->parts > 1) {
     // every problem of a split plan has the global face store (validated; a
     // circuit without faces has none and needs none): all columns, one launch
     work.insert(work.end(), work_g.begin(), work_g.end());
     const std::vector<WorkItem> split = split_work(plan, work);
-    if (int rc = reset_counters(plan, work.size(), stream)) return rc;
-    if (int rc = upload_work(plan, split, stream)) return rc;
-    return launch_evolve_split(plan, static_cast<int32_t>(split.size()), n_steps, step0, stream);
+    if ((rc = reset_counters(plan, work.size(), stream))) return rc;
+    if ((rc = upload_work(plan, split, stream))) return rc;
+    rc = launch_evolve_split(plan, static_cast<int32_t>(split.size()), n_steps, step0, stream);
+  } else {
+    // one launch: a kernel for one kind of face store, or one that takes either
+    const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
+    work.insert(work.end(), work_g.begin(), work_g.end());
+    order_work(plan, work);
+    if ((rc = upload_work(plan, work, stream))) return rc;
+    if (lds < 8) lds = 8;
+    const int32_t n = n_lds + n_g;
+    if (n_g == 0)
+      rc = launch_evolve_bs<0>(plan, plan->d_work, n, lds, n_steps, step0, stream);
+    else if (n_lds == 0)
+      rc = launch_evolve_bs<1>(plan, plan->d_work, n, 8, n_steps, step0, stream);
+    else
+      rc = launch_evolve_bs<2>(plan, plan->d_work, n, lds, n_steps, step0, stream);
   }
-  // one launch: a kernel for one kind of face store, or one that takes either
-  const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
-  work.insert(work.end(), work_g.begin(), work_g.end());
-  order_work(plan, work);
-  if (int rc = upload_work(plan, work, stream)) return rc;
-  if (lds < 8) lds = 8;
-  const int32_t n = n_lds + n_g;
-  if (n_g == 0) return launch_evolve_bs<0>(plan, plan->d_work, n, lds, n_steps, step0, stream);
-  if (n_lds == 0) return launch_evolve_bs<1>(plan, plan->d_work, n, 8, n_steps, step0, stream);
-  return launch_evolve_bs<2>(plan, plan->d_work, n, lds, n_steps, step0, stream);
+  if (rc) return rc;
+  return mark_launched(plan, stream);
 }
 
 int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t k_lo, void* stream_ptr) {
@@ -1960,6 +2026,8 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
     const jja_adjoint_run& r = runs[i];
     if (r.W <= 0 || !r.hist || !r.lam || !r.u || !r.grad_Ic || !r.grad_Is || !r.grad_f)
       return fail(JJA_EINVAL, "jja_adjoint: run needs W > 0, hist, lam, u and the gradient arrays");
+    if (r.W > plan->max_cols[i])
+      return fail(JJA_EINVAL, "jja_adjoint: problem " + std::to_string(i) + " has more columns than its max_columns");
     if (r.hist_m0 > k_lo - 2) return fail(JJA_EINVAL, "jja_adjoint: hist must start at k_lo - 2 or earlier");
     if (r.g && (!r.g_slot || r.g_lo < r.slot_t0 || r.g_hi < r.g_lo))
       return fail(JJA_EINVAL, "jja_adjoint: g needs g_slot covering [g_lo, g_hi)");
@@ -1971,32 +2039,29 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
     if (!global) lds = std::max(lds, plan->lds_bytes[i]);
     for (int32_t w = 0; w < items_of(plan, r.W); ++w) (global ? work_g : work).push_back({i, w});
   }
-  if (!plan->d_adj) {
-    void* d = nullptr;
-    JJA_HIP(hipMalloc(&d, sizeof(jja_adjoint_run) * np));
-    plan->allocations.push_back(d);
-    plan->d_adj = static_cast<jja_adjoint_run*>(d);
-  }
   hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-  JJA_HIP(hipMemcpyAsync(plan->d_adj, runs, sizeof(jja_adjoint_run) * np, hipMemcpyHostToDevice, stream));
+  if (int rc = claim_stream(plan, stream)) return rc;
+  if (int rc = upload_if_changed(plan->d_adj, runs, static_cast<size_t>(np), plan->h_adj, plan->adj_valid, stream))
+    return rc;
+  int rc = JJA_OK;
   if (plan->parts > 1) {
     work.insert(work.end(), work_g.begin(), work_g.end());
     const std::vector<WorkItem> split = split_work(plan, work);
-    if (int rc = reset_counters(plan, work.size(), stream)) return rc;
-    if (int rc = upload_work(plan, split, stream)) return rc;
-    return launch_adjoint_split(plan, static_cast<int32_t>(split.size()), k_hi, k_lo, stream);
+    if ((rc = reset_counters(plan, work.size(), stream))) return rc;
+    if ((rc = upload_work(plan, split, stream))) return rc;
+    rc = launch_adjoint_split(plan, static_cast<int32_t>(split.size()), k_hi, k_lo, stream);
+  } else {
+    const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
+    order_work(plan, work);
+    order_work(plan, work_g);
+    work.insert(work.end(), work_g.begin(), work_g.end());
+    if ((rc = upload_work(plan, work, stream))) return rc;
+    if (lds < 8) lds = 8;
+    if (n_lds) rc = launch_adjoint_bs<false>(plan, plan->d_work, n_lds, lds, k_hi, k_lo, stream);
+    if (!rc && n_g) rc = launch_adjoint_bs<true>(plan, plan->d_work + n_lds, n_g, 8, k_hi, k_lo, stream);
   }
-  const int32_t n_lds = static_cast<int32_t>(work.size()), n_g = static_cast<int32_t>(work_g.size());
-  order_work(plan, work);
-  order_work(plan, work_g);
-  work.insert(work.end(), work_g.begin(), work_g.end());
-  if (int rc = upload_work(plan, work, stream)) return rc;
-  if (lds < 8) lds = 8;
-  if (n_lds)
-    if (int rc = launch_adjoint_bs<false>(plan, plan->d_work, n_lds, lds, k_hi, k_lo, stream)) return rc;
-  if (n_g)
-    if (int rc = launch_adjoint_bs<true>(plan, plan->d_work + n_lds, n_g, 8, k_hi, k_lo, stream)) return rc;
-  return JJA_OK;
+  if (rc) return rc;
+  return mark_launched(plan, stream);
 }
 
 int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_t* n_out,

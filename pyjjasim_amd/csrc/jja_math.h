@@ -15,6 +15,16 @@
 // Measured against glibc's sin (tests/test_jja_math.py): at most 1 ulp for
 // |x| <= 1e9; near the zeros k*pi/2, where the result is tiny, the absolute
 // error stays below 1e-24.
+//
+// The kernel polynomials jja_ksin and jja_kcos follow fdlibm's k_sin.c and
+// k_cos.c, which carry this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
 #ifndef JJA_MATH_H
 #define JJA_MATH_H
 

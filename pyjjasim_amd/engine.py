@@ -66,7 +66,10 @@ class DeviceSource:
             self.src.base = self.tensor.data_ptr()
             self.src.s_i, self.src.s_w, self.src.s_t = s_i, s_w, s_t
             self.src.t0 = 0
-            self.bytes_per_step = 8 * (n if s_i else 1) * (W if s_w else 1)
+            # read every step, per column: a term that varies over the columns
+            # (a term shared by all columns stays cache-resident, like the
+            # junction coefficients, and costs no HBM traffic per column)
+            self.bytes_per_step = 8 * (n if s_i else 1) * W if s_w else 0
             if sep:
                 # amplitude[w, t] as [t][w]: value = base * amplitude, one product
                 amp, (a_1, a_w, a_t) = source_layout(value.amplitude[None, :, :], 1, W, Nt)
@@ -164,7 +167,7 @@ class Engine:
             if getattr(problem, "column_pairs", False):
                 self.host = self.host.paired()
                 self.block_size = self.host.block_size
-            self._desc = self.host.desc()
+            self._desc = self.host.desc(max_columns=W)
             self.handle = C.c_void_p()
             _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
             self.owns_plan, self.index = True, 0
@@ -316,19 +319,25 @@ class Engine:
         return (np.concatenate((pre, th), axis=2), np.concatenate((pre_I, I), axis=2), n)
 
     def bytes_per_step(self):
-        """Algorithmic HBM bytes of one time step of the whole batch:
-        W (24 Nj + 48 Nf) + 12 nnz + 8 Nj k_par + sources + outputs.
+        """Algorithmic HBM bytes of one time step of the whole batch, without
+        stored outputs (bytes_per_stored_step adds those):
+        W (24 Nj + 48 Nf) + 24 nnz + 8 Nj k_par + sources.
         24 Nj: theta_{t-1}, theta_{t-2} read and theta_t written; 48 Nf: face
         right-hand side written, read and written by each triangular pass and
-        read by the A^T gather; nnz: factor entries both passes stream (off-block
-        rows of L and L^T plus the dense diagonal-block inverses), counted once per
-        step as an 8-byte value and a 4-byte index; k_par = 5 coefficients per
-        junction (Ic, c0, c1, c2, noise scale)."""
+        read by the A^T gather; nnz: factor entries one triangular pass streams
+        (off-block rows of L plus the dense diagonal-block inverses), each pass
+        reading an 8-byte value and a 4-byte index; k_par = 5 coefficients per
+        junction (Ic, c0, c1, c2, noise scale); sources: terms that vary over
+        the columns, 8 n W bytes each."""
         sp = self.host.solve_plan
-        factor = 12 * (len(sp.lo_val) + len(sp.up_val) + len(sp.inv_cm) + len(sp.inv_rm))
+        nnz = len(sp.lo_val) + len(sp.inv_cm)
         src = sum(s.bytes_per_step for k, s in self.srcs.items() if not s.src.zero)
-        out = 8 * self.Nj * self.W * (self.n_th + self.n_I) / max(self.Nt, 1)
-        return (self.W * (24 * self.Nj + 48 * self.Nf) + factor + 8 * self.Nj * 5 + src + out)
+        return self.W * (24 * self.Nj + 48 * self.Nf) + 24 * nnz + 8 * self.Nj * 5 + src
+
+    def bytes_per_stored_step(self):
+        """Output bytes of a step whose theta and current are both stored
+        (8 Nj W each); a step with one of them stores half."""
+        return 8 * self.Nj * self.W
 
     def close(self):
         if getattr(self, "owns_plan", False) and getattr(self, "handle", None) is not None \
@@ -369,7 +378,8 @@ class BatchEngine:
                                       block_size=self.block_size,
                                       face_store=getattr(p, "face_store", "auto"),
                                       local_stages=getattr(p, "local_stages", "auto")) for p in problems]
-        descs = (_native.ProblemDesc * len(problems))(*[h.desc() for h in self.hosts])
+        descs = (_native.ProblemDesc * len(problems))(*[h.desc(max_columns=p.problem_count)
+                                                         for h, p in zip(self.hosts, problems)])
         self._descs = descs
         self.handle = C.c_void_p()
         _native.check(self.lib.jja_plan_create(len(problems), descs, C.byref(self.handle)))

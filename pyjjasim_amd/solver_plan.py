@@ -978,8 +978,9 @@ class CircuitPlan:
             arr[i].r0, arr[i].n_iter, arr[i].flags = r0, it, fl
         return arr
 
-    def desc(self):
-        """ctypes ProblemDesc pointing at this object's arrays (keep self alive)."""
+    def desc(self, max_columns=1):
+        """ctypes ProblemDesc pointing at this object's arrays (keep self alive),
+        for runs of at most ``max_columns`` columns."""
         sp = self.solve_plan
         P = _native.ptr
         d = _native.ProblemDesc()
@@ -987,6 +988,7 @@ class CircuitPlan:
         d.stencil_width, d.block_size = self.stencil_width, self.column_threads
         d.parts = self.parts
         d.column_pairs = int(getattr(self, "column_pairs", False))
+        d.max_columns = int(max_columns)
         d.Ic = P(self.Ic, C.c_double)
         d.Rv = P(self.Rv, C.c_double)
         d.coef = P(self.coef, C.c_double)

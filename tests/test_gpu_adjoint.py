@@ -118,3 +118,62 @@ def test_adjoint_global_face_store_bit_identical():
         grads[fs] = [t.grad.cpu().numpy() for t in leaves]
     for a, b in zip(grads["lds"], grads["global"]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_differentiation_workload_shape():
+    """The differentiation workload's own shape: SquareArray(64,64), beta_C = 1,
+    f = 0.1, W = 32 bias points 0.5..2.0, log-normal Ic (sigma 0.2), 300 steps
+    with checkpoints every 200. Gradients of <G, theta_stored> in Ic, f and Is
+    against autograd of the FP64 transcription (dense Cholesky solve, run with
+    torch's own kernels on the device) on every junction and face, to 1e-9 of
+    the largest component; the Ic gradient also against Richardson-extrapolated
+    central differences (h = 1e-4, 5e-5) of the GPU forward on 50 seeded
+    junctions."""
+    from adjoint_ref import transcribe
+    print("[adjoint] SquareArray(64,64) W=32, 300 steps", flush=True)
+    c = pa.SquareArray(64, 64)
+    c.set_capacitance(1.0)
+    Nj, Nf = c.junction_count(), c.face_count()
+    W, dt, Nt = 32, 0.05, 300
+    dev = "cuda"
+    Ic = torch.tensor(np.exp(0.2 * np.random.default_rng(101).standard_normal(Nj)), dtype=torch.float64, device=dev)
+    f = torch.full((Nf,), 0.1, dtype=torch.float64, device=dev)
+    Is = torch.tensor(c.current_base(0.0)[:, None] * np.linspace(0.5, 2.0, W)[None, :], dtype=torch.float64,
+                      device=dev)
+    store = np.zeros(Nt, dtype=bool)
+    store[[Nt // 2, Nt - 1]] = True
+    rng = np.random.default_rng(12)
+    G = torch.tensor(rng.standard_normal((Nj, W, 2)), dtype=torch.float64, device=dev)
+    with pa.DifferentiableEvolution(c, time_step=dt, time_step_count=Nt, problem_count=W,
+                                    store_time_steps=store, checkpoint_every=200) as evo:
+        leaves = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
+        th = evo(*leaves)
+        (th * G).sum().backward()
+        ref = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
+        th_ref = transcribe(c, dt, Nt, W, store, *ref, cholesky=True)
+        (th_ref * G).sum().backward()
+        err = (th - th_ref).abs().max().item() / th_ref.abs().max().item()
+        print(f"  theta: max rel err {err:.2e}", flush=True)
+        assert err < 1e-9
+        for name, a, b in zip(("Ic", "f", "Is"), leaves, ref):
+            e = (a.grad - b.grad).abs().max().item() / b.grad.abs().max().item()
+            print(f"  grad {name}: max rel err {e:.2e}", flush=True)
+            assert e < 1e-9, name
+        g = leaves[0].grad.cpu().numpy()
+
+        def loss(Ic_):
+            with torch.no_grad():
+                return (evo(Ic_, f, Is) * G).sum().item()
+
+        def central(j, h):
+            p, m = Ic.clone(), Ic.clone()
+            p[j] += h
+            m[j] -= h
+            return (loss(p) - loss(m)) / (2 * h)
+        idx = np.random.default_rng(303).choice(Nj, size=50, replace=False)
+        h = 1e-4
+        fd = np.array([(4.0 * central(j, h / 2) - central(j, h)) / 3.0 for j in idx])
+    scaled = np.abs(g[idx] - fd).max() / np.abs(g).max()
+    print(f"  finite differences on 50 junctions: max |adjoint - fd| / max|grad| = {scaled:.2e}", flush=True)
+    assert scaled < 1e-7

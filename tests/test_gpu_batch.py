@@ -117,3 +117,22 @@ def test_gpu_batch_autotune_short_horizon():
             eng.launch(3)
     finally:
         eng.close()
+
+
+def test_gpu_batch_autotune_keeps_reference_noise_stream():
+    """Autotuning launches draw from numpy's global RNG; the tuned run starts
+    from the caller's RNG state, so with noise="reference" at T > 0 a tuned
+    batch of one problem equals that problem's own compute() after the same
+    np.random.seed (with several problems the launches draw the problems'
+    normals chunk by chunk, an order of their own)."""
+    print("[batch] autotune with reference noise", flush=True)
+
+    def problem():
+        c = pa.SquareArray(7, 6)
+        Is = c.current_base(0.0)[:, None, None] * np.array([0.5, 1.1])[None, :, None]
+        return pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=120, external_flux=0.2,
+                                       current_sources=Is, temperature=0.02, noise="reference")
+    np.random.seed(1234)
+    batch = pa.TimeEvolutionBatch([problem()], autotune=True, block_sizes=(64, 128), tune_steps=10).compute()
+    np.random.seed(1234)
+    _same(batch[0], problem().compute())

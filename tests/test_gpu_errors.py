@@ -69,3 +69,79 @@ def test_gpu_missing_face_store_is_rejected():
         eng.launch(1)
     finally:
         eng.close()
+
+
+def test_gpu_plan_busy_on_another_stream_is_refused():
+    """The plan's launches share its device descriptors, so they run on one
+    stream at a time: while a long launch runs on stream A, a launch on stream
+    B is refused (JJA_EBUSY) before anything is enqueued; once A has finished,
+    B may launch, and the run equals one made on a single stream."""
+    print("[errors] busy plan on another stream", flush=True)
+    import torch
+
+    def problem():
+        c = pa.SquareArray(60, 60)
+        Is = c.current_base(0.0)[:, None, None] * np.linspace(0.5, 1.5, 64)[None, :, None]
+        return pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=4001, external_flux=0.2,
+                                       current_sources=Is, store_time_steps=[4000], store_voltage=False,
+                                       store_current=False)
+    p = problem()
+    eng = Engine(p, *_masks(p))
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    try:
+        eng.launch(4000, stream=sa.cuda_stream)
+        with pytest.raises(_native.NativeError, match="another stream"):
+            eng.launch(1, stream=sb.cuda_stream)
+        sa.synchronize()
+        eng.launch(1, stream=sb.cuda_stream)
+        sb.synchronize()
+        th = eng.results()[0]
+    finally:
+        eng.close()
+    ref = problem().compute()
+    np.testing.assert_array_equal(th[:, :, -1], ref.get_theta()[:, :, -1])
+
+
+def test_gpu_more_columns_than_planned_is_refused():
+    """A run with more columns than the plan was created for (its launch
+    descriptors are sized at creation) is refused with ValueError."""
+    print("[errors] more columns than planned", flush=True)
+    p = _problem(parts=1)
+    eng = Engine(p, *_masks(p))
+    try:
+        eng.run.W = p.problem_count + 1
+        with pytest.raises(ValueError, match="max_columns"):
+            eng.launch(1)
+    finally:
+        eng.close()
+
+
+def test_gpu_split_differentiable_evolution_reports_give_up():
+    """A split DifferentiableEvolution whose column barriers give up (wait limit
+    zero, every kernel still running to its end) raises NativeError from the
+    forward pass instead of returning invalid phases; with the default limit
+    its forward and gradient equal those of one workgroup per column."""
+    print("[errors] split DifferentiableEvolution give-up", flush=True)
+    import torch
+    c = pa.SquareArray(12, 10)
+    W, Nt = 2, 40
+    Nj, Nf = c.junction_count(), c.face_count()
+    Ic = torch.ones(Nj, dtype=torch.float64, device="cuda")
+    f = torch.full((Nf,), 0.1, dtype=torch.float64, device="cuda")
+    Is = torch.tensor(c.current_base(0.0)[:, None] * np.array([0.6, 1.3])[None, :], device="cuda")
+    grads = []
+    for parts in (4, 1):
+        with pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W,
+                                        store_time_steps=[Nt - 1], checkpoint_every=16, block_size=256,
+                                        face_store="global", parts=parts) as evo:
+            if parts > 1:
+                _native.check(evo.eng.lib.jja_plan_set_wait_limit(evo.eng.handle, 0))
+                with pytest.raises(_native.NativeError, match="gave up"):
+                    evo(Ic, f, Is)
+                _native.check(evo.eng.lib.jja_plan_set_wait_limit(evo.eng.handle, 1 << 21))
+            leaf = Ic.clone().requires_grad_(True)
+            th = evo(leaf, f, Is)
+            th.sum().backward()
+            grads.append((th.detach().cpu().numpy(), leaf.grad.cpu().numpy()))
+    np.testing.assert_array_equal(grads[0][0], grads[1][0])
+    np.testing.assert_array_equal(grads[0][1], grads[1][1])
