@@ -322,6 +322,21 @@ typedef struct jja_adjoint_run {
 int jja_adjoint(jja_plan* plan, const jja_adjoint_run* runs, int64_t k_hi, int64_t k_lo, void* stream);
 
 /*
+ * Solve K x = b for `cols` right-hand sides with the factor and solve program
+ * of problem i (K: the matrix its descriptor was built from; b and x in face
+ * order, [cols][n] with leading dimensions ldb, ldx >= n; DEVICE pointers).
+ * zbuf: [cols][n + max_stage_rows] doubles of device scratch when the problem
+ * has the global face store (jja_plan_zbuf_doubles > 0), else NULL. Replaces
+ * the factorize-once / solve-many pattern of scipy.sparse.linalg.factorized as
+ * the reference uses it for K (time_evolution.py:505-506) and for the cycle and
+ * cut-space systems A A^T and Mr Mr^T (josephson_circuit.py:93-128). A plan
+ * made only for solving has no junctions (n_junctions 0; jja_evolve refuses
+ * it). Not for plans whose columns are split over workgroups.
+ */
+int jja_solve(jja_plan* plan, int32_t i, const double* b, int64_t ldb, double* x, int64_t ldx, int32_t cols,
+              double* zbuf, void* stream);
+
+/*
  * Vortex configuration n[f] = -sum_j A[f,j] rint(theta_j / (2 pi))
  * (time_evolution.py:734-755) for `cols` phase vectors of problem p:
  * theta [cols][Nj] -> n [cols][Nf] (int32).

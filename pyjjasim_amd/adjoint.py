@@ -80,7 +80,8 @@ class DifferentiableEvolution:
             store_voltage=False, store_current=False, config_at_minus_1=config_at_minus_1,
             config_at_minus_2=config_at_minus_2, seed=seed if seed is not None else 0,
             steps_per_launch=self.C, block_size=block_size, face_store=face_store, parts=parts,
-            local_stages=local_stages, column_pairs=column_pairs)
+            local_stages=local_stages, column_pairs=column_pairs,
+            max_resident_output_bytes=1 << 62)   # stored phases stay on the device
         self.problem = prob
         self.th_mask = prob.store_time_steps.copy()
         self.eng = Engine(prob, self.th_mask, np.zeros(self.Nt, dtype=bool))

@@ -185,6 +185,9 @@ class Circuit:
         (Nj, Nj) dense or sparse matrix (josephson_circuit.py:595-604, 718-756)."""
         self.inductance, pd, self._has_inductance_v = self._prepare_inductance(inductance,
                                                                                self._Nj())
+        # a full matrix given as a dense array is written back dense by save(),
+        # as the reference keeps and saves it (josephson_circuit.py:650-658)
+        self._inductance_dense = (np.ndim(inductance) == 2 and not scipy.sparse.issparse(inductance))
         if not pd:
             raise ValueError("Inductance matrix not positive definite")
         return self
@@ -340,8 +343,10 @@ class Circuit:
             for arr in (x, y, n1, n2, self.critical_current, self.resistance, self.capacitance):
                 np.save(fh, arr)
             L = self.inductance
-            sparse = scipy.sparse.issparse(L)
+            sparse = not getattr(self, "_inductance_dense", False)
             np.save(fh, sparse)
+            if not sparse:
+                L = np.ascontiguousarray(L.toarray())
             if sparse:
                 L = scipy.sparse.csc_matrix(L)
                 np.save(fh, L.indptr)

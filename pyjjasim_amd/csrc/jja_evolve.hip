@@ -1271,6 +1271,39 @@ __global__ __launch_bounds__(BS) void vortex_kernel(const DevProblem* __restrict
   }
 }
 
+// K x = b for `cols` right-hand sides with problem p's solve program: one
+// workgroup per column loads b into the face store by solve position (and the
+// first stage's scratch, which equals b), runs the staged forward and backward
+// solves, and writes x back in face order. b and x are [cols][Nf] with leading
+// dimensions ldb and ldx.
+template <int BS, bool GLOBAL>
+__global__ __launch_bounds__(BS) void spd_solve_kernel(const DevProblem* __restrict__ probs, int32_t p,
+                                                       const double* __restrict__ b, int64_t ldb,
+                                                       double* __restrict__ x, int64_t ldx, double* zbuf) {
+  extern __shared__ double lds[];
+  const DevProblem& P = probs[p];
+  const int64_t col = blockIdx.x;
+  FaceStore<GLOBAL> zs;
+  if constexpr (GLOBAL)
+    zs.p = gblw(zbuf) + col * (static_cast<int64_t>(P.Nf) + P.max_stage_rows);
+  else
+    zs.p = lds;
+  zs.vt = threadIdx.x;
+  zs.vbs = BS;
+  const GAS double* __restrict__ bc = gbl(b) + col * ldb;
+  GAS double* __restrict__ xc = gblw(x) + col * ldx;
+  const GAS int32_t* __restrict__ fop = gbl(P.face_of_pos);
+  for (int32_t q = threadIdx.x; q < P.Nf; q += BS) {
+    const double v = bc[fop[q]];
+    zs.wr(q, v);
+    if (q < P.copy_rows) zs.wr(P.Nf + q, v);
+  }
+  zs.sync();
+  solve(P, zs);
+  zs.sync();
+  for (int32_t q = threadIdx.x; q < P.Nf; q += BS) xc[fop[q]] = zs.rd(q);
+}
+
 struct Plan {
   int device = 0;
   std::vector<void*> allocations;
@@ -1408,7 +1441,7 @@ int check_plan_shapes(int32_t n, const jja_problem_desc* descs) {
 // Every index the kernels follow is checked here, once, on the host.
 int validate(const jja_problem_desc& d) {
   const int32_t Nj = d.n_junctions, Nf = d.n_faces, BS = d.block_size, ms = d.max_stage_rows;
-  if (Nj <= 0 || Nf < 0 || d.stencil_width < 3 || d.stencil_width > 5 || d.n_face_slices < 0 ||
+  if (Nj < 0 || Nf < 0 || (Nj == 0 && Nf == 0) || d.stencil_width < 3 || d.stencil_width > 5 || d.n_face_slices < 0 ||
       d.n_solve_slices < 0 || ms < 0)
     return fail(JJA_EINVAL, "jja_plan_create: bad sizes");
   const int32_t parts = d.parts > 0 ? d.parts : 1;
@@ -1423,7 +1456,7 @@ int validate(const jja_problem_desc& d) {
                             "of at most 512 threads");
   if (d.max_columns < 1 || d.max_columns > (1 << 20))
     return fail(JJA_EINVAL, "jja_plan_create: max_columns must be 1 .. 2^20");
-  if (!d.Ic || !d.Rv || !d.coef || !d.junc_code)
+  if (Nj > 0 && (!d.Ic || !d.Rv || !d.coef || !d.junc_code))
     return fail(JJA_EINVAL, "jja_plan_create: missing junction arrays");
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
     if (d.junc_code[e] < -1 || (d.junc_code[e] >> 1) >= Nf)
@@ -1757,6 +1790,24 @@ int launch_adjoint_bs(Plan* plan, const WorkItem* work, int32_t n, int64_t lds, 
   }
 }
 
+template <int BS>
+int launch_spd_solve(Plan* plan, int32_t p, const double* b, int64_t ldb, double* x, int64_t ldx,
+                     int32_t cols, double* zbuf, hipStream_t stream) {
+  const bool global = plan->zbuf_doubles[p] > 0;
+  const int lds = global ? 8 : static_cast<int>(std::max<int64_t>(plan->lds_bytes[p], 8));
+  const void* kern = global ? reinterpret_cast<const void*>(spd_solve_kernel<BS, true>)
+                            : reinterpret_cast<const void*>(spd_solve_kernel<BS, false>);
+  if (int rc = set_lds_limit(plan, kern, lds)) return rc;
+  if (global)
+    hipLaunchKernelGGL((spd_solve_kernel<BS, true>), dim3(cols), dim3(BS), static_cast<size_t>(lds), stream,
+                       plan->d_problems, p, b, ldb, x, ldx, zbuf);
+  else
+    hipLaunchKernelGGL((spd_solve_kernel<BS, false>), dim3(cols), dim3(BS), static_cast<size_t>(lds), stream,
+                       plan->d_problems, p, b, ldb, x, ldx, zbuf);
+  JJA_HIP(hipGetLastError());
+  return JJA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1966,6 +2017,8 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
   int64_t lds = 0;
   for (int32_t i = 0; i < np; ++i) {
     const jja_problem_run& r = runs[i];
+    if (plan->problems[i].Nj == 0)
+      return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " is a matrix plan without junctions");
     if (r.W <= 0 || !r.theta || !r.u) return fail(JJA_EINVAL, "jja_evolve: run needs W > 0, theta and u");
     if (r.W > plan->max_cols[i])
       return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " has more columns than its max_columns");
@@ -2062,6 +2115,29 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
   }
   if (rc) return rc;
   return mark_launched(plan, stream);
+}
+
+int jja_solve(jja_plan* p, int32_t i, const double* b, int64_t ldb, double* x, int64_t ldx, int32_t cols,
+              double* zbuf, void* stream_ptr) {
+  Plan* plan = reinterpret_cast<Plan*>(p);
+  if (!plan || i < 0 || i >= static_cast<int32_t>(plan->problems.size()) || cols < 0 || (cols && (!b || !x)))
+    return fail(JJA_EINVAL, "jja_solve: invalid arguments");
+  const DevProblem& P = plan->problems[i];
+  if (P.parts != 1) return fail(JJA_EUNSUPPORTED, "jja_solve: the plan's columns are split over workgroups");
+  if (ldb < P.Nf || ldx < P.Nf) return fail(JJA_EINVAL, "jja_solve: leading dimension below n");
+  if (cols == 0 || P.Nf == 0) return JJA_OK;
+  const bool global = plan->zbuf_doubles[i] > 0;
+  if (global && !zbuf) return fail(JJA_EINVAL, "jja_solve: the problem needs a global face store (zbuf)");
+  if (!global && plan->lds_bytes[i] > plan->lds_limit)
+    return fail(JJA_EUNSUPPORTED, "jja_solve: the face vector exceeds LDS");
+  hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+  switch (plan->block_size) {
+    case 64: return launch_spd_solve<64>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
+    case 128: return launch_spd_solve<128>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
+    case 256: return launch_spd_solve<256>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
+    case 512: return launch_spd_solve<512>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
+    default: return launch_spd_solve<1024>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
+  }
 }
 
 int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_t* n_out,

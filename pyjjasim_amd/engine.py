@@ -108,6 +108,10 @@ def reference_noise(Nj, W):
     return draw
 
 
+# stored steps beyond this many bytes stream to the host during the run
+DEFAULT_RESIDENT_OUTPUT_BYTES = 16 << 30
+
+
 def default_block_size(Nj, Nf):
     work = max(Nj, Nf)
     for bs in (64, 128, 256, 512):
@@ -219,13 +223,31 @@ class Engine:
                 sig = sig[:, :1]
             self.sigma = DeviceSource(torch, sig[:, :, None], Nj, W, Nt, False)
         self.th_mask, self.I_mask = th_mask, I_mask
-        th_slot = np.where(th_mask, np.cumsum(th_mask) - 1, -1).astype(np.int32)
-        I_slot = np.where(I_mask, np.cumsum(I_mask) - 1, -1).astype(np.int32)
         self.n_th, self.n_I = int(th_mask.sum()), int(I_mask.sum())
+        # stored steps: all resident on the device, or (when they exceed the
+        # budget) streamed to the host through rings of output slots, each
+        # launch's slots copied out after it
+        budget = getattr(problem, "max_resident_output_bytes", None)
+        budget = DEFAULT_RESIDENT_OUTPUT_BYTES if budget is None else int(budget)
+        row = 8 * W * Nj
+        self.streaming = shared is None and (self.n_th + self.n_I) * row > budget
+        if self.streaming:
+            self.ring = max(1, budget // (2 * row))
+            self.th_host = np.empty((self.n_th, W, Nj))
+            self.I_host = np.empty((self.n_I, W, Nj))
+            self.n_host = np.empty((self.n_th, W, Nf), dtype=np.int32) if Nf else None
+            cap_th, cap_I = min(self.n_th, self.ring), min(self.n_I, self.ring)
+        else:
+            self.ring = max(self.n_th, self.n_I, 1)
+            cap_th, cap_I = self.n_th, self.n_I
+        self._th_rank = np.cumsum(th_mask)        # stored theta steps up to and including t
+        self._I_rank = np.cumsum(I_mask)
+        th_slot = np.where(th_mask, (self._th_rank - 1) % self.ring, -1).astype(np.int32)
+        I_slot = np.where(I_mask, (self._I_rank - 1) % self.ring, -1).astype(np.int32)
         self.th_slot = torch.from_numpy(th_slot).to(dev)
         self.I_slot = torch.from_numpy(I_slot).to(dev)
-        self.th_out = torch.empty((max(self.n_th, 1), W, Nj), dtype=f64, device=dev)
-        self.I_out = torch.empty((max(self.n_I, 1), W, Nj), dtype=f64, device=dev)
+        self.th_out = torch.empty((max(cap_th, 1), W, Nj), dtype=f64, device=dev)
+        self.I_out = torch.empty((max(cap_I, 1), W, Nj), dtype=f64, device=dev)
         run = self.run = _native.ProblemRun()
         run.W = W
         run.column_offset = 0
@@ -284,30 +306,78 @@ class Engine:
             torch.cuda.current_stream().synchronize()      # chunk buffers are reused
         self.t += n_steps
 
+    def _stored_between(self, rank, t0, t1):
+        """Ranks [r0, r1) of the stored steps in [t0, t1)."""
+        return (int(rank[t0 - 1]) if t0 > 0 else 0), (int(rank[t1 - 1]) if t1 > 0 else 0)
+
+    def _ring_limit(self, end):
+        """The longest launch from the current step (at most to `end`) whose
+        stored steps each fit in their ring without overwriting a slot that has
+        not been copied out."""
+        n = end - self.t
+        for rank in (self._th_rank, self._I_rank):
+            r0 = int(rank[self.t - 1]) if self.t > 0 else 0
+            # last step whose rank stays within r0 + ring
+            k = int(np.searchsorted(rank[self.t:end], r0 + self.ring, side="right"))
+            n = min(n, max(k, 1))
+        return n
+
+    def _flush(self, t0, t1):
+        """Copy the stored steps of [t0, t1) from their ring slots to the host
+        arrays, with their vortex counts (computed on the device first)."""
+        torch = self.torch
+        stream = torch.cuda.current_stream().cuda_stream
+        W = self.W
+        for rank, dev_out, host, with_n in ((self._th_rank, self.th_out, self.th_host, True),
+                                            (self._I_rank, self.I_out, self.I_host, False)):
+            r0, r1 = self._stored_between(rank, t0, t1)
+            r = r0
+            while r < r1:                       # ring segments [s0, s0 + cnt) without wrap-around
+                s0 = r % self.ring
+                cnt = min(r1 - r, self.ring - s0)
+                if with_n and self.n_host is not None:
+                    n_dev = torch.empty((cnt, W, self.Nf), dtype=torch.int32, device=self.dev)
+                    _native.check(self.lib.jja_vortex(self.handle, self.index, dev_out[s0].data_ptr(),
+                                                      cnt * W, n_dev.data_ptr(), stream))
+                    self.n_host[r:r + cnt] = n_dev.cpu().numpy()
+                host[r:r + cnt] = dev_out[s0:s0 + cnt].cpu().numpy()
+                r += cnt
+
     def advance(self, n_steps):
         if self.t + n_steps > self.Nt:
             raise ValueError("advance beyond time_step_count")
         end = self.t + n_steps
         while self.t < end:
-            self.launch(min(self.steps_per_launch, end - self.t))
+            n = min(self.steps_per_launch, end - self.t)
+            if self.streaming:
+                n = self._ring_limit(self.t + n)
+            t0 = self.t
+            self.launch(n)
+            if self.streaming:
+                self._flush(t0, self.t)
 
     def results(self):
         torch = self.torch
         Nj, W, S = self.Nj, self.W, self.S
         stream = torch.cuda.current_stream().cuda_stream
         n_dev = None
-        if self.n_th and self.Nf:
+        if self.n_th and self.Nf and not self.streaming:
             n_dev = torch.empty((self.n_th, W, self.Nf), dtype=torch.int32, device=self.dev)
             _native.check(self.lib.jja_vortex(self.handle, self.index, self.th_out.data_ptr(),
                                               self.n_th * W, n_dev.data_ptr(), stream))
         torch.cuda.synchronize()
         _native.check(self.lib.jja_plan_status(self.handle))
-        th = self.th_out[:self.n_th].cpu().numpy().transpose(2, 1, 0) if self.n_th \
-            else np.zeros((Nj, W, 0))
-        I = self.I_out[:self.n_I].cpu().numpy().transpose(2, 1, 0) if self.n_I \
-            else np.zeros((Nj, W, 0))
+        if self.streaming:
+            th_s, I_s = self.th_host, self.I_host
+        else:
+            th_s = self.th_out[:self.n_th].cpu().numpy() if self.n_th else None
+            I_s = self.I_out[:self.n_I].cpu().numpy() if self.n_I else None
+        th = th_s.transpose(2, 1, 0) if self.n_th else np.zeros((Nj, W, 0))
+        I = I_s.transpose(2, 1, 0) if self.n_I else np.zeros((Nj, W, 0))
         n = None
-        if n_dev is not None:
+        if self.streaming and self.n_th and self.Nf:
+            n = self.n_host.astype(np.int64).transpose(2, 1, 0)
+        elif n_dev is not None:
             n = n_dev.cpu().numpy().astype(np.int64).transpose(2, 1, 0)
         elif self.n_th:
             n = np.zeros((0, W, self.n_th), dtype=np.int64)

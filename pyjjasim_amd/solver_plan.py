@@ -1052,6 +1052,24 @@ class CircuitPlan:
                 cp = ok & (pos < self.face_copy_rows)
                 zs[Nf + pos[cp]] = acc[cp]
                 acc = np.zeros(BS)
+        return self._emulate_records(zs)
+
+    def emulate_solve(self, b):
+        """Exactly the solve kernel (jja_solve) on the host: b (Nf,) in face order
+        is loaded by solve position (the first stage's scratch too), the
+        per-wave record programs run as in emulate_face_and_solve, and K^-1 b
+        comes back in face order."""
+        Nf = self.Nf
+        zs = np.zeros(Nf + max(self.scratch_rows, 1))
+        pos = self.solve_plan.face_pos.astype(np.int64)
+        zs[pos] = b
+        cp = pos < self.face_copy_rows
+        zs[Nf + pos[cp]] = np.asarray(b)[cp]
+        return self._emulate_records(zs)
+
+    def _emulate_records(self, zs):
+        """The per-wave solve programs on the face store zs (emulate_face_and_solve)."""
+        BS, Nf = self.column_threads, self.Nf
         # split every wave's program at its column barriers into phases, and a
         # phase at its workgroup barriers into sub-phases: phase p of all waves
         # runs before phase p+1; within a phase, the waves of one part meet at
@@ -1138,3 +1156,47 @@ class CircuitPlan:
             for mem, wm in zip(part_mem, part_w):
                 zs[wm] = mem[wm]
         return zs[self.solve_plan.face_pos]
+
+
+class MatrixPlan(CircuitPlan):
+    """Factor and solve program of one sparse symmetric positive definite matrix
+    on its own (no circuit): the device solves K x = b for many right-hand
+    sides with it (jja_solve), as scipy's ``factorized(K)`` does on the host for
+    the reference's cycle and cut-space systems (josephson_circuit.py:93-128).
+    ``xy`` (n, 2): coordinates of the unknowns, for the nested-dissection
+    order. The descriptor has no junctions, so the time-evolution launch
+    refuses it."""
+
+    def __init__(self, K, xy, leaf_size=16, block_size=None, face_store="auto"):
+        # a private copy: index sorting during the ordering must not reorder
+        # arrays the caller's matrix shares
+        K = scipy.sparse.csc_matrix(K, dtype=np.float64, copy=True)
+        n = K.shape[0]
+        if K.shape != (n, n) or n == 0:
+            raise ValueError("MatrixPlan needs a non-empty square matrix")
+        if face_store not in ("auto", "lds", "global"):
+            raise ValueError("face_store must be 'auto', 'lds' or 'global'")
+        self.face_store = face_store
+        self.parts = 1
+        self.local_option = "auto"
+        self.column_pairs = False
+        self.Nj, self.Nf = 0, n
+        self.stencil_width = 3
+        self.dt = 1.0
+        if block_size is None:
+            block_size = 1024
+            for bs in (64, 128, 256, 512):
+                if n <= 2 * bs:
+                    block_size = bs
+                    break
+        self.block_size = block_size
+        self.face_ptr = np.zeros(n + 1, dtype=np.int32)
+        self.face_code = np.zeros(0, dtype=np.int32)
+        self.junc_code = np.zeros(0, dtype=np.int32)
+        self.coef = np.zeros((3, 0))
+        self.Rv = np.zeros(0)
+        self.Ic = np.zeros(0)
+        self.K = K
+        self.solve_plan = SolvePlan(K, np.asarray(xy, dtype=np.float64), leaf_size=leaf_size)
+        self.target_iter = 8
+        self._build_layouts(self.target_iter)

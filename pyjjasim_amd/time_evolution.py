@@ -42,6 +42,12 @@ Extensions (keyword-only, defaults keep reference behaviour):
                    and workgroups of 512 or 1024 threads. Default False: on
                    SquareArray(512,512) it measured the same speed at W=16 and
                    1% faster at W=32. Results do not depend on it.
+  max_resident_output_bytes
+                   Device memory for stored steps (default 16 GiB). When the
+                   stored theta and current exceed it, they stream to host
+                   memory during the run through rings of output slots (each
+                   launch's slots, with their vortex counts, are copied out
+                   after it). Results do not depend on it.
 
 There is no CPU fallback: without a HIP device ``compute()`` raises.
 """
@@ -131,7 +137,7 @@ class TimeEvolutionProblem:
                  config_at_minus_2=None, config_at_minus_3=None, config_at_minus_4=None,
                  stencil_width=3, *, noise="philox", seed=None, steps_per_launch=None,
                  block_size=None, leaf_size=16, face_store="auto", parts=None, local_stages="auto",
-                 column_pairs=False):
+                 column_pairs=False, max_resident_output_bytes=None):
         self.circuit = circuit
         self.time_step = time_step
         self.time_step_count = time_step_count
@@ -188,6 +194,7 @@ class TimeEvolutionProblem:
         self.parts = parts
         self.local_stages = local_stages
         self.column_pairs = bool(column_pairs)
+        self.max_resident_output_bytes = max_resident_output_bytes
 
     # ------------------------------------------------------------- getters
     def get_problem_count(self):
@@ -480,82 +487,77 @@ class TimeEvolutionResult:
         return self._select(select_time_points, self.get_circuit()._Nf(),
                             lambda tp: -A @ np.round(self._th(tp) / (2.0 * np.pi))).astype(int)
 
+    # Derived quantities (time_evolution.py:692-983) are evaluated on the
+    # device for all selected stored steps at once (result_queries.py); the
+    # exceptions are the reference's.
+    def _queries(self):
+        if getattr(self, "_dq", None) is None:
+            from .result_queries import DeviceQueries
+            self._dq = DeviceQueries(self)
+        return self._dq
+
+    def _stored(self, name, exc, message):
+        arr = getattr(self, name)
+        if arr is None:
+            raise exc(message)
+        return arr
+
     def get_phase(self, select_time_points=None):
-        c = self.get_circuit()
-        M, Nj = c.get_cut_matrix(), c._Nj()
-        try:
-            return self._select(select_time_points, c._Nn(),
-                                lambda tp: c.Msq_solve(M @ self._th(tp).reshape(Nj, -1)))
-        except ThetaNotStored:
-            raise ThetaNotStored("Cannot compute phi; requires theta to be stored in "
-                                 "TimeEvolutionConfig")
+        th = self._stored("theta", ThetaNotStored, "Cannot compute phi; requires theta to be stored in "
+                                                   "TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(self.get_circuit()._Nn(), lambda tps, idx: q.phase(th, tps, idx), select_time_points)
 
     def get_potential(self, select_time_points=None):
-        c = self.get_circuit()
-        M, Nj = c.get_cut_matrix(), c._Nj()
-        try:
-            return self._select(select_time_points, c._Nn(),
-                                lambda tp: c.Msq_solve(M @ self._V(tp).reshape(Nj, -1)))
-        except VoltageNotStored:
-            raise VoltageNotStored("Cannot compute electric potential U; requires voltage to be "
-                                   "stored in TimeEvolutionConfig")
+        V = self._stored("voltage", VoltageNotStored, "Cannot compute electric potential U; requires voltage "
+                                                      "to be stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(self.get_circuit()._Nn(), lambda tps, idx: q.potential(V, tps, idx), select_time_points)
 
     def get_josephson_energy(self, select_time_points=None):
-        try:
-            return self._select(select_time_points, self.get_circuit()._Nj(),
-                                lambda tp: self.problem._icp(self._th(tp)))
-        except ThetaNotStored:
-            raise ThetaNotStored("Cannot compute Josephson energy EJ; requires theta to be "
-                                 "stored in TimeEvolutionConfig")
+        th = self._stored("theta", ThetaNotStored, "Cannot compute Josephson energy EJ; requires theta to be "
+                                                   "stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(self.get_circuit()._Nj(), lambda tps, idx: q.josephson_energy(th, tps, idx),
+                        select_time_points)
 
     def get_supercurrent(self, select_time_points=None):
-        try:
-            return self._select(select_time_points, self.get_circuit()._Nj(),
-                                lambda tp: self.problem._cp(self._th(tp)))
-        except ThetaNotStored:
-            raise ThetaNotStored("Cannot compute supercurrent Isup; requires theta to be stored "
-                                 "in TimeEvolutionConfig")
+        th = self._stored("theta", ThetaNotStored, "Cannot compute supercurrent Isup; requires theta to be "
+                                                   "stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(self.get_circuit()._Nj(), lambda tps, idx: q.supercurrent(th, tps, idx),
+                        select_time_points)
 
     def get_cycle_current(self, select_time_points=None):
-        c = self.get_circuit()
-        A = c.get_cycle_matrix()
-        try:
-            return self._select(select_time_points, c._Nf(),
-                                lambda tp: c.Asq_solve(A @ (self._I(tp) - self.problem._Is(tp))))
-        except CurrentNotStored:
-            raise CurrentNotStored("Cannot compute cycle-current J; requires current to be "
-                                   "stored in TimeEvolutionConfig")
+        I = self._stored("current", CurrentNotStored, "Cannot compute cycle-current J; requires current to be "
+                                                      "stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(self.get_circuit()._Nf(), lambda tps, idx: q.cycle_current(I, tps, idx),
+                        select_time_points)
 
     def get_flux(self, select_time_points=None):
-        c = self.get_circuit()
-        A = c.get_cycle_matrix()
-        try:
-            return self._select(select_time_points, c._Nf(),
-                                lambda tp: self.problem._f(tp) + A @ (c._L() @ self._I(tp)) / (2 * np.pi))
-        except CurrentNotStored:
-            raise CurrentNotStored("Cannot compute magnetic flux; requires current to be stored "
-                                   "in TimeEvolutionConfig")
+        I = self._stored("current", CurrentNotStored, "Cannot compute magnetic flux; requires current to be "
+                                                      "stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(self.get_circuit()._Nf(), lambda tps, idx: q.flux(I, tps, idx), select_time_points)
 
     def get_magnetic_energy(self, select_time_points=None):
         c = self.get_circuit()
-        try:
-            return self._select(select_time_points, c._Nj(),
-                                lambda tp: 0.5 * c._L() @ (self._I(tp) ** 2),
-                                is_zero=not c._has_inductance())
-        except CurrentNotStored:
-            raise CurrentNotStored("Cannot compute magnetic energy EM; requires current to be "
-                                   "stored in TimeEvolutionConfig")
+        if not c._has_inductance():          # zeros, stored or not (time_evolution.py:885-888)
+            return self._select(select_time_points, c._Nj(), None, is_zero=True)
+        I = self._stored("current", CurrentNotStored, "Cannot compute magnetic energy EM; requires current to "
+                                                      "be stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(c._Nj(), lambda tps, idx: q.magnetic_energy(I, tps, idx), select_time_points)
 
     def get_capacitive_energy(self, select_time_points=None):
         c = self.get_circuit()
-        Cj = c._C()
-        try:
-            return self._select(select_time_points, c._Nj(),
-                                lambda tp: 0.5 * Cj[:, None] * self._V(tp) ** 2,
-                                is_zero=not c._has_capacitance())
-        except VoltageNotStored:
-            raise VoltageNotStored("Cannot compute capacitive energy EC; requires voltage to be "
-                                   "stored in TimeEvolutionConfig")
+        if not c._has_capacitance():         # zeros, stored or not (time_evolution.py:957-960)
+            return self._select(select_time_points, c._Nj(), None, is_zero=True)
+        V = self._stored("voltage", VoltageNotStored, "Cannot compute capacitive energy EC; requires voltage "
+                                                      "to be stored in TimeEvolutionConfig")
+        q = self._queries()
+        return q.select(c._Nj(), lambda tps, idx: q.capacitive_energy(V, tps, idx), select_time_points)
 
     def get_energy(self, select_time_points=None):
         return (self.get_josephson_energy(select_time_points) +

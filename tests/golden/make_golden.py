@@ -283,7 +283,125 @@ def make_trajectories(pj):
     print("trajectories:", sorted(cases))
 
 
+QUERY_GETTERS = ("get_phase", "get_potential", "get_josephson_energy", "get_magnetic_energy",
+                 "get_capacitive_energy", "get_energy", "get_supercurrent", "get_cycle_current", "get_flux")
+
+
+def make_queries(pj):
+    """Derived result quantities (time_evolution.py:692-983) of two runs with
+    inductance and capacitance: every getter at every stored step, and at a
+    selection of stored steps. Inputs as in make_trajectories (the same cases,
+    shorter horizons)."""
+    cases = {}
+    rng = np.random.default_rng(99)
+    sqL = pj.SquareArray(8, 8)
+    sqL.set_inductance(1.0)
+    sqL.set_capacitance(1.0)
+    Is = sqL.current_base(0.0)[:, None, None] * np.array([0.7, 1.3])[None, :, None]
+    ts = np.arange(0, 300, 30)
+    cases["sq8_L"] = (sqL, dict(Is=Is, ts=ts, f=0.1, L=1.0, C=1.0),
+                      dict(time_step=0.05, time_step_count=300, external_flux=0.1, current_sources=Is,
+                           store_time_steps=ts))
+    s = pj.SquareArray(7, 6)
+    Nj = s.junction_count()
+    Ic = np.exp(0.2 * rng.standard_normal(Nj))
+    R = 0.5 + rng.random(Nj)
+    C = 0.3 * rng.random(Nj)
+    s.set_critical_current(Ic)
+    s.set_resistance(R)
+    s.set_capacitance(C)
+    Is = s.current_base(0.0)[:, None, None] * np.array([0.4, 0.9, 1.6])[None, :, None]
+    f = np.array([0.0, 0.1, 0.33])[None, :, None]
+    ts = np.arange(0, 200, 20)
+    cases["sq76_hetero"] = (s, dict(Is=Is, f=f, ts=ts, Ic=Ic, R=R, C=C),
+                            dict(time_step=0.05, time_step_count=200, external_flux=f, current_sources=Is,
+                                 store_time_steps=ts))
+    flat = {}
+    for cname, (circuit, inputs, kw) in cases.items():
+        res = pj.TimeEvolutionProblem(circuit, **kw).compute()
+        sel = np.zeros(kw["time_step_count"], dtype=bool)
+        sel[kw["store_time_steps"][[1, 4, 7]]] = True
+        flat[f"{cname}__theta"] = res.get_theta()
+        flat[f"{cname}__current"] = res.get_current()
+        flat[f"{cname}__voltage"] = res.get_voltage()
+        flat[f"{cname}__select"] = sel
+        for g in QUERY_GETTERS:
+            flat[f"{cname}__{g}"] = np.asarray(getattr(res, g)())
+            flat[f"{cname}__{g}__sel"] = np.asarray(getattr(res, g)(sel))
+        for k, v in inputs.items():
+            flat[f"{cname}__in_{k}"] = np.asarray(v)
+    np.savez_compressed(os.path.join(HERE, "queries.npz"), **flat)
+    print("queries:", sorted(cases))
+
+
+def _general_graphs(pj):
+    """The custom graph (unsorted, partly reversed edges), the same graph with a
+    dangling junction, a square array with a hole, and the SQUID."""
+    x = [0, 1, 2, 0, 1, 2, 0, 1, 2]
+    y = [0, 0, 0, 1, 1, 1, 2, 2, 2]
+    n1 = [4, 2, 7, 0, 5, 6, 4, 1, 8, 7, 7]
+    n2 = [3, 1, 6, 3, 2, 3, 5, 0, 5, 4, 8]
+    out = {"custom": pj.Circuit(pj.EmbeddedGraph(x, y, n1, n2)),
+           "custom_dangling": pj.Circuit(pj.EmbeddedGraph(x + [0.4], y + [1.6], n1 + [9], n2 + [3]))}
+    full = pj.SquareArray(8, 8)
+    fx, fy = full.get_node_coordinates()
+    out["hole_8"] = full.remove_nodes((fx > 2) & (fx < 5) & (fy > 2) & (fy < 5))
+    out["squid"] = pj.SQUID()
+    return out
+
+
+def make_graph_trajectories(pj):
+    """End-to-end runs on graphs that are not lattices: a current injected at the
+    lowest-left node and taken out at the highest-right one (node_to_junction_
+    current, static_problem.py:1174-1192) at two amplitudes, flux 0.2, capacitance
+    0.5; theta, current, voltage and vortices at every 10th of 400 steps."""
+    flat = {}
+    for name, c in _general_graphs(pj).items():
+        x, y = c.get_node_coordinates()
+        node = np.zeros(c.node_count())
+        node[np.argmin(x + y)] = 1.0
+        node[np.argmax(x + y)] = -1.0
+        base = pj.node_to_junction_current(c, node)
+        c.set_capacitance(0.5)
+        Is = base[:, None, None] * np.array([0.8, 1.9])[None, :, None]
+        ts = np.arange(0, 400, 10)
+        rec = run_case(pj, name, c, dict(Is=Is, ts=ts, f=0.2, C=0.5, base=base), time_step=0.05,
+                       time_step_count=400, external_flux=0.2, current_sources=Is, store_time_steps=ts)
+        for k, v in rec.items():
+            flat[f"{name}__{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "graph_trajectories.npz"), **flat)
+    print("graph trajectories:", sorted(_general_graphs(pj)))
+
+
+def make_circuit_files(pj):
+    """Circuit files written by the reference's Circuit.save
+    (josephson_circuit.py:643-663): a lattice with heterogeneous Ic, R, C and a
+    sparse mutual inductance, and a custom graph with a dense inductance matrix."""
+    import scipy.sparse
+    rng = np.random.default_rng(5)
+    c = pj.SquareArray(5, 4)
+    Nj = c.junction_count()
+    c.set_critical_current(np.exp(0.2 * rng.standard_normal(Nj)))
+    c.set_resistance(0.5 + rng.random(Nj))
+    c.set_capacitance(rng.random(Nj))
+    i = np.arange(Nj - 1)
+    L = scipy.sparse.diags(np.full(Nj, 0.9)) + scipy.sparse.coo_matrix(
+        (np.full(Nj - 1, 0.05), (i, i + 1)), shape=(Nj, Nj)) + scipy.sparse.coo_matrix(
+        (np.full(Nj - 1, 0.05), (i + 1, i)), shape=(Nj, Nj))
+    c.set_inductance(scipy.sparse.csc_matrix(L))
+    c.save(os.path.join(HERE, "circuit_sq5x4_mutual.npy"))
+    g = _general_graphs(pj)["custom"]
+    Nj = g.junction_count()
+    Ld = 0.3 * np.eye(Nj) + 0.02 * (np.ones((Nj, Nj)) - np.eye(Nj))
+    g.set_inductance(Ld)
+    g.set_critical_current(1.0 + 0.1 * np.arange(Nj))
+    g.save(os.path.join(HERE, "circuit_custom_dense.npy"))
+    print("circuit files written")
+
+
 if __name__ == "__main__":
     pj = _import_reference()
-    make_graphs(pj)
-    make_trajectories(pj)
+    parts = sys.argv[1:] or ["graphs", "trajectories", "queries", "graph_trajectories", "circuit_files"]
+    for part in parts:
+        {"graphs": make_graphs, "trajectories": make_trajectories, "queries": make_queries,
+         "graph_trajectories": make_graph_trajectories, "circuit_files": make_circuit_files}[part](pj)

@@ -1,10 +1,15 @@
 """Planar graph builder vs the reference's face order, orientation and bias vectors
 (golden fixtures from tests/golden/make_golden.py) and closed forms."""
+import os
+
+import scipy.sparse
 import numpy as np
 import pytest
 
 import pyjjasim_amd as pa
 from cases import graphs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 X9 = [0, 1, 2, 0, 1, 2, 0, 1, 2]
 Y9 = [0, 0, 0, 1, 1, 1, 2, 2, 2]
@@ -113,3 +118,38 @@ def test_save_load_roundtrip(tmp_path):
     assert np.array_equal(np.asarray(d.get_cycle_matrix().todense()),
                           np.asarray(c.get_cycle_matrix().todense()))
     assert abs(d.get_inductance() - c.get_inductance()).max() == 0
+
+
+@pytest.mark.parametrize("name", ["circuit_sq5x4_mutual.npy", "circuit_custom_dense.npy"])
+def test_circuit_load_reads_reference_files(name, tmp_path):
+    """Circuit files written by the reference's Circuit.save
+    (josephson_circuit.py:643-663): a lattice with heterogeneous Ic, R, C and a
+    sparse mutual inductance, and a custom graph with a dense inductance
+    matrix. Circuit.load reads the graph, the junction parameters and L, the
+    loaded circuit's A and M equal the reference's for the same graph, and
+    saving it again writes the reference's bytes."""
+    path = os.path.join(HERE, "golden", name)
+    c = pa.Circuit.load(path)
+    with open(path, "rb") as fh:
+        recs = [np.load(fh) for _ in range(8)]
+        x, y, n1, n2, Ic, R, C, sparse = recs
+        if bool(sparse):
+            indptr, indices, data = np.load(fh), np.load(fh), np.load(fh)
+            L = scipy.sparse.csc_matrix((data, indices, indptr), shape=(len(n1), len(n1))).toarray()
+        else:
+            L = np.load(fh)
+    cx, cy = c.get_node_coordinates()
+    np.testing.assert_array_equal(cx, x)
+    np.testing.assert_array_equal(cy, y)
+    c1, c2 = c.get_junction_nodes()
+    np.testing.assert_array_equal(c1, n1)
+    np.testing.assert_array_equal(c2, n2)
+    np.testing.assert_array_equal(c.get_critical_current(), Ic)
+    np.testing.assert_array_equal(c.get_resistance(), R)
+    np.testing.assert_array_equal(c.get_capacitance(), C)
+    np.testing.assert_array_equal(np.asarray(c.get_inductance().todense()), L)
+    g = pa.Circuit(pa.EmbeddedGraph(x, y, n1, n2))
+    assert (c.get_cycle_matrix() != g.get_cycle_matrix()).nnz == 0
+    out = tmp_path / "again.npy"
+    c.save(str(out))
+    assert out.read_bytes() == open(path, "rb").read()
