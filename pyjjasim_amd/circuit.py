@@ -338,37 +338,59 @@ class Circuit:
         """Sequential ``np.save`` records, the reference's circuit file layout
         (josephson_circuit.py:643-663)."""
         with open(filename, "wb") as fh:
-            x, y = self.graph.coo()
-            n1, n2 = self.graph.get_edges()
-            for arr in (x, y, n1, n2, self.critical_current, self.resistance, self.capacitance):
-                np.save(fh, arr)
-            L = self.inductance
-            sparse = not getattr(self, "_inductance_dense", False)
-            np.save(fh, sparse)
-            if not sparse:
-                L = np.ascontiguousarray(L.toarray())
-            if sparse:
-                L = scipy.sparse.csc_matrix(L)
-                np.save(fh, L.indptr)
-                np.save(fh, L.indices)
-                np.save(fh, L.data)
-            else:
-                np.save(fh, L)
+            self._save_records(fh)
+
+    def _save_records(self, fh):
+        x, y = self.graph.coo()
+        n1, n2 = self.graph.get_edges()
+        for arr in (x, y, n1, n2, self.critical_current, self.resistance, self.capacitance):
+            np.save(fh, arr)
+        L = self.inductance
+        sparse = not getattr(self, "_inductance_dense", False)
+        np.save(fh, sparse)
+        if not sparse:
+            L = np.ascontiguousarray(L.toarray())
+        if sparse:
+            L = scipy.sparse.csc_matrix(L)
+            np.save(fh, L.indptr)
+            np.save(fh, L.indices)
+            np.save(fh, L.data)
+        else:
+            np.save(fh, L)
 
     @staticmethod
     def load(filename):
         """Read a file written by ``save`` (josephson_circuit.py:665-684)."""
         with open(filename, "rb") as fh:
-            x, y, n1, n2, Ic, R, C = (np.load(fh) for _ in range(7))
-            sparse = bool(np.load(fh))
-            if sparse:
-                indptr, indices, data = np.load(fh), np.load(fh), np.load(fh)
-                Nj = len(n1)
-                L = scipy.sparse.csc_matrix((data, indices, indptr), shape=(Nj, Nj))
-            else:
-                L = np.load(fh)
+            return Circuit._load_records(fh)
+
+    @staticmethod
+    def _load_records(fh):
+        x, y, n1, n2, Ic, R, C = (np.load(fh) for _ in range(7))
+        sparse = bool(np.load(fh))
+        if sparse:
+            indptr, indices, data = np.load(fh), np.load(fh), np.load(fh)
+            Nj = len(n1)
+            L = scipy.sparse.csc_matrix((data, indices, indptr), shape=(Nj, Nj))
+        else:
+            L = np.load(fh)
         return Circuit(EmbeddedGraph(x, y, n1, n2), critical_current=Ic, resistance=R,
                        capacitance=C, inductance=L)
+
+    def _A_solve(self, b):
+        """A x = b with x integral for integral b (josephson_circuit.py:782-801):
+        the graph's integral cycle-space solve."""
+        return self.graph._cycle_space_solve_for_integral_x(b)
+
+    def _has_identical_critical_current(self):
+        Ic = self.get_critical_current()
+        return bool(np.allclose(Ic, Ic[0]))
+
+    def _has_only_identical_self_inductance(self):
+        if self._has_only_self_inductance():
+            L = self.inductance.diagonal()
+            return bool(np.allclose(L, L[0]))
+        return False
 
     def __str__(self):
         return f"Circuit(Nn={self._Nn()}, Nj={self._Nj()}, Nf={self._Nf()})"

@@ -171,6 +171,7 @@ class EmbeddedGraph:
                 break
             lab = new
             jump = jump[jump]
+        self._lab = lab
         labels, inverse, lengths = np.unique(lab, return_inverse=True, return_counts=True)
         tail = np.concatenate((self._lo, self._hi))
         head = np.concatenate((self._hi, self._lo))
@@ -200,6 +201,115 @@ class EmbeddedGraph:
 
     def face_count(self):
         return len(self._compute_faces()[0]) - 1
+
+    # ------------------------------------------------- l-cycles (with boundary)
+    def _l_cycles(self):
+        """Every cycle of the face walk, boundary cycles included, in the
+        reference's l-cycle order (embedded_graph.py:987-1033: length, then the
+        smallest directed edge id on the cycle). Returns (l-cycle of every
+        directed edge, the cycles' directed edges in walk order from their
+        smallest id as (ptr, edges), indices of the boundary cycles: area not
+        positive, embedded_graph.py:913-916)."""
+        if getattr(self, "_lcyc", None) is not None:
+            return self._lcyc
+        self._compute_faces()
+        E = self.E
+        nxt = self._successor()
+        labels, inverse, lengths = np.unique(self._lab, return_inverse=True, return_counts=True)
+        order = np.lexsort((labels, lengths))
+        rank = np.empty(len(labels), dtype=np.int64)
+        rank[order] = np.arange(len(labels))
+        cyc_of_dir = rank[inverse]
+        tail = np.concatenate((self._lo, self._hi))
+        head = np.concatenate((self._hi, self._lo))
+        det = self.x[tail] * self.y[head] - self.x[head] * self.y[tail]
+        area = 0.5 * np.bincount(cyc_of_dir, weights=det, minlength=len(labels))
+        boundary = np.flatnonzero((area < 0) | np.isclose(area, 0.0))
+        f_len = lengths[order]
+        ptr = np.concatenate(([0], np.cumsum(f_len)))
+        edges = np.empty(2 * E, dtype=np.int64)
+        cur = labels[order].copy()
+        active = np.arange(len(order))
+        step = 0
+        while len(active):
+            edges[ptr[active] + step] = cur
+            step += 1
+            alive = f_len[active] > step
+            active = active[alive]
+            cur = nxt[cur[alive]]
+        self._lcyc = (cyc_of_dir, ptr, edges, boundary)
+        return self._lcyc
+
+    def l_cycle_matrix(self):
+        """(L, E) cycle matrix of all l-cycles, boundary included, one entry per
+        traversed edge in walk order (embedded_graph.py:505-523)."""
+        _, ptr, edges, _ = self._l_cycles()
+        E = self.E
+        cols = self._order[edges % E]
+        data = np.where(edges < E, 1, -1)
+        data = np.where(self._flip[cols], -data, data)
+        return scipy.sparse.csr_matrix((data, cols, ptr), shape=(len(ptr) - 1, E)).tocsc()
+
+    def get_common_edge_of_l_cycles(self, cycle1, cycle2, return_orientation=False):
+        """An edge both l-cycles contain, the lowest in sorted edge order, or -1
+        (embedded_graph.py:569-612); with return_orientation, whether cycle1
+        passes it counter-clockwise from its lowest-index node."""
+        cyc_of_dir = self._l_cycles()[0]
+        E = self.E
+        f1, f2 = cyc_of_dir[:E], cyc_of_dir[E:2 * E]
+        mask = f1 < f2
+        lo, hi = np.where(mask, f1, f2), np.where(mask, f2, f1)
+        sorter = np.lexsort((hi, lo))
+        key = lo[sorter] * (len(cyc_of_dir) + 1) + hi[sorter]
+        c1 = np.asarray(cycle1, dtype=np.int64)
+        c2 = np.asarray(cycle2, dtype=np.int64)
+        shape = c1.shape
+        c1, c2 = c1.ravel(), c2.ravel()
+        mask2 = c1 < c2
+        fmin, fmax = np.where(mask2, c1, c2), np.where(mask2, c2, c1)
+        pos = np.searchsorted(key, fmin * (len(cyc_of_dir) + 1) + fmax)
+        pos[pos >= E] = E - 1
+        found = key[pos] == fmin * (len(cyc_of_dir) + 1) + fmax
+        orient = mask[sorter][pos] ^ ~mask2
+        ids = self._order[sorter[pos]]
+        ids = np.where(found, ids, -1).reshape(shape)
+        if return_orientation:
+            orient = np.where(found, orient, False).reshape(shape)
+            return ids, orient ^ np.where(ids >= 0, self._flip[np.maximum(ids, 0)], False)
+        return ids
+
+    def _cycle_space_solve_for_integral_x(self, b):
+        """A x = b (A: the face cycle matrix) with x integral when b is
+        (embedded_graph.py:1041-1090): a depth-first tree of the dual graph from
+        the boundary cycle; each face's tally passes to its parent across their
+        common edge, leaves first."""
+        if self.get_num_components() != 1 or not self.is_planar_embedding():
+            raise ValueError("only implemented for single component planar embedding")
+        _, ptr, _, boundary = self._l_cycles()
+        E, F = self.E, self.face_count()
+        b = np.array(b, dtype=np.double)
+        shape = list(b.shape)
+        b = b.reshape(-1, F)
+        b_idx = int(boundary[0])
+        tally = np.concatenate((b[:, :b_idx], np.zeros((b.shape[0], 1)), b[:, b_idx:]), axis=1)
+        Al = self.l_cycle_matrix()
+        cur, pred = scipy.sparse.csgraph.depth_first_order(Al @ Al.T, b_idx)
+        prev = pred[cur]
+        prev[0] = -1
+        juncs, orient = self.get_common_edge_of_l_cycles(cur, prev, return_orientation=True)
+        sgn = -1 + 2 * orient.astype(np.double)
+        x = np.zeros((b.shape[0], E))
+        for i in reversed(range(F + 1)):
+            if prev[i] >= 0:
+                tally[:, prev[i]] += tally[:, cur[i]]
+                x[:, juncs[i]] += tally[:, cur[i]] * sgn[i]
+                tally[:, cur[i]] = 0
+        keep = np.ones(len(ptr) - 1, dtype=bool)
+        keep[boundary] = False
+        if not np.allclose((Al @ x.T)[keep, :], b.T):
+            raise ValueError("failed integral solve of cycle space linear problem")
+        shape[-1] = E
+        return x.reshape(tuple(shape))
 
     def l_cycle_count(self):
         return int(self._compute_faces()[3])

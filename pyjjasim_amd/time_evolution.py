@@ -200,6 +200,15 @@ class TimeEvolutionProblem:
     def get_problem_count(self):
         return self.problem_count
 
+    def get_static_problem(self, vortex_configuration, problem_nr=0, time_step=0):
+        """StaticProblem with this problem's sources at one column and step
+        (time_evolution.py:151-171)."""
+        from .static_problem import StaticProblem
+        return StaticProblem(self.circuit, current_sources=self._Is(time_step)[:, problem_nr].copy(),
+                             external_flux=self._f(time_step)[:, problem_nr].copy(),
+                             vortex_configuration=vortex_configuration,
+                             current_phase_relation=self.current_phase_relation)
+
     def get_circuit(self) -> Circuit:
         return self.circuit
 

@@ -354,9 +354,13 @@ def make_graph_trajectories(pj):
     """End-to-end runs on graphs that are not lattices: a current injected at the
     lowest-left node and taken out at the highest-right one (node_to_junction_
     current, static_problem.py:1174-1192) at two amplitudes, flux 0.2, capacitance
-    0.5; theta, current, voltage and vortices at every 10th of 400 steps."""
+    0.5; theta, current, voltage and vortices at every 10th of 400 steps. The
+    SQUID's vertical junctions (Ic = 1000) make the explicit current-phase term
+    stiff: its run takes dt = 0.01 (at 0.05 the scheme amplifies rounding
+    without bound)."""
     flat = {}
     for name, c in _general_graphs(pj).items():
+        dt = 0.01 if name == "squid" else 0.05
         x, y = c.get_node_coordinates()
         node = np.zeros(c.node_count())
         node[np.argmin(x + y)] = 1.0
@@ -365,7 +369,7 @@ def make_graph_trajectories(pj):
         c.set_capacitance(0.5)
         Is = base[:, None, None] * np.array([0.8, 1.9])[None, :, None]
         ts = np.arange(0, 400, 10)
-        rec = run_case(pj, name, c, dict(Is=Is, ts=ts, f=0.2, C=0.5, base=base), time_step=0.05,
+        rec = run_case(pj, name, c, dict(Is=Is, ts=ts, f=0.2, C=0.5, base=base, dt=dt), time_step=dt,
                        time_step_count=400, external_flux=0.2, current_sources=Is, store_time_steps=ts)
         for k, v in rec.items():
             flat[f"{name}__{k}"] = v
@@ -399,9 +403,116 @@ def make_circuit_files(pj):
     print("circuit files written")
 
 
+def _graph_set(pj):
+    """The graphs of make_graphs, by name."""
+    out = {"sq_3x4": pj.SquareArray(3, 4), "sq_10x10": pj.SquareArray(10, 10),
+           "hc_3x3": pj.HoneycombArray(3, 3), "hc_6x6": pj.HoneycombArray(6, 6),
+           "tri_4x3": pj.TriangularArray(4, 3), "tri_6x6": pj.TriangularArray(6, 6)}
+    out.update(_general_graphs(pj))
+    return out
+
+
+def make_integral_solves(pj):
+    """The integral cycle-space solve A x = b (embedded_graph.py:1041-1090) on
+    every graph, for three seeded integer right-hand sides."""
+    flat = {}
+    rng = np.random.default_rng(8)
+    for name, c in _graph_set(pj).items():
+        b = rng.integers(-3, 4, size=(3, c.face_count())).astype(np.double)
+        flat[f"{name}__b"] = b
+        flat[f"{name}__x"] = c.graph._cycle_space_solve_for_integral_x(b)
+    np.savez_compressed(os.path.join(HERE, "integral_solve.npz"), **flat)
+    print("integral solves:", sorted(_graph_set(pj)))
+
+
+def _static_cases(pj):
+    """Static problems: (circuit, Is, f, n) by name. The reference's Newton
+    converges on the first four and stops on the last two (a bias above the
+    critical current; a vortex-antivortex pair that does not survive)."""
+    out = {}
+    c = pj.SquareArray(6, 6)
+    out["sq6_bias"] = (c, 0.3 * c.current_base(0.0), 0.1, 0)
+    c = pj.SquareArray(7, 7)
+    c.set_inductance(0.5)
+    n = np.zeros(c.face_count(), dtype=int)
+    n[c.face_count() // 2] = 1
+    out["sq7_L_vortex"] = (c, 0.0, 0.05, n)
+    c = pj.HoneycombArray(5, 5)
+    rng = np.random.default_rng(31)
+    c.set_critical_current(np.exp(0.2 * rng.standard_normal(c.junction_count())))
+    out["hc5_hetero"] = (c, 0.2 * c.current_base(0.0), 0.05, 0)
+    c = pj.TriangularArray(5, 5)
+    c.set_inductance(0.4 + 0.3 * rng.random(c.junction_count()))
+    out["tri5_selfL"] = (c, 0.1 * c.current_base(np.pi / 2), 0.02, 0)
+    c = pj.SquareArray(6, 6)
+    out["sq6_overbias"] = (c, 2.5 * c.current_base(0.0), 0.0, 0)
+    c = pj.SquareArray(8, 8)
+    n = np.zeros(c.face_count(), dtype=int)
+    n[[20, 21]] = [1, -1]
+    out["sq8_pair"] = (c, 0.0, 0.0, n)
+    return out
+
+
+STATIC_GETTERS = ("get_phase", "get_current", "get_cycle_current", "get_flux", "get_magnetic_energy",
+                  "get_josephson_energy", "get_energy", "get_vortex_configuration")
+
+
+def make_static(pj):
+    """StaticProblem.approximate() and .compute() (static_problem.py:550-607,
+    1385-1515) on six problems: the London approximation, the Newton result
+    (theta, status, iterations, residual history), the configuration's
+    quantities, errors and stability; plus a problem file written by
+    StaticProblem.save (static_problem.py:433-457)."""
+    flat = {}
+    for name, (c, Is, f, n) in _static_cases(pj).items():
+        prob = pj.StaticProblem(c, current_sources=Is, external_flux=f, vortex_configuration=n)
+        appr = prob.approximate()
+        cfg, status, info = prob.compute()
+        flat[f"{name}__approx_theta"] = appr.get_theta()
+        flat[f"{name}__theta"] = cfg.get_theta()
+        flat[f"{name}__status"] = np.array(status)
+        flat[f"{name}__iterations"] = np.array(info.get_number_of_iterations())
+        flat[f"{name}__residual"] = info.get_residual()
+        flat[f"{name}__is_target_n"] = info.get_is_target_vortex_configuration()
+        flat[f"{name}__is_stable"] = np.array(cfg.is_stable())
+        flat[f"{name}__error"] = np.array(cfg.get_error())
+        for g in STATIC_GETTERS:
+            flat[f"{name}__{g}"] = np.asarray(getattr(cfg, g)())
+        flat[f"{name}__in_Is"] = np.broadcast_to(np.asarray(Is, dtype=np.double), (c.junction_count(),)).copy()
+        flat[f"{name}__in_f"] = np.asarray(f, dtype=np.double)
+        flat[f"{name}__in_n"] = np.broadcast_to(np.asarray(n), (c.face_count(),)).copy()
+        if name == "sq7_L_vortex":
+            prob.save(os.path.join(HERE, "static_problem_sq7_L_vortex.npy"))
+    np.savez_compressed(os.path.join(HERE, "static.npz"), **flat)
+    print("static:", sorted(_static_cases(pj)))
+
+
+def make_annealing(pj):
+    """AnnealingProblem.compute() (time_evolution.py:1070-1191) with numpy's
+    global RNG seeded: temperature profiles, final configurations and status of
+    two runs (square and honeycomb lattices, two columns each)."""
+    flat = {}
+    cases = {"sq6": (pj.SquareArray(6, 6), 0.2, 0.0, 25), "hc4": (pj.HoneycombArray(4, 4), 0.1, 0.0, 20)}
+    for name, (c, f, Is, count) in cases.items():
+        ap = pj.AnnealingProblem(c, time_step=0.5, interval_steps=10, external_flux=f, current_sources=Is,
+                                 problem_count=2, interval_count=count, vortex_mobility=0.001, start_T=0.5,
+                                 T_factor=1.03)
+        np.random.seed(2024)
+        status, configs, profiles = ap.compute()
+        flat[f"{name}__status"] = status
+        flat[f"{name}__profiles"] = profiles
+        flat[f"{name}__theta"] = np.stack([cfg.get_theta() for cfg in configs], axis=1)
+        flat[f"{name}__n"] = np.stack([cfg.get_vortex_configuration() for cfg in configs], axis=1)
+    np.savez_compressed(os.path.join(HERE, "annealing.npz"), **flat)
+    print("annealing:", sorted(cases))
+
+
 if __name__ == "__main__":
     pj = _import_reference()
-    parts = sys.argv[1:] or ["graphs", "trajectories", "queries", "graph_trajectories", "circuit_files"]
+    parts = sys.argv[1:] or ["graphs", "trajectories", "queries", "graph_trajectories", "circuit_files",
+                             "integral_solves", "static", "annealing"]
     for part in parts:
         {"graphs": make_graphs, "trajectories": make_trajectories, "queries": make_queries,
-         "graph_trajectories": make_graph_trajectories, "circuit_files": make_circuit_files}[part](pj)
+         "graph_trajectories": make_graph_trajectories, "circuit_files": make_circuit_files,
+         "integral_solves": make_integral_solves, "static": make_static,
+         "annealing": make_annealing}[part](pj)

@@ -90,7 +90,8 @@ def test_gpu_result_query_errors():
 def test_gpu_general_graph_trajectories_match_reference(case):
     """Time evolution on graphs that are not lattices (a custom graph with
     reversed edges, the same with a dangling junction, a lattice with a hole, the
-    SQUID), with a current injected at one node and taken out at another:
+    SQUID, at a time step short enough for its Ic = 1000 junctions), with a
+    current injected at one node and taken out at another:
     theta, current and voltage within 1e-9 of the reference's largest value and
     vortex counts exact, at every 10th of 400 steps."""
     g = _load("graph_trajectories.npz")[case]
@@ -115,7 +116,7 @@ def test_gpu_general_graph_trajectories_match_reference(case):
     base = pa.node_to_junction_current(c, node)
     np.testing.assert_allclose(base, g["in_base"], rtol=0, atol=1e-13)
     c.set_capacitance(0.5)
-    res = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=400, external_flux=0.2,
+    res = pa.TimeEvolutionProblem(c, time_step=float(g["in_dt"]), time_step_count=400, external_flux=0.2,
                                   current_sources=g["in_Is"], store_time_steps=g["in_ts"]).compute()
     print(f"[graphs] {case}: Nj={c.junction_count()} Nf={c.face_count()}", flush=True)
     _close(res.get_theta(), g["theta"], "theta")

@@ -153,3 +153,43 @@ def test_circuit_load_reads_reference_files(name, tmp_path):
     out = tmp_path / "again.npy"
     c.save(str(out))
     assert out.read_bytes() == open(path, "rb").read()
+
+
+def _named_graphs():
+    full = pa.SquareArray(8, 8)
+    fx, fy = full.get_node_coordinates()
+    return {"sq_3x4": pa.SquareArray(3, 4), "sq_10x10": pa.SquareArray(10, 10), "hc_3x3": pa.HoneycombArray(3, 3),
+            "hc_6x6": pa.HoneycombArray(6, 6), "tri_4x3": pa.TriangularArray(4, 3),
+            "tri_6x6": pa.TriangularArray(6, 6), "custom": pa.Circuit(pa.EmbeddedGraph(X9, Y9, N1, N2)),
+            "custom_dangling": pa.Circuit(pa.EmbeddedGraph(X9 + [0.4], Y9 + [1.6], N1 + [9], N2 + [3])),
+            "hole_8": full.remove_nodes((fx > 2) & (fx < 5) & (fy > 2) & (fy < 5)), "squid": pa.SQUID()}
+
+
+def test_integral_cycle_space_solve_matches_reference():
+    """A x = b with integral x (embedded_graph.py:1041-1090; the phase-zone
+    change of static problems): the same x as the reference for three integer
+    right-hand sides on ten graphs (lattices, a custom graph, a dangling edge, a
+    hole, the SQUID); and A x = b holds."""
+    d = np.load(os.path.join(HERE, "golden", "integral_solve.npz"), allow_pickle=False)
+    for name, c in _named_graphs().items():
+        b = d[name + "__b"]
+        x = c.graph._cycle_space_solve_for_integral_x(b)
+        np.testing.assert_array_equal(x, d[name + "__x"], err_msg=name)
+        assert np.array_equal(np.round(x), x)
+        np.testing.assert_array_equal(c.get_cycle_matrix() @ x.T, b.T)
+
+
+def test_static_problem_load_reads_reference_file(tmp_path):
+    """A problem file written by the reference's StaticProblem.save
+    (static_problem.py:433-457): the circuit, Is, f and n come back, and
+    saving again writes the same bytes."""
+    path = os.path.join(HERE, "golden", "static_problem_sq7_L_vortex.npy")
+    prob = pa.StaticProblem.load(path)
+    c = prob.get_circuit()
+    assert c.junction_count() == pa.SquareArray(7, 7).junction_count()
+    np.testing.assert_array_equal(c.get_inductance().diagonal(), 0.5)
+    d = np.load(os.path.join(HERE, "golden", "static.npz"), allow_pickle=False)
+    np.testing.assert_array_equal(prob.get_vortex_configuration(), d["sq7_L_vortex__in_n"])
+    out = tmp_path / "again.npy"
+    prob.save(str(out))
+    assert out.read_bytes() == open(path, "rb").read()

@@ -145,3 +145,31 @@ def test_noise_variance_matches_closed_form():
     x = eta
     y = A.T @ np.linalg.solve(A @ A.T / Rv, A @ (x / Rv))
     np.testing.assert_allclose(th0, (y - x) / Rv, rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["custom", "custom_dangling", "hole_8", "squid"])
+def test_oracle_reproduces_reference_on_general_graphs(case):
+    """The reference's runs on graphs that are not lattices
+    (graph_trajectories.npz): the oracle, on this package's graph and circuit,
+    reproduces theta, current, voltage and vortex counts bit for bit."""
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    d = np.load(os.path.join(here, "golden", "graph_trajectories.npz"), allow_pickle=False)
+    x9, y9 = [0, 1, 2, 0, 1, 2, 0, 1, 2], [0, 0, 0, 1, 1, 1, 2, 2, 2]
+    n1, n2 = [4, 2, 7, 0, 5, 6, 4, 1, 8, 7, 7], [3, 1, 6, 3, 2, 3, 5, 0, 5, 4, 8]
+    if case == "custom":
+        c = pa.Circuit(pa.EmbeddedGraph(x9, y9, n1, n2))
+    elif case == "custom_dangling":
+        c = pa.Circuit(pa.EmbeddedGraph(x9 + [0.4], y9 + [1.6], n1 + [9], n2 + [3]))
+    elif case == "hole_8":
+        full = pa.SquareArray(8, 8)
+        fx, fy = full.get_node_coordinates()
+        c = full.remove_nodes((fx > 2) & (fx < 5) & (fy > 2) & (fy < 5))
+    else:
+        c = pa.SQUID()
+    c.set_capacitance(0.5)
+    r = to.evolve(c, dt=float(d[case + "__in_dt"]), Nt=400, f=0.2, Is=d[case + "__in_Is"],
+                  store_time_steps=d[case + "__in_ts"])
+    for key in ("theta", "current", "voltage"):
+        np.testing.assert_array_equal(r[key], d[f"{case}__{key}"])
+    np.testing.assert_array_equal(to.vortex(c.get_cycle_matrix(), r["theta"]), d[case + "__n"])
