@@ -149,7 +149,7 @@ class AnnealingProblem:
         finally:
             eng.close()
         # five runs at T = 0 with half the time step
-        end = self._engine(self.time_step / 2, 5, 0.0)
+        end = self._engine(self.time_step / 2, 5, np.zeros((1, W, 1)))
         try:
             for i in range(5):
                 th = self._interval(end, th)
