@@ -10,6 +10,7 @@ counts computed on the device. Device memory is owned by torch tensors.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -416,6 +417,11 @@ class Engine:
             self.handle = C.c_void_p()
 
     def __del__(self):
+        # at interpreter exit the HIP runtime's own exit handlers may already
+        # have run: leave the memory to the process teardown instead of calling
+        # into the runtime (close() explicitly to free it earlier)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -502,6 +508,11 @@ class BatchEngine:
             self.handle = C.c_void_p()
 
     def __del__(self):
+        # at interpreter exit the HIP runtime's own exit handlers may already
+        # have run: leave the memory to the process teardown instead of calling
+        # into the runtime (close() explicitly to free it earlier)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
