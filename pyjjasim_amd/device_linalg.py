@@ -12,6 +12,7 @@ every solve runs on the device.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 import scipy.sparse
@@ -71,6 +72,11 @@ class DeviceSolver:
             self.handle = C.c_void_p()
 
     def __del__(self):
+        # at interpreter exit the HIP runtime's own exit handlers may already
+        # have run: leave the memory to the process teardown instead of calling
+        # into the runtime (close() explicitly to free it earlier)
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
