@@ -337,6 +337,20 @@ int jja_solve(jja_plan* plan, int32_t i, const double* b, int64_t ldb, double* x
               double* zbuf, void* stream);
 
 /*
+ * jja_solve for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference. It calls no hipMalloc, hipFree or allocating library routine: the
+ * global face store of its columns (cols * jja_plan_zbuf_doubles(plan, i)
+ * doubles) is a library array of 2^21 doubles (16 MiB) per GPU, held until the
+ * process ends; a call needing more is refused (JJA_EINVAL). Every call on one
+ * GPU uses that GPU's array, so calls that overlap on one GPU (other streams,
+ * other threads) race. The plan's device must be the current device. Results
+ * equal jja_solve's, bit for bit.
+ */
+int jja_solve_noalloc(jja_plan* plan, int32_t i, const double* b, int64_t ldb, double* x, int64_t ldx,
+                      int32_t cols, void* stream);
+
+/*
  * Vortex configuration n[f] = -sum_j A[f,j] rint(theta_j / (2 pi))
  * (time_evolution.py:734-755) for `cols` phase vectors of problem p:
  * theta [cols][Nj] -> n [cols][Nf] (int32).

@@ -94,7 +94,7 @@ EXPORTS = ["jja_cholesky", "jja_factor_get", "jja_factor_free", "jja_factor_last
            "jja_plan_validate", "jja_plan_create", "jja_plan_destroy", "jja_plan_lds_bytes",
            "jja_plan_zbuf_doubles", "jja_plan_status", "jja_plan_set_wait_limit",
            "jja_plan_one_xcd_parts", "jja_evolve",
-           "jja_vortex", "jja_adjoint", "jja_solve", "jja_last_error"]
+           "jja_vortex", "jja_adjoint", "jja_solve", "jja_solve_noalloc", "jja_last_error"]
 
 
 def lib():
@@ -128,6 +128,8 @@ def lib():
     L.jja_adjoint.argtypes = [C.c_void_p, C.POINTER(AdjointRun), C.c_int64, C.c_int64, C.c_void_p]
     L.jja_solve.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32,
                             C.c_void_p, C.c_void_p]
+    L.jja_solve_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                    C.c_int32, C.c_void_p]
     L.jja_last_error.restype = C.c_char_p
     _lib = L
     return L

@@ -28,11 +28,18 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/jja.h"
 #include "jja_math.h"
+
+// Device scratch of jja_solve_noalloc (a global face store for its columns):
+// 2^21 doubles (16 MiB) per GPU, held until the process ends. Calls on one
+// GPU share it; calls that overlap on one GPU race.
+constexpr int64_t kScratchDoubles = int64_t{1} << 21;
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 
@@ -1808,6 +1815,28 @@ int launch_spd_solve(Plan* plan, int32_t p, const double* b, int64_t ldb, double
   return JJA_OK;
 }
 
+// scratch's device address on each GPU, taken once per device with that device
+// current (the array's name is a host-side stand-in, not a device address)
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(JJA_EINVAL, "jja_solve_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    JJA_HIP(hipGetDevice(&cur));
+    if (cur != device) return fail(JJA_EINVAL, "jja_solve_noalloc: the plan's device is not the current device");
+    void* ptr = nullptr;
+    JJA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(JJA_EHIP, "jja_solve_noalloc: no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return JJA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2138,6 +2167,22 @@ int jja_solve(jja_plan* p, int32_t i, const double* b, int64_t ldb, double* x, i
     case 512: return launch_spd_solve<512>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
     default: return launch_spd_solve<1024>(plan, i, b, ldb, x, ldx, cols, zbuf, stream);
   }
+}
+
+int jja_solve_noalloc(jja_plan* p, int32_t i, const double* b, int64_t ldb, double* x, int64_t ldx,
+                      int32_t cols, void* stream_ptr) {
+  Plan* plan = reinterpret_cast<Plan*>(p);
+  if (!plan || i < 0 || i >= static_cast<int32_t>(plan->problems.size()) || cols < 0)
+    return fail(JJA_EINVAL, "jja_solve_noalloc: invalid arguments");
+  double* zbuf = nullptr;
+  if (plan->zbuf_doubles[i] > 0 && cols > 0) {
+    const int64_t need = static_cast<int64_t>(cols) * plan->zbuf_doubles[i];
+    if (need > kScratchDoubles)
+      return fail(JJA_EINVAL, "jja_solve_noalloc: " + std::to_string(cols) + " columns need " + std::to_string(need) +
+                                  " doubles of scratch; the limit is " + std::to_string(kScratchDoubles));
+    if (int rc = scratch_ptr(plan->device, &zbuf)) return rc;
+  }
+  return jja_solve(p, i, b, ldb, x, ldx, cols, zbuf, stream_ptr);
 }
 
 int jja_vortex(jja_plan* p, int32_t i, const double* theta, int32_t cols, int32_t* n_out,

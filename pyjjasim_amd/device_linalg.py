@@ -33,10 +33,10 @@ class DeviceSolver:
     (factorised once). ``solve(B)``: B a float64 device tensor (n,) or (n, k);
     returns a new tensor of the same shape."""
 
-    def __init__(self, K, xy, leaf_size=16, block_size=None, max_columns=1024):
+    def __init__(self, K, xy, leaf_size=16, block_size=None, max_columns=1024, face_store="auto"):
         self.torch = torch = _torch()
         self.lib = _native.lib()
-        self.host = MatrixPlan(K, xy, leaf_size=leaf_size, block_size=block_size)
+        self.host = MatrixPlan(K, xy, leaf_size=leaf_size, block_size=block_size, face_store=face_store)
         self.n = self.host.Nf
         self._desc = self.host.desc(max_columns=max_columns)
         self.handle = C.c_void_p()
@@ -63,6 +63,22 @@ class DeviceSolver:
         stream = torch.cuda.current_stream().cuda_stream
         _native.check(self.lib.jja_solve(self.handle, 0, Bc.data_ptr(), self.n, X.data_ptr(), self.n, cols,
                                          zb, stream))
+        X = X.t()
+        return X[:, 0] if squeeze else X
+
+    def solve_noalloc(self, B):
+        """solve() through jja_solve_noalloc: the face store comes from the
+        library's fixed device scratch (16 MiB per GPU) instead of a workspace,
+        and nothing is allocated below this call except the output tensor."""
+        torch = self.torch
+        squeeze = B.dim() == 1
+        Bc = (B[:, None] if squeeze else B).t().contiguous()
+        if Bc.dtype != torch.float64 or Bc.shape[1] != self.n:
+            raise ValueError(f"right-hand sides must be float64 with {self.n} rows")
+        X = torch.empty_like(Bc)
+        stream = torch.cuda.current_stream().cuda_stream
+        _native.check(self.lib.jja_solve_noalloc(self.handle, 0, Bc.data_ptr(), self.n, X.data_ptr(), self.n,
+                                                 Bc.shape[0], stream))
         X = X.t()
         return X[:, 0] if squeeze else X
 
