@@ -118,6 +118,9 @@ const char* jja_factor_last_error(void);
  * 64 + 4l + q hold its entries 2q and 2q+1 as 16-bit offsets from the base */
 #define JJA_SLICE_NARROW 0x1000
 
+/* most sine harmonics of a current-phase relation on the device */
+#define JJA_CPR_MAX_TERMS 8
+
 typedef struct jja_slice {
   int64_t base;    /* first value (face slices: first code) */
   int64_t ibase;   /* first index (solve slices) */
@@ -181,6 +184,12 @@ typedef struct jja_problem_desc {
    * so that jja_evolve and jja_adjoint allocate nothing; a run with more
    * columns is refused (JJA_EINVAL) */
   int32_t max_columns;
+  /* current-phase relation: 0 terms for the default Ic sin(theta)
+   * (static_problem.py:77-85); n terms for Ic * sum_{k<n} cpr_coef[k] sin((k+1) theta),
+   * the device form of a sine-series relation (jja_adjoint: default only) */
+  int32_t cpr_terms;
+  int32_t cpr_pad_;
+  double cpr_coef[JJA_CPR_MAX_TERMS];
 } jja_problem_desc;
 
 /*

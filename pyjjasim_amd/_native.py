@@ -52,6 +52,7 @@ class ProblemDesc(C.Structure):
         ("max_stage_rows", C.c_int32), ("face_copy_rows", C.c_int32),
         ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("parts", C.c_int32),
         ("column_pairs", C.c_int32), ("max_columns", C.c_int32),
+        ("cpr_terms", C.c_int32), ("cpr_pad_", C.c_int32), ("cpr_coef", C.c_double * 8),
     ]
 
 

@@ -20,7 +20,7 @@ from .graph import (EmbeddedGraph, EmbeddedSquareGraph, EmbeddedHoneycombGraph,
                     EmbeddedTriangularGraph)
 
 __all__ = ["Circuit", "SquareArray", "HoneycombArray", "TriangularArray", "SQUID",
-           "CurrentPhaseRelation", "DefaultCPR", "node_to_junction_current",
+           "CurrentPhaseRelation", "DefaultCPR", "FourierCPR", "node_to_junction_current",
            "NoCurrentConservationError"]
 
 
@@ -66,6 +66,54 @@ class DefaultCPR(CurrentPhaseRelation):
 
     def __init__(self):
         super().__init__(_sin_cpr, _cos_cpr, _int_cpr)
+
+
+class FourierCPR(CurrentPhaseRelation):
+    """Icp = Ic * sum_k a_k sin(k theta), k = 1 .. n with n <= 8: a sine-series
+    current-phase relation (higher harmonics, skewed relations), the family the
+    GPU step kernel evaluates besides the default. Its derivative
+    Ic sum_k k a_k cos(k theta) and integral Ic sum_k a_k (1 - cos(k theta)) / k
+    complete it as a CurrentPhaseRelation (static_problem.py:34-75).
+    FourierCPR([1.0]) is the default relation."""
+
+    MAX_TERMS = 8
+
+    def __init__(self, coefficients):
+        a = np.atleast_1d(np.asarray(coefficients, dtype=np.float64))
+        if a.ndim != 1 or not 1 <= len(a) <= self.MAX_TERMS:
+            raise ValueError(f"FourierCPR takes 1 to {self.MAX_TERMS} sine coefficients")
+        self.coefficients = a
+
+        def func(Ic, th):
+            acc = a[0] * np.sin(th)
+            for k in range(2, len(a) + 1):
+                acc = acc + a[k - 1] * np.sin(k * th)
+            return Ic * acc
+
+        def d_func(Ic, th):
+            acc = a[0] * np.cos(th)
+            for k in range(2, len(a) + 1):
+                acc = acc + (k * a[k - 1]) * np.cos(k * th)
+            return Ic * acc
+
+        def i_func(Ic, th):
+            acc = a[0] * (1.0 - np.cos(th))
+            for k in range(2, len(a) + 1):
+                acc = acc + (a[k - 1] / k) * (1.0 - np.cos(k * th))
+            return Ic * acc
+        super().__init__(func, d_func, i_func)
+
+
+def device_cpr_terms(cpr):
+    """(terms, coefficients) of a relation the step kernel evaluates: 0 terms for
+    the default Ic sin(theta), the sine series of a FourierCPR; None otherwise."""
+    if type(cpr) is DefaultCPR:
+        return 0, np.zeros(FourierCPR.MAX_TERMS)
+    if isinstance(cpr, FourierCPR):
+        c = np.zeros(FourierCPR.MAX_TERMS)
+        c[:len(cpr.coefficients)] = cpr.coefficients
+        return len(cpr.coefficients), c
+    return None
 
 
 class Circuit:

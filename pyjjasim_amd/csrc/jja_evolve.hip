@@ -123,7 +123,19 @@ struct DevProblem {
   const int32_t* face_fid;  // [n_slice_rows]: face index of a face record's row slot, or -1
   int32_t parts;            // workgroups per column (1, or more with a global face store)
   int32_t coef_si;          // junction stride of coef: 1, or 0 when every row is one value
+  int32_t cpr_terms;        // 0: Ic sin(arg); else Ic * sum_k cpr_coef[k] sin((k+1) arg)
+  double cpr_coef[JJA_CPR_MAX_TERMS];
 };
+
+// The current-phase relation's sum at one argument: sin(arg) for the default
+// relation, else sum_k a_k sin((k+1) arg) accumulated in ascending k (each
+// (k+1) arg formed as one product, as numpy forms k * theta).
+__device__ __forceinline__ double cpr_sum(const DevProblem& P, double arg) {
+  if (P.cpr_terms == 0) return jja_sin(arg);
+  double acc = P.cpr_coef[0] * jja_sin(arg);
+  for (int k = 1; k < P.cpr_terms; ++k) acc = acc + P.cpr_coef[k] * jja_sin(static_cast<double>(k + 1) * arg);
+  return acc;
+}
 
 // Stencil coefficient k of junction j (an identical value for every junction
 // is read from one address).
@@ -727,11 +739,11 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
   if (PREP) {
     double X;
     if constexpr (SW == 3) {
-      X = Ic * jja_sin(2.0 * t[0] - t[1]) + c[1] * t[0] + c[2] * t[1];
+      X = Ic * cpr_sum(P, 2.0 * t[0] - t[1]) + c[1] * t[0] + c[2] * t[1];
     } else if constexpr (SW == 4) {
-      X = Ic * jja_sin(3.0 * t[0] - 3.0 * t[1] + t[2]) + c[1] * t[0] + c[2] * t[1] + c[3] * t[2];
+      X = Ic * cpr_sum(P, 3.0 * t[0] - 3.0 * t[1] + t[2]) + c[1] * t[0] + c[2] * t[1] + c[3] * t[2];
     } else {
-      X = Ic * jja_sin(4.0 * t[0] - 6.0 * t[1] + 4.0 * t[2] - t[3]) + c[1] * t[0] + c[2] * t[1] +
+      X = Ic * cpr_sum(P, 4.0 * t[0] - 6.0 * t[1] + 4.0 * t[2] - t[3]) + c[1] * t[0] + c[2] * t[1] +
           c[3] * t[2] + c[4] * t[3];
     }
     double eta = 0.0;
@@ -820,14 +832,14 @@ __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionSt
 }
 
 template <class Z>
-__device__ __forceinline__ void jfast_compute(const JunctionStep& J, const Z& zs, int32_t j,
+__device__ __forceinline__ void jfast_compute(const DevProblem& P, const JunctionStep& J, const Z& zs, int32_t j,
                                               const JFast& L, double z) {
   double y = 0.0;
   if (L.code.x >= 0) y = (L.code.x & 1) ? y - zs.rd(L.code.x >> 1) : y + zs.rd(L.code.x >> 1);
   if (L.code.y >= 0) y = (L.code.y & 1) ? y - zs.rd(L.code.y >> 1) : y + zs.rd(L.code.y >> 1);
   const double t1 = (y - L.xo) / L.c0;
   J.th_n[j] = t1;
-  const double X = L.Ic * jja_sin(2.0 * t1 - L.t2) + L.c1 * t1 + L.c2 * L.t2;
+  const double X = L.Ic * cpr_sum(P, 2.0 * t1 - L.t2) + L.c1 * t1 + L.c2 * L.t2;
   double eta = 0.0;
   if (!J.t_zero) eta = L.sg * z;
   const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
@@ -851,8 +863,8 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
     // the Philox draws run while the loads are in flight
     double zA = 0.0, zB = 0.0;
     if (gauss) wave_normals<BS>(J, k, zA, zB);
-    if (jA < Nj) jfast_compute(J, zs, jA, A, zA);
-    if (jB < Nj) jfast_compute(J, zs, jB, B, zB);
+    if (jA < Nj) jfast_compute(P, J, zs, jA, A, zA);
+    if (jB < Nj) jfast_compute(P, J, zs, jB, B, zB);
   }
 }
 
@@ -1463,6 +1475,8 @@ int validate(const jja_problem_desc& d) {
                             "of at most 512 threads");
   if (d.max_columns < 1 || d.max_columns > (1 << 20))
     return fail(JJA_EINVAL, "jja_plan_create: max_columns must be 1 .. 2^20");
+  if (d.cpr_terms < 0 || d.cpr_terms > JJA_CPR_MAX_TERMS)
+    return fail(JJA_EINVAL, "jja_plan_create: cpr_terms must be 0 .. " + std::to_string(JJA_CPR_MAX_TERMS));
   if (Nj > 0 && (!d.Ic || !d.Rv || !d.coef || !d.junc_code))
     return fail(JJA_EINVAL, "jja_plan_create: missing junction arrays");
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
@@ -1886,6 +1900,8 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.copy_rows = d.face_copy_rows;
     dp.index_bytes = d.index_bytes;
     dp.val_bytes = static_cast<int32_t>(d.n_entries * static_cast<int64_t>(sizeof(double)));
+    dp.cpr_terms = d.cpr_terms;
+    for (int k = 0; k < JJA_CPR_MAX_TERMS; ++k) dp.cpr_coef[k] = d.cpr_coef[k];
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
@@ -2101,6 +2117,8 @@ int jja_adjoint(jja_plan* p, const jja_adjoint_run* runs, int64_t k_hi, int64_t 
   Plan* plan = reinterpret_cast<Plan*>(p);
   if (!plan || !runs || k_lo < 0 || k_hi < k_lo) return fail(JJA_EINVAL, "jja_adjoint: invalid arguments");
   if (plan->stencil_width != 3) return fail(JJA_EUNSUPPORTED, "jja_adjoint: stencil width 3 only");
+  for (const DevProblem& P : plan->problems)
+    if (P.cpr_terms != 0) return fail(JJA_EUNSUPPORTED, "jja_adjoint: the default current-phase relation only");
   const int32_t np = static_cast<int32_t>(plan->problems.size());
   std::vector<WorkItem> work, work_g;
   int64_t lds = 0;

@@ -15,7 +15,7 @@ import sys
 import numpy as np
 
 from . import _native
-from .circuit import DefaultCPR
+from .circuit import device_cpr_terms
 from .solver_plan import CircuitPlan
 
 
@@ -113,6 +113,14 @@ def reference_noise(Nj, W):
 DEFAULT_RESIDENT_OUTPUT_BYTES = 16 << 30
 
 
+def set_cpr(desc, cpr):
+    """The current-phase relation's sine series in a problem descriptor."""
+    terms, coef = cpr
+    desc.cpr_terms = int(terms)
+    for k in range(len(coef)):
+        desc.cpr_coef[k] = float(coef[k])
+
+
 def default_block_size(Nj, Nf):
     work = max(Nj, Nf)
     for bs in (64, 128, 256, 512):
@@ -146,9 +154,10 @@ class Engine:
     host)`` the problem is problem ``index`` of a plan that a BatchEngine owns."""
 
     def __init__(self, problem, th_mask, I_mask, shared=None):
-        if not isinstance(problem.current_phase_relation, DefaultCPR):
-            raise ValueError("the GPU time evolution evaluates the default current-phase "
-                             "relation Ic*sin(theta) only; got a custom CurrentPhaseRelation")
+        cpr = device_cpr_terms(problem.current_phase_relation)
+        if cpr is None:
+            raise ValueError("the GPU time evolution evaluates the default current-phase relation "
+                             "Ic*sin(theta) and sine series (FourierCPR); got another CurrentPhaseRelation")
         torch = self.torch = _torch()
         self.problem = problem
         circuit = problem.circuit
@@ -173,6 +182,7 @@ class Engine:
                 self.host = self.host.paired()
                 self.block_size = self.host.block_size
             self._desc = self.host.desc(max_columns=W)
+            set_cpr(self._desc, cpr)
             self.handle = C.c_void_p()
             _native.check(lib.jja_plan_create(1, C.byref(self._desc), C.byref(self.handle)))
             self.owns_plan, self.index = True, 0
@@ -456,6 +466,12 @@ class BatchEngine:
                                       local_stages=getattr(p, "local_stages", "auto")) for p in problems]
         descs = (_native.ProblemDesc * len(problems))(*[h.desc(max_columns=p.problem_count)
                                                          for h, p in zip(self.hosts, problems)])
+        for d, p in zip(descs, problems):
+            cpr = device_cpr_terms(p.current_phase_relation)
+            if cpr is None:
+                raise ValueError("the GPU time evolution evaluates the default current-phase relation "
+                                 "Ic*sin(theta) and sine series (FourierCPR); got another CurrentPhaseRelation")
+            set_cpr(d, cpr)
         self._descs = descs
         self.handle = C.c_void_p()
         _native.check(self.lib.jja_plan_create(len(problems), descs, C.byref(self.handle)))

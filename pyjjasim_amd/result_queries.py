@@ -139,15 +139,40 @@ class DeviceQueries:
     def potential(self, voltage, tps, idx):
         return self.phase(voltage, tps, idx)
 
+    def _series(self):
+        """(terms, coefficients) of the problem's current-phase relation: 0 terms
+        for the default Ic sin(theta), else its sine series (FourierCPR)."""
+        from .circuit import device_cpr_terms
+        series = device_cpr_terms(self.problem.current_phase_relation)
+        if series is None:
+            raise ValueError("device queries evaluate the default and sine-series current-phase relations")
+        return series
+
     def josephson_energy(self, theta, tps, idx):
+        """The relation's integral Ic sum_k a_k (1 - cos(k theta)) / k, in the
+        operation order of FourierCPR (Ic (1 - cos theta) for the default)."""
+        torch = self.torch
         th = self._dev(theta[:, :, idx])
         Ic = self._dev(self.circuit._Ic())[:, None, None]
-        return Ic * (1.0 - self.torch.cos(th))
+        n, a = self._series()
+        if n == 0:
+            return Ic * (1.0 - torch.cos(th))
+        acc = a[0] * (1.0 - torch.cos(th))
+        for k in range(2, n + 1):
+            acc = acc + (a[k - 1] / k) * (1.0 - torch.cos(k * th))
+        return Ic * acc
 
     def supercurrent(self, theta, tps, idx):
+        torch = self.torch
         th = self._dev(theta[:, :, idx])
         Ic = self._dev(self.circuit._Ic())[:, None, None]
-        return Ic * self.torch.sin(th)
+        n, a = self._series()
+        if n == 0:
+            return Ic * torch.sin(th)
+        acc = a[0] * torch.sin(th)
+        for k in range(2, n + 1):
+            acc = acc + a[k - 1] * torch.sin(k * th)
+        return Ic * acc
 
     def cycle_current(self, current, tps, idx):
         Nj, W, S = current.shape[0], current.shape[1], len(idx)

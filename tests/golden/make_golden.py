@@ -507,12 +507,60 @@ def make_annealing(pj):
     print("annealing:", sorted(cases))
 
 
+def make_custom_cpr(pj):
+    """Time evolution with a custom current-phase relation
+    (CurrentPhaseRelation(func, d_func, i_func), static_problem.py:34-75; as in
+    examples/static_example_10_custum_current_phase_relation.py) that is a sine
+    series, Ic (sin theta + a2 sin 2 theta + a3 sin 3 theta), written with the
+    numpy operations of pyjjasim_amd.FourierCPR."""
+    flat = {}
+    for name, a in (("harm2", (1.0, 0.3)), ("harm3", (0.8, -0.25, 0.1))):
+        a = np.asarray(a)
+
+        def func(Ic, th, a=a):
+            acc = a[0] * np.sin(th)
+            for k in range(2, len(a) + 1):
+                acc = acc + a[k - 1] * np.sin(k * th)
+            return Ic * acc
+
+        def d_func(Ic, th, a=a):
+            acc = a[0] * np.cos(th)
+            for k in range(2, len(a) + 1):
+                acc = acc + (k * a[k - 1]) * np.cos(k * th)
+            return Ic * acc
+
+        def i_func(Ic, th, a=a):
+            acc = a[0] * (1.0 - np.cos(th))
+            for k in range(2, len(a) + 1):
+                acc = acc + (a[k - 1] / k) * (1.0 - np.cos(k * th))
+            return Ic * acc
+        cp = pj.CurrentPhaseRelation(func, d_func, i_func)
+        c = pj.SquareArray(7, 6)
+        c.set_capacitance(0.5)
+        Is = c.current_base(0.0)[:, None, None] * np.array([0.5, 1.1])[None, :, None]
+        ts = np.arange(0, 400, 20)
+        prob = pj.TimeEvolutionProblem(c, time_step=0.05, time_step_count=400, external_flux=0.15,
+                                       current_sources=Is, store_time_steps=ts, current_phase_relation=cp)
+        res = prob.compute()
+        flat[f"{name}__coef"] = a
+        flat[f"{name}__theta"] = res.get_theta()
+        flat[f"{name}__voltage"] = res.get_voltage()
+        flat[f"{name}__current"] = res.get_current()
+        flat[f"{name}__n"] = res.get_vortex_configuration().astype(np.int64)
+        flat[f"{name}__get_josephson_energy"] = res.get_josephson_energy()
+        flat[f"{name}__get_supercurrent"] = res.get_supercurrent()
+        flat[f"{name}__in_Is"] = Is
+        flat[f"{name}__in_ts"] = ts
+    np.savez_compressed(os.path.join(HERE, "custom_cpr.npz"), **flat)
+    print("custom cpr: harm2, harm3")
+
+
 if __name__ == "__main__":
     pj = _import_reference()
     parts = sys.argv[1:] or ["graphs", "trajectories", "queries", "graph_trajectories", "circuit_files",
-                             "integral_solves", "static", "annealing"]
+                             "integral_solves", "static", "annealing", "custom_cpr"]
     for part in parts:
         {"graphs": make_graphs, "trajectories": make_trajectories, "queries": make_queries,
          "graph_trajectories": make_graph_trajectories, "circuit_files": make_circuit_files,
          "integral_solves": make_integral_solves, "static": make_static,
-         "annealing": make_annealing}[part](pj)
+         "annealing": make_annealing, "custom_cpr": make_custom_cpr}[part](pj)

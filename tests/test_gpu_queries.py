@@ -144,3 +144,23 @@ def test_gpu_streamed_outputs_equal_resident_outputs():
     streamed, resident = run(7 * row), run(None)
     for g in ("get_theta", "get_current", "get_voltage", "get_vortex_configuration"):
         np.testing.assert_array_equal(getattr(streamed, g)(), getattr(resident, g)())
+
+
+@pytest.mark.parametrize("case", ["harm2", "harm3"])
+def test_gpu_sine_series_cpr_matches_reference(case):
+    """A sine-series current-phase relation on the device (FourierCPR) against the
+    reference run with the same relation as a custom CurrentPhaseRelation
+    (custom_cpr.npz): theta, current, voltage, Josephson energy and supercurrent
+    within 1e-9 of the reference's largest value, vortex counts exact."""
+    g = _load("custom_cpr.npz")[case]
+    c = pa.SquareArray(7, 6)
+    c.set_capacitance(0.5)
+    res = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=400, external_flux=0.15,
+                                  current_sources=g["in_Is"], store_time_steps=g["in_ts"],
+                                  current_phase_relation=pa.FourierCPR(g["coef"])).compute()
+    print(f"[cpr] {case}: coefficients {g['coef']}", flush=True)
+    for name, got in (("theta", res.get_theta()), ("current", res.get_current()), ("voltage", res.get_voltage()),
+                      ("get_josephson_energy", res.get_josephson_energy()),
+                      ("get_supercurrent", res.get_supercurrent())):
+        print(f"  {name}: rel err {_close(got, g[name], name):.2e}", flush=True)
+    np.testing.assert_array_equal(res.get_vortex_configuration(), g["n"])

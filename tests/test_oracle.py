@@ -173,3 +173,40 @@ def test_oracle_reproduces_reference_on_general_graphs(case):
     for key in ("theta", "current", "voltage"):
         np.testing.assert_array_equal(r[key], d[f"{case}__{key}"])
     np.testing.assert_array_equal(to.vortex(c.get_cycle_matrix(), r["theta"]), d[case + "__n"])
+
+
+@pytest.mark.parametrize("case", ["harm2", "harm3"])
+def test_oracle_reproduces_reference_with_sine_series_cpr(case):
+    """The reference's runs with a custom sine-series current-phase relation
+    (custom_cpr.npz): the oracle with FourierCPR's evaluation reproduces
+    theta, current and voltage bit for bit."""
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    d = np.load(os.path.join(here, "golden", "custom_cpr.npz"), allow_pickle=False)
+    cpr = pa.FourierCPR(d[case + "__coef"])
+    c = pa.SquareArray(7, 6)
+    c.set_capacitance(0.5)
+    r = to.evolve(c, dt=0.05, Nt=400, f=0.15, Is=d[case + "__in_Is"], store_time_steps=d[case + "__in_ts"],
+                  cp=cpr.eval)
+    for key in ("theta", "current", "voltage"):
+        np.testing.assert_array_equal(r[key], d[f"{case}__{key}"])
+
+
+def test_fourier_cpr_is_consistent():
+    """FourierCPR([1]) is the default relation; d_eval is the derivative and
+    i_eval the integral (from 0) of eval, checked by central differences and
+    the trapezoid rule."""
+    th = np.linspace(-4, 4, 2001)
+    Ic = np.array([1.3])
+    one = pa.FourierCPR([1.0])
+    dflt = pa.DefaultCPR()
+    np.testing.assert_array_equal(one.eval(Ic, th), dflt.eval(Ic, th))
+    np.testing.assert_array_equal(one.i_eval(Ic, th), dflt.i_eval(Ic, th))
+    cpr = pa.FourierCPR([0.8, -0.25, 0.1])
+    h = 1e-6
+    fd = (cpr.eval(Ic, th + h) - cpr.eval(Ic, th - h)) / (2 * h)
+    assert np.abs(fd - cpr.d_eval(Ic, th)).max() < 1e-8
+    x = np.linspace(0, 2.5, 200001)
+    y = cpr.eval(Ic, x)
+    integral = np.sum((y[1:] + y[:-1]) * np.diff(x)) / 2
+    assert abs(integral - cpr.i_eval(Ic, np.array([2.5]))[0]) < 1e-9
