@@ -7,8 +7,9 @@ stream, dt = 0.05, theta stored at steps Nt//3 and Nt-1.
 
 A bench "step" is one time step of all 256 columns. The timed region covers
 ``--steps`` steps (persistent kernel launches) with all inputs resident in HBM,
-after ``--warmup`` untimed steps. The plain loop runs before them (its GPU work
-also brings the device to its working clock). Reported:
+after ``--warmup`` untimed steps. The plain loop runs between the batched
+run's host set-up and its warmup (its GPU work also brings the device to its
+working clock). Reported:
 
   value      junction-steps/s = Nj * W * steps / wall time
   roofline   algorithmic HBM bytes per launch / mean launch time (HIP events on the
@@ -67,7 +68,10 @@ def make_problem(pa, n, W, Nt, T=0.01, seed=12345, steps_per_launch=None, block_
 
 
 def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events=True, amplitudes=None,
-            **engine_options):
+            prelude=None, **engine_options):
+    """The timed batched run. ``prelude`` (GPU work of its own, such as the
+    plain loop) runs after the host has built this run's plan and before the
+    warmup steps, so that no host set-up lies between it and the timed steps."""
     from pyjjasim_amd.engine import Engine
     Nt = warmup + steps
     c, prob = make_problem(pa, n, W, Nt, steps_per_launch=steps_per_launch, block_size=block_size,
@@ -75,6 +79,8 @@ def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events
     th_mask = prob.store_time_steps.copy()
     I_mask = np.zeros(Nt, dtype=bool)
     eng = Engine(prob, th_mask, I_mask)
+    if prelude is not None:
+        prelude()
     eng.advance(warmup)
     torch.cuda.synchronize()
     launches = []
@@ -533,14 +539,18 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    # the plain loop (one problem at a time) is measured first; its few seconds
-    # of GPU work also bring the device to its working clock before the
-    # batched steps are timed
+    # the plain loop (one problem at a time) is measured between the batched
+    # run's host set-up and its warmup steps; its few seconds of GPU work also
+    # bring the device to its working clock right before the timed steps
     plain = None
-    if args.plain_steps > 0:
+
+    def plain_loop():
+        nonlocal plain
         plain = plain_best(pa, torch, args.n, args.plain_steps)
+
     wall, launch_ms, th, nv, c, info = run_gpu(pa, torch, args.n, args.W, args.steps, args.warmup,
-                                               args.steps_per_launch, args.block_size)
+                                               args.steps_per_launch, args.block_size,
+                                               prelude=plain_loop if args.plain_steps > 0 else None)
     Nj, W, K = info["Nj"], args.W, args.steps
     value = Nj * W * K / wall
     # dominant kernel: evolve_kernel; one launch advances k steps of the whole

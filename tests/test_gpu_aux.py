@@ -5,7 +5,12 @@
   bit (T = 0: nothing depends on the absolute step).
 * NaN input: a NaN critical current propagates to the phases it reaches (as
   numpy would) without a fault, a hang or a status error, and the next run on
-  the device is exact again."""
+  the device is exact again.
+* Noise modes: the in-kernel Philox stream and the replay of the reference's
+  legacy draws (time_evolution.py:533-537) are different realisations of the
+  thermal noise; the mean voltage of a thermally activated IV point agrees
+  between them within its statistical band, below and above the Nj = 500 switch
+  to the reference's every-third-step draws."""
 import numpy as np
 import pytest
 
@@ -54,3 +59,25 @@ def test_gpu_nan_input_propagates_without_fault():
                                    store_time_steps=[49]).compute()
     ref2 = to.evolve(good, dt=0.05, Nt=50, f=0.1, Is=Is, store_time_steps=[49])
     assert np.abs(res2.get_theta() - ref2["theta"]).max() <= 1e-9 * np.abs(ref2["theta"]).max()
+
+
+@pytest.mark.parametrize("n", [5, 17])
+def test_gpu_noise_modes_agree_in_mean_voltage(n):
+    """<V> over W realisations at I = 0.9 Ic, T = 0.05 (thermally activated
+    phase slips): the two noise modes differ by less than 4 standard errors."""
+    print(f"[aux] noise modes SquareArray({n},{n})", flush=True)
+    c = pa.SquareArray(n, n)
+    W, Nt, t0 = 64, 4000, 1000
+    Is = 0.9 * c.current_base(0.0)[:, None, None] * np.ones((1, W, 1))
+    hor = np.abs(c.current_base(0.0)) > 0
+    v = {}
+    np.random.seed(3)          # the reference mode draws from numpy's global RNG
+    for mode in ("philox", "reference"):
+        res = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, current_sources=Is, temperature=0.05,
+                                      store_time_steps=[t0, Nt - 1], store_current=False, store_voltage=False,
+                                      noise=mode, seed=11).compute()
+        th = res.get_theta()
+        v[mode] = (th[hor, :, 1] - th[hor, :, 0]).mean(axis=0) / ((Nt - 1 - t0) * 0.05)
+    se = np.sqrt(v["philox"].var(ddof=1) / W + v["reference"].var(ddof=1) / W)
+    assert v["philox"].mean() > 0 and v["reference"].mean() > 0
+    assert abs(v["philox"].mean() - v["reference"].mean()) < 4 * se, (v["philox"].mean(), v["reference"].mean(), se)
