@@ -120,6 +120,8 @@ const char* jja_factor_last_error(void);
 
 /* most sine harmonics of a current-phase relation on the device */
 #define JJA_CPR_MAX_TERMS 8
+/* sine modes one wave of the lattice solve accumulates at once (even) */
+#define JJA_LATTICE_CHUNK 14
 
 typedef struct jja_slice {
   int64_t base;    /* first value (face slices: first code) */
@@ -190,6 +192,26 @@ typedef struct jja_problem_desc {
   int32_t cpr_terms;
   int32_t cpr_pad_;
   double cpr_coef[JJA_CPR_MAX_TERMS];
+  /* Lattice solve: 0 rows, or the face system of a rectangular lattice of
+   * R x C faces at positions r*C + c whose face matrix is the Kronecker sum
+   * K = g (T_R (x) I_C) + g_c (I_R (x) T_C), T_n = tridiag(-1, 2, -1). The
+   * step then solves K by a sine transform (DST-I) of every row, one
+   * tridiagonal solve along r per sine mode and the inverse transform, in
+   * place of the staged factor program (which is then empty: n_solve_slices 0,
+   * face_copy_rows 0). The reference factorises K with SuperLU for every
+   * circuit (time_evolution.py:505-506); this is an exact direct solve of the
+   * same K. Needs parts 1, R >= 2, C >= 2, R*C = n_faces and max_stage_rows >=
+   * (n_faces & 1) + 2*R*ES with H = (C+1)/2, ES = H + (H & 1). */
+  int32_t lattice_rows;        /* R (0: no lattice solve) */
+  int32_t lattice_cols;        /* C */
+  /* [ES][CP], CP = C rounded up to a multiple of JJA_LATTICE_CHUNK:
+   * sqrt(2/(C+1)) sin(pi (i+1)(j+1)/(C+1)) for i < H and j < C, else 0 */
+  const double* lattice_sine;
+  /* [R][CP]: 1 / pivot of sine mode k's tridiagonal factor g T_R + g_c mu_k I
+   * at row r; the solve is d_r = (y_r + g d_{r-1}) inv[r][k], then
+   * x_r = d_r - ((-g) inv[r][k]) x_{r+1} */
+  const double* lattice_inv;
+  double lattice_g;            /* g */
 } jja_problem_desc;
 
 /*

@@ -53,6 +53,8 @@ class ProblemDesc(C.Structure):
         ("solve_wave_ptr", i32p), ("index_bytes", C.c_int32), ("parts", C.c_int32),
         ("column_pairs", C.c_int32), ("max_columns", C.c_int32),
         ("cpr_terms", C.c_int32), ("cpr_pad_", C.c_int32), ("cpr_coef", C.c_double * 8),
+        ("lattice_rows", C.c_int32), ("lattice_cols", C.c_int32),
+        ("lattice_sine", f64p), ("lattice_inv", f64p), ("lattice_g", C.c_double),
     ]
 
 

@@ -65,7 +65,7 @@ class DifferentiableEvolution:
     def __init__(self, circuit, time_step=0.05, time_step_count=1000, problem_count=1,
                  temperature=0.0, store_time_steps=None, config_at_minus_1=None,
                  config_at_minus_2=None, seed=None, checkpoint_every=200, block_size=None,
-                 face_store="auto", parts=None, local_stages="auto", column_pairs=False):
+                 face_store="auto", parts=None, local_stages="auto", column_pairs=False, solver="auto"):
         torch = self.torch = _torch()
         W, Nj, Nf = int(problem_count), circuit._Nj(), circuit._Nf()
         self.W, self.Nj, self.Nf, self.Nt = W, Nj, Nf, int(time_step_count)
@@ -80,7 +80,7 @@ class DifferentiableEvolution:
             store_voltage=False, store_current=False, config_at_minus_1=config_at_minus_1,
             config_at_minus_2=config_at_minus_2, seed=seed if seed is not None else 0,
             steps_per_launch=self.C, block_size=block_size, face_store=face_store, parts=parts,
-            local_stages=local_stages, column_pairs=column_pairs,
+            local_stages=local_stages, column_pairs=column_pairs, solver=solver,
             max_resident_output_bytes=1 << 62)   # stored phases stay on the device
         self.problem = prob
         self.th_mask = prob.store_time_steps.copy()

@@ -65,9 +65,9 @@ def balanced_parts(hosts, columns, n_cu, max_parts=16):
 
 
 def _plan_one(args):
-    circuit, dt, sw, leaf, bs, store, local = args
+    circuit, dt, sw, leaf, bs, store, local, solver = args
     return CircuitPlan(circuit, dt, sw, leaf_size=leaf, block_size=bs, face_store=store,
-                       local_stages=local)
+                       local_stages=local, solver=solver)
 
 
 def build_plans(problems, block_size, workers=None):
@@ -77,7 +77,8 @@ def build_plans(problems, block_size, workers=None):
     only run host code; they are started fresh (spawn), never forked from a
     process that may hold a device context."""
     jobs = [(p.circuit, p._dt(), p.stencil_width, p.leaf_size, block_size,
-             getattr(p, "face_store", "auto"), getattr(p, "local_stages", "auto")) for p in problems]
+             getattr(p, "face_store", "auto"), getattr(p, "local_stages", "auto"),
+             getattr(p, "solver", "auto")) for p in problems]
     size = sum(p.circuit._Nj() for p in problems)
     if workers is None:
         workers = min(16, len(os.sched_getaffinity(0)), len(jobs)) if size > 200_000 else 1
@@ -129,6 +130,7 @@ class TimeEvolutionBatch:
         self.parts = parts if parts == "balanced" else int(parts)
         self.max_parts = int(max_parts)
         self.tuning = None
+        self.solvers = None
 
     # --------------------------------------------------------------- tuning
     def _engine(self, hosts, masks):
@@ -139,7 +141,8 @@ class TimeEvolutionBatch:
         workgroup per column, and 1024-thread workgroups with 2, 4, ... parts per
         column (all face stores global) while every part stays resident."""
         out = [(bs, 1) for bs in self.block_sizes]
-        if any(p.stencil_width != 3 or getattr(p, "face_store", "auto") == "lds" for p in self.problems):
+        if any(p.stencil_width != 3 or getattr(p, "face_store", "auto") == "lds" or
+               getattr(p, "solver", "auto") == "lattice" for p in self.problems):
             return out
         cols = sum(p.problem_count for p in self.problems)
         n_cu = device_compute_units()
@@ -161,8 +164,11 @@ class TimeEvolutionBatch:
         else:
             per = [int(parts)] * len(base)
         split = max(per) > 1
-        return [h if (h.block_size, h.parts) == (bs, q) and (h.global_store or not split)
-                else h.relayout(bs, q, "global" if split else None) for h, q in zip(base, per)]
+        if split:
+            # a plan with split columns solves every problem with the factor
+            return [h if (h.block_size, h.parts) == (bs, q) and h.global_store and not h.uses_lattice
+                    else h.relayout(bs, q, "global", solver="cholesky") for h, q in zip(base, per)]
+        return [h if (h.block_size, h.parts) == (bs, q) else h.relayout(bs, q) for h, q in zip(base, per)]
 
     def _tune(self, masks):
         """Time each candidate (workgroup size, parts) on tune_steps steps;
@@ -238,6 +244,9 @@ class TimeEvolutionBatch:
             if self.parts == "balanced" or self.parts > 1:
                 hosts = self._hosts_for(hosts, self.block_size, self.parts)
         np.random.set_state(rng_state)
+        # how each problem's face system is solved in this batch (a plan with
+        # split columns solves every problem with the factor)
+        self.solvers = ["lattice" if h.uses_lattice else "cholesky" for h in hosts]
         eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=hosts)
         try:
             if self.autotune and not self.steps_per_launch:

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -125,6 +126,13 @@ struct DevProblem {
   int32_t coef_si;          // junction stride of coef: 1, or 0 when every row is one value
   int32_t cpr_terms;        // 0: Ic sin(arg); else Ic * sum_k cpr_coef[k] sin((k+1) arg)
   double cpr_coef[JJA_CPR_MAX_TERMS];
+  // lattice solve (lat_rows 0: the staged factor program solves K)
+  int32_t lat_rows, lat_cols;  // R, C: faces at positions r*C + c
+  int32_t lat_es, lat_cp;      // folded row length ES (even) and sine-table row stride CP
+  int64_t lat_t0;              // store offset of the folded rows (even: 16-byte loads)
+  const double* lat_sine;      // [ES][CP]
+  const double* lat_inv;       // [R][CP] inverse Thomas pivots
+  double lat_g;
 };
 
 // The current-phase relation's sum at one argument: sin(arg) for the default
@@ -238,6 +246,10 @@ struct FaceStore<false> {
     return *reinterpret_cast<const LAS double*>(static_cast<uintptr_t>(i) << 3);
   }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
+  // elements i and i+1 (i even) in one 16-byte load
+  __device__ __forceinline__ vdouble2 rd2(int64_t i) const {
+    return *reinterpret_cast<const LAS vdouble2*>(static_cast<uintptr_t>(i) << 3);
+  }
   // orders LDS traffic only: global loads in flight stay in flight
   __device__ __forceinline__ void barrier() const { lds_barrier(); }
   __device__ __forceinline__ void wg_barrier() const { lds_barrier(); }
@@ -252,6 +264,9 @@ struct FaceStore<true> {
   __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
   __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
+  __device__ __forceinline__ vdouble2 rd2(int64_t i) const {
+    return *reinterpret_cast<const GAS vdouble2*>(p + i);
+  }
   __device__ __forceinline__ void barrier() const { __syncthreads(); }
   __device__ __forceinline__ void wg_barrier() const { __syncthreads(); }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
@@ -501,6 +516,154 @@ __device__ __forceinline__ void solve(const DevProblem& P, const Z& zs) {
     m1 = m4;
     m2 = m5;
   }
+}
+
+// ------------------------------------------------------------ lattice solve
+// The face matrix of a rectangular lattice of R x C faces (positions r*C + c)
+// with one junction weight per direction is the Kronecker sum
+// g (T_R (x) I_C) + g_c (I_R (x) T_C), T_n = tridiag(-1, 2, -1), so
+// K^-1 = (I (x) S) M^-1 (I (x) S) with the orthonormal symmetric sine matrix S
+// (DST-I) and M tridiagonal along r for each sine mode (Hockney, J. ACM 12
+// (1965) 95-113). Per step: fold every row (S[C-1-i][j] = (-1)^j S[i][j]),
+// transform (out[r][j] = sum_{i<H} F_j[r][i] S[i][j], H = ceil(C/2)), one
+// tridiagonal solve per mode along r, then fold and transform again. Each
+// output is summed by one thread in ascending i with fused multiply-adds, and
+// each mode's recurrence runs in one thread, so results depend neither on the
+// block size nor on the face store.
+
+template <class Z>
+struct IsFaceStore {
+  static constexpr bool value = false;
+};
+template <bool G>
+struct IsFaceStore<FaceStore<G>> {
+  static constexpr bool value = true;
+};
+
+// E[r][i] = v[r][i] + v[r][C-1-i], O[r][i] = v[r][i] - v[r][C-1-i] for i < H
+// (the middle of an odd row pairs with itself: E = v, O = 0) and 0 for
+// H <= i < ES, at lat_t0 + r*ES + i and lat_t0 + (R + r)*ES + i.
+template <class Z>
+__device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
+  const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, H = (C + 1) >> 1;
+  const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * ES;
+  for (int32_t e = zs.vt; e < R * ES; e += zs.vbs) {
+    const int32_t r = e / ES, i = e - r * ES;
+    double a = 0.0, b = 0.0;
+    if (i < H) {
+      const int64_t row = static_cast<int64_t>(r) * C;
+      const double x = zs.rd(row + i), y = zs.rd(row + (C - 1 - i));
+      if (2 * i + 1 == C) {
+        a = x;
+      } else {
+        a = x + y;
+        b = x - y;
+      }
+    }
+    zs.wr(e0 + e, a);
+    zs.wr(o0 + e, b);
+  }
+}
+
+// out[r][j] = sum_{i<ES} (j even ? E : O)[r][i] S[i][j] into the face vector.
+// A wave takes 64 rows (one per lane) and JJA_LATTICE_CHUNK consecutive modes
+// from an even j0, so a mode's parity is its slot's; the table's values are
+// wave-uniform scalar loads.
+template <class Z>
+__device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& zs) {
+  constexpr int MC = JJA_LATTICE_CHUNK;
+  const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, CP = P.lat_cp;
+  const int lane = zs.vt & (kWave - 1);
+  const int32_t wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
+  const int32_t NW = zs.vbs / kWave;
+  const int32_t nrb = (R + kWave - 1) / kWave, tasks = nrb * (CP / MC);
+  const CAS double* __restrict__ S = (const CAS double*)P.lat_sine;
+  for (int32_t task = wave; task < tasks; task += NW) {
+    const int32_t rb = task % nrb, j0 = (task / nrb) * MC;
+    const int32_t r = rb * kWave + lane;
+    const int64_t e = P.lat_t0 + static_cast<int64_t>(r < R ? r : R - 1) * ES;
+    const int64_t o = e + static_cast<int64_t>(R) * ES;
+    double acc[MC];
+#pragma unroll
+    for (int q = 0; q < MC; ++q) acc[q] = 0.0;
+    for (int32_t i = 0; i < ES; i += 2) {
+      const vdouble2 ev = zs.rd2(e + i), ov = zs.rd2(o + i);
+      const CAS double* s0 = S + static_cast<int64_t>(i) * CP + j0;
+      const CAS double* s1 = s0 + CP;
+#pragma unroll
+      for (int q = 0; q < MC; ++q) {
+        acc[q] = __builtin_fma((q & 1) ? ov.x : ev.x, s0[q], acc[q]);
+        acc[q] = __builtin_fma((q & 1) ? ov.y : ev.y, s1[q], acc[q]);
+      }
+    }
+    if (r < R) {
+      const int64_t row = static_cast<int64_t>(r) * C;
+#pragma unroll
+      for (int q = 0; q < MC; ++q)
+        if (j0 + q < C) zs.wr(row + j0 + q, acc[q]);
+    }
+  }
+}
+
+// Mode k's tridiagonal solve along r, in place in the face vector, with the
+// inverse pivots staged in the (then free) folded-row area: forward
+// d_r = (y_r + g d_{r-1}) inv_r, backward x_r = d_r - ((-g) inv_r) x_{r+1}.
+template <class Z>
+__device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs) {
+  const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp;
+  const int64_t t0 = P.lat_t0;
+  const GAS double* __restrict__ inv = gbl(P.lat_inv);
+  for (int32_t e = zs.vt; e < R * C; e += zs.vbs) {
+    const int32_t r = e / C, k = e - r * C;
+    zs.wr(t0 + e, inv[static_cast<int64_t>(r) * CP + k]);
+  }
+  zs.barrier();
+  const double g = P.lat_g, mg = -P.lat_g;
+  for (int32_t k = zs.vt; k < C; k += zs.vbs) {
+    double d = 0.0;
+    double y = zs.rd(k), iv = zs.rd(t0 + k);
+    for (int32_t r = 0; r < R; ++r) {
+      const int64_t q = static_cast<int64_t>(r) * C + k;
+      const bool more = r + 1 < R;
+      const double yn = more ? zs.rd(q + C) : 0.0, ivn = more ? zs.rd(t0 + q + C) : 0.0;
+      d = __builtin_fma(g, d, y) * iv;
+      zs.wr(q, d);
+      y = yn;
+      iv = ivn;
+    }
+    double x = d;
+    for (int32_t r = R - 2; r >= 0; --r) {
+      const int64_t q = static_cast<int64_t>(r) * C + k;
+      x = __builtin_fma(-(mg * zs.rd(t0 + q)), x, zs.rd(q));
+      zs.wr(q, x);
+    }
+  }
+}
+
+template <class Z>
+__device__ __forceinline__ void solve_lattice(const DevProblem& P, const Z& zs) {
+  lattice_fold(P, zs);
+  zs.barrier();
+  lattice_transform(P, zs);
+  zs.barrier();
+  lattice_tridiag(P, zs);
+  zs.barrier();
+  lattice_fold(P, zs);
+  zs.barrier();
+  lattice_transform(P, zs);
+  zs.barrier();
+}
+
+// K^-1 b of the face vector in place: the lattice solve or the factor program.
+template <class Z>
+__device__ __forceinline__ void solve_face_system(const DevProblem& P, const Z& zs) {
+  if constexpr (IsFaceStore<Z>::value) {
+    if (P.lat_rows) {
+      solve_lattice(P, zs);
+      return;
+    }
+  }
+  solve(P, zs);
 }
 
 // Two columns of one problem through the same record (their face stores za and
@@ -934,7 +1097,7 @@ __device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_prob
     if (P.Nf > 0) {  // a circuit without faces has nothing to solve (K is 0 x 0)
       face_phase<BS>(P, &R.f, u, zs, w, t);
       zs.barrier();
-      solve(P, zs);
+      solve_face_system(P, zs);
     }
     // finish step t and prepare step t+1 in one pass
     junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
@@ -1167,7 +1330,7 @@ __device__ __forceinline__ void adjoint_steps(const DevProblem& P, const jja_adj
     if (P.Nf > 0) {
       face_phase<BS>(P, nullptr, C.u, zs, w, k);
       zs.barrier();
-      solve(P, zs);
+      solve_face_system(P, zs);
     }
     for (int32_t p = zs.vt; p < P.Nf; p += zs.vbs) gf[p] = gf[p] - kTwoPi * zs.rd(p);
     adjoint_junctions(P, A, C, zs, k, w);
@@ -1318,7 +1481,7 @@ __global__ __launch_bounds__(BS) void spd_solve_kernel(const DevProblem* __restr
     if (q < P.copy_rows) zs.wr(P.Nf + q, v);
   }
   zs.sync();
-  solve(P, zs);
+  solve_face_system(P, zs);
   zs.sync();
   for (int32_t q = threadIdx.x; q < P.Nf; q += BS) xc[fop[q]] = zs.rd(q);
 }
@@ -1451,6 +1614,8 @@ int check_plan_shapes(int32_t n, const jja_problem_desc* descs) {
       return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size/parts");
     if (split && d.index_bytes != 4)
       return fail(JJA_EINVAL, "jja_plan_create: with a split column, every problem needs the global face store");
+    if (split && d.lattice_rows != 0)
+      return fail(JJA_EINVAL, "jja_plan_create: a lattice solve cannot share a plan with split columns");
     if ((d.column_pairs != 0) != (descs[0].column_pairs != 0))
       return fail(JJA_EINVAL, "jja_plan_create: column_pairs must be the same for every problem of a plan");
   }
@@ -1482,7 +1647,9 @@ int validate(const jja_problem_desc& d) {
   for (int64_t e = 0; e < 2 * static_cast<int64_t>(Nj); ++e)
     if (d.junc_code[e] < -1 || (d.junc_code[e] >> 1) >= Nf)
       return fail(JJA_EINVAL, "junc_code out of range");
-  if (Nf == 0) return (d.n_face_slices || d.n_solve_slices) ? fail(JJA_EINVAL, "slices without faces") : JJA_OK;
+  if (Nf == 0)
+    return (d.n_face_slices || d.n_solve_slices || d.lattice_rows) ? fail(JJA_EINVAL, "slices without faces")
+                                                                   : JJA_OK;
   if (!d.face_of_pos || !d.face_code || !d.face_slices || !d.slice_rows ||
       (d.n_solve_slices && (!d.solve_slices || !d.ent_val || !d.ent_off)))
     return fail(JJA_EINVAL, "jja_plan_create: missing face or solve arrays");
@@ -1520,6 +1687,17 @@ int validate(const jja_problem_desc& d) {
     }
     for (int64_t e = m.base; e < m.base + static_cast<int64_t>(m.n_iter) * BS; ++e)
       if (d.face_code[e] < -1 || (d.face_code[e] >> 1) >= Nj) return fail(JJA_EINVAL, "face_code out of range");
+  }
+  if (d.lattice_rows != 0) {
+    const int64_t R = d.lattice_rows, C = d.lattice_cols, H = (C + 1) / 2, ES = H + (H & 1);
+    if (R < 2 || C < 2 || R * C != Nf) return fail(JJA_EINVAL, "lattice rows * columns must equal n_faces (>= 2 each)");
+    if (parts != 1) return fail(JJA_EINVAL, "a lattice solve needs one workgroup per column");
+    if (!d.lattice_sine || !d.lattice_inv) return fail(JJA_EINVAL, "jja_plan_create: missing lattice tables");
+    if (d.n_solve_slices != 0 || d.face_copy_rows != 0)
+      return fail(JJA_EINVAL, "a lattice solve has no solve records and no face copy rows");
+    if (ms < (Nf & 1) + 2 * R * ES) return fail(JJA_EINVAL, "max_stage_rows too small for the lattice solve");
+    if (!(d.lattice_g > 0.0) || !std::isfinite(d.lattice_g)) return fail(JJA_EINVAL, "lattice_g must be finite and > 0");
+    return JJA_OK;
   }
   // solve programs: one per wave of 64 lanes; each runs a multiple of three
   // records and loads four more ahead, all of which must be in range
@@ -1902,6 +2080,17 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     dp.val_bytes = static_cast<int32_t>(d.n_entries * static_cast<int64_t>(sizeof(double)));
     dp.cpr_terms = d.cpr_terms;
     for (int k = 0; k < JJA_CPR_MAX_TERMS; ++k) dp.cpr_coef[k] = d.cpr_coef[k];
+    if (d.lattice_rows) {
+      const int32_t R = d.lattice_rows, C = d.lattice_cols, H = (C + 1) / 2;
+      dp.lat_rows = R;
+      dp.lat_cols = C;
+      dp.lat_es = H + (H & 1);
+      dp.lat_cp = (C + JJA_LATTICE_CHUNK - 1) / JJA_LATTICE_CHUNK * JJA_LATTICE_CHUNK;
+      dp.lat_t0 = static_cast<int64_t>(d.n_faces) + (d.n_faces & 1);
+      dp.lat_g = d.lattice_g;
+      if ((rc = upload(plan, d.lattice_sine, static_cast<size_t>(dp.lat_es) * dp.lat_cp, &dp.lat_sine))) break;
+      if ((rc = upload(plan, d.lattice_inv, static_cast<size_t>(R) * dp.lat_cp, &dp.lat_inv))) break;
+    }
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
     if ((rc = upload(plan, d.Ic, Nj, &dp.Ic))) break;
     if ((rc = upload(plan, d.Rv, Nj, &dp.Rv))) break;
@@ -1952,7 +2141,9 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
     plan->problems.push_back(dp);
     const int64_t store = (static_cast<int64_t>(d.n_faces) + d.max_stage_rows) * static_cast<int64_t>(sizeof(double));
     plan->lds_bytes.push_back(d.index_bytes == 2 ? store : 0);
-    plan->cost.push_back(d.n_entries + d.n_face_entries + 8 * static_cast<int64_t>(d.n_junctions));
+    // work of one column step in factor-entry units (a lattice solve: its multiply-adds)
+    const int64_t solve_work = d.lattice_rows ? static_cast<int64_t>(d.n_faces) * (d.lattice_cols + 1) : d.n_entries;
+    plan->cost.push_back(solve_work + d.n_face_entries + 8 * static_cast<int64_t>(d.n_junctions));
     plan->zbuf_doubles.push_back(d.index_bytes == 2 ? 0 : static_cast<int64_t>(d.n_faces) + d.max_stage_rows);
     plan->max_cols.push_back(d.max_columns);
   }

@@ -172,7 +172,8 @@ class Engine:
             self.host = CircuitPlan(circuit, dt, sw, leaf_size=problem.leaf_size,
                                     block_size=self.block_size,
                                     face_store=getattr(problem, "face_store", "auto"),
-                                    local_stages=getattr(problem, "local_stages", "auto"))
+                                    local_stages=getattr(problem, "local_stages", "auto"),
+                                    solver=getattr(problem, "solver", "auto"))
             parts = getattr(problem, "parts", None)
             if parts is None:
                 parts = auto_parts(self.host, W, sw)
@@ -463,7 +464,8 @@ class BatchEngine:
             self.hosts = [CircuitPlan(p.circuit, p._dt(), sw, leaf_size=p.leaf_size,
                                       block_size=self.block_size,
                                       face_store=getattr(p, "face_store", "auto"),
-                                      local_stages=getattr(p, "local_stages", "auto")) for p in problems]
+                                      local_stages=getattr(p, "local_stages", "auto"),
+                                      solver=getattr(p, "solver", "auto")) for p in problems]
         descs = (_native.ProblemDesc * len(problems))(*[h.desc(max_columns=p.problem_count)
                                                          for h, p in zip(self.hosts, problems)])
         for d, p in zip(descs, problems):

@@ -24,12 +24,14 @@ import scipy.sparse
 import scipy.sparse.csgraph
 
 from . import _native
+from .lattice_solve import lattice_solve
 from .ordering import nested_dissection
 
 MAX_ITER = 8          # entries per lane and solve record (kernel register budget)
 FACE_ITER = 4         # junction codes per thread and face record
 LDS_BYTES = 163840    # dynamic LDS one workgroup may use on MI355X (gfx950)
 WAVE = 64             # lanes of one wavefront: solve records are per wave
+SOLVERS = ("auto", "lattice", "cholesky")
 
 STENCILS = {
     3: ((1.0, -1.0, 0.0), (1.0, -2.0, 1.0)),
@@ -357,9 +359,11 @@ class CircuitPlan:
     on the row."""
 
     def __init__(self, circuit, dt, stencil_width=3, leaf_size=16, block_size=1024, target_iter=8,
-                 face_store="auto", parts=1, local_stages="auto"):
+                 face_store="auto", parts=1, local_stages="auto", solver="auto"):
         if face_store not in ("auto", "lds", "global"):
             raise ValueError("face_store must be 'auto', 'lds' or 'global'")
+        if solver not in SOLVERS:
+            raise ValueError("solver must be 'auto', 'lattice' or 'cholesky'")
         if parts < 1 or parts & (parts - 1) or parts > 32:
             raise ValueError("parts must be a power of two from 1 to 32")
         if parts > 1 and face_store == "lds":
@@ -369,6 +373,7 @@ class CircuitPlan:
         self.face_store = face_store
         self.parts = parts
         self.local_option = local_stages
+        self.solver = solver
         self.column_pairs = False
         if stencil_width not in STENCILS:
             raise ValueError(f"stencil width must be 3, 4 or 5 (equals {stencil_width})")
@@ -388,20 +393,29 @@ class CircuitPlan:
         self.K = face_system_matrix(circuit, dt)
         if self.Nf:
             cx, cy = circuit.get_face_centroids()
+            self.circuit_xy = (np.asarray(cx), np.asarray(cy))
             xy = np.stack([cx, cy], axis=1)
         else:
+            self.circuit_xy = None
             xy = np.zeros((0, 2))
         self.solve_plan = SolvePlan(self.K, xy, leaf_size=leaf_size)
+        self.lattice = lattice_solve(self.K, xy) if self.Nf and solver != "cholesky" else None
         self.target_iter = target_iter
         self._build_layouts(target_iter)
 
-    def relayout(self, block_size, parts=None, face_store=None, local_stages=None):
+    def relayout(self, block_size, parts=None, face_store=None, local_stages=None, solver=None):
         """The same circuit, factor and schedule laid out for another workgroup
-        size, part count, face store or wave-local stage count (the
+        size, part count, face store, wave-local stage count or solver (the
         factorisation depends on none of them)."""
         other = copy.copy(self)
         other.block_size = block_size
         other.column_pairs = False
+        if solver is not None:
+            if solver not in SOLVERS:
+                raise ValueError("solver must be 'auto', 'lattice' or 'cholesky'")
+            if solver != "cholesky" and self.lattice is None and self.Nf:
+                other.lattice = lattice_solve(self.K, self._face_xy())
+            other.solver = solver
         if local_stages is not None:
             if local_stages != "auto" and (int(local_stages) != local_stages or local_stages < 0):
                 raise ValueError("local_stages must be 'auto' or a count >= 0")
@@ -428,6 +442,18 @@ class CircuitPlan:
         other.parts = 2 * self.parts
         other.column_pairs = True
         return other
+
+    @property
+    def uses_lattice(self):
+        """The face system is solved by the lattice solve (a rectangular lattice
+        with one junction weight per direction, one workgroup per column)."""
+        return getattr(self, "_lattice_layout", False)
+
+    def _face_xy(self):
+        if not self.Nf:
+            return np.zeros((0, 2))
+        cx, cy = self.circuit_xy
+        return np.stack([cx, cy], axis=1)
 
     @property
     def column_threads(self):
@@ -555,8 +581,16 @@ class CircuitPlan:
 
     def _build_layouts(self, target_iter=8):
         BS, sp, Nf, Nj = self.column_threads, self.solve_plan, self.Nf, self.Nj
+        self._lattice_layout = False
         if Nf == 0:
             self._build_faceless()
+            return
+        lattice_ok = self.lattice is not None and self.parts == 1 and self.solver != "cholesky"
+        if self.solver == "lattice" and not lattice_ok:
+            raise ValueError("the lattice solve needs a rectangular lattice with one junction weight per "
+                             "direction (no mutual inductance) and one workgroup per column")
+        if lattice_ok:
+            self._build_lattice_layout()
             return
         # the face vector and stage scratch of a column: in the workgroup's LDS
         # when they fit (16-bit indices), else in a global per-column array.
@@ -695,6 +729,51 @@ class CircuitPlan:
         self._face_rec = self._slice_array(self.face_slices)
         self._solve_rec = self._slice_array(self.solve_slices)
         self._wave_ptr = np.ascontiguousarray(self.wave_ptr, dtype=np.int32)
+
+    def _build_lattice_layout(self):
+        """Face records by lattice position (r*C + c) and no solve program: the
+        kernel solves K by the lattice solve (lattice_solve.py)."""
+        lat, Nf, Nj, BS = self.lattice, self.Nf, self.Nj, self.column_threads
+        scratch = lat.scratch_rows(Nf)
+        fits = Nf + scratch <= min(LDS_BYTES // 8, 65536)
+        if self.face_store == "lds" and not fits:
+            raise ValueError(f"circuit with {Nf} faces needs {8 * (Nf + scratch)} bytes of face store per "
+                             f"column; more than one workgroup's LDS ({LDS_BYTES})")
+        self.global_store = self.face_store == "global" or not fits
+        self.index_bytes = 4 if self.global_store else 2
+        self.local_stages = self.part_stages = 0
+        self.scratch_rows = scratch
+        pos = lat.pos_of_face
+        jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
+        jpos = np.where(jc >= 0, 2 * pos[np.maximum(jc, 0) >> 1] + (jc & 1), -1)
+        self.junc_pos_code = jpos.astype(np.int32).ravel()
+        self.face_of_pos = lat.face_of_pos.astype(np.int32)
+        fp = self.face_ptr.astype(np.int64)
+        F = sliced(fp, self.face_code.astype(np.int64), np.ones(int(fp[-1])), BS, pos, fixed_group=1,
+                   target_iter=FACE_ITER)
+        self.face_code_sliced = np.where(F["val"] != 0, F["idx"], -1).astype(np.int32)
+        recs = []
+        for k in range(len(F["base"])):
+            it = int(F["iters"][k])
+            live = int(np.count_nonzero(F["rows"][F["rowbase"][k]:F["rowbase"][k] + BS] >= 0))
+            n_chunks = max(1, -(-it // FACE_ITER))
+            for c in range(n_chunks):
+                flags = 1 | (live << 16)
+                if c == n_chunks - 1:
+                    flags |= _native.SLICE_FINISH
+                recs.append((int(F["base"][k]) + c * FACE_ITER * BS, 0, int(F["rowbase"][k]), 0,
+                             min(FACE_ITER, it - c * FACE_ITER), flags))
+        self.face_slices, self.solve_slices = recs, []
+        self.slice_rows = F["rows"].astype(np.int32)
+        self.ent_val = np.zeros(0)
+        self.ent_off = np.zeros(0, np.uint32 if self.global_store else np.uint16)
+        self.face_copy_rows = 0
+        self.n_slices = self.n_solve_exec = 0
+        self.wave_ptr = np.zeros(BS // WAVE + 1, dtype=np.int64)
+        self._face_rec = self._slice_array(self.face_slices)
+        self._solve_rec = self._slice_array([])
+        self._wave_ptr = np.ascontiguousarray(self.wave_ptr, dtype=np.int32)
+        self._lattice_layout = True
 
     def _build_faceless(self):
         """A circuit without faces (a tree): K is 0 x 0 and the step is
@@ -1010,6 +1089,8 @@ class CircuitPlan:
         d.max_stage_rows = self.scratch_rows
         d.face_copy_rows = self.face_copy_rows
         d.index_bytes = self.index_bytes
+        if self.uses_lattice:
+            self.lattice.set_desc(d)
         return d
 
     def lds_bytes(self):
@@ -1052,6 +1133,8 @@ class CircuitPlan:
                 cp = ok & (pos < self.face_copy_rows)
                 zs[Nf + pos[cp]] = acc[cp]
                 acc = np.zeros(BS)
+        if self.uses_lattice:
+            return self.lattice.emulate(zs[:Nf][self.lattice.pos_of_face])
         return self._emulate_records(zs)
 
     def emulate_solve(self, b):
@@ -1059,6 +1142,8 @@ class CircuitPlan:
         is loaded by solve position (the first stage's scratch too), the
         per-wave record programs run as in emulate_face_and_solve, and K^-1 b
         comes back in face order."""
+        if self.uses_lattice:
+            return self.lattice.emulate(b)
         Nf = self.Nf
         zs = np.zeros(Nf + max(self.scratch_rows, 1))
         pos = self.solve_plan.face_pos.astype(np.int64)
@@ -1179,6 +1264,9 @@ class MatrixPlan(CircuitPlan):
         self.face_store = face_store
         self.parts = 1
         self.local_option = "auto"
+        self.solver = "cholesky"
+        self.lattice = None
+        self.circuit_xy = None
         self.column_pairs = False
         self.Nj, self.Nf = 0, n
         self.stencil_width = 3

@@ -5,7 +5,8 @@ reference (time_evolution.py:20-1067). ``compute()`` keeps the reference's
 pre/post-processing in Python (``time_evolution``, time_evolution.py:422-458) and
 replaces its core loop (``time_evolution_core``, :461-582) by libjja's persistent
 HIP kernel: one workgroup per problem column integrates a chunk of steps, the face
-system is solved with the staged Cholesky schedule, and vortex configurations of
+system is solved with the staged Cholesky schedule (or, for a uniform rectangular
+lattice, by sine transforms and tridiagonal solves), and vortex configurations of
 stored steps are counted on the device.
 
 Extensions (keyword-only, defaults keep reference behaviour):
@@ -42,6 +43,14 @@ Extensions (keyword-only, defaults keep reference behaviour):
                    and workgroups of 512 or 1024 threads. Default False: on
                    SquareArray(512,512) it measured the same speed at W=16 and
                    1% faster at W=32. Results do not depend on it.
+  solver           How the face system K is solved each step: "auto" (default)
+                   takes the lattice solve for a rectangular lattice with one
+                   junction weight per direction (no mutual inductance) and one
+                   workgroup per column -- a sine transform of every lattice
+                   row, one tridiagonal solve per sine mode, the inverse
+                   transform -- and the staged Cholesky factor otherwise;
+                   "lattice" requires the former, "cholesky" forces the latter.
+                   Both are exact solves; results agree to rounding.
   max_resident_output_bytes
                    Device memory for stored steps (default 16 GiB). When the
                    stored theta and current exceed it, they stream to host
@@ -137,7 +146,7 @@ class TimeEvolutionProblem:
                  config_at_minus_2=None, config_at_minus_3=None, config_at_minus_4=None,
                  stencil_width=3, *, noise="philox", seed=None, steps_per_launch=None,
                  block_size=None, leaf_size=16, face_store="auto", parts=None, local_stages="auto",
-                 column_pairs=False, max_resident_output_bytes=None):
+                 column_pairs=False, max_resident_output_bytes=None, solver="auto"):
         self.circuit = circuit
         self.time_step = time_step
         self.time_step_count = time_step_count
@@ -195,6 +204,9 @@ class TimeEvolutionProblem:
         self.local_stages = local_stages
         self.column_pairs = bool(column_pairs)
         self.max_resident_output_bytes = max_resident_output_bytes
+        if solver not in ("auto", "lattice", "cholesky"):
+            raise ValueError("solver must be 'auto', 'lattice' or 'cholesky'")
+        self.solver = solver
 
     # ------------------------------------------------------------- getters
     def get_problem_count(self):

@@ -29,6 +29,18 @@ def _problems():
     return [pa_, pb, pc]
 
 
+def _own(p, solver):
+    """The problem's own compute() with the face-system solver the batch used
+    for it (a batch with split columns solves every problem with the factor;
+    the lattice solve and the factor agree to rounding, not bit for bit)."""
+    saved = p.solver
+    p.solver = solver
+    try:
+        return p.compute()
+    finally:
+        p.solver = saved
+
+
 def _same(r1, r2):
     np.testing.assert_array_equal(r1.get_theta(), r2.get_theta())
     np.testing.assert_array_equal(r1.get_voltage(), r2.get_voltage())
@@ -38,9 +50,11 @@ def _same(r1, r2):
 def test_gpu_batch_matches_individual_runs():
     print("[batch] mixed lattices, LDS and global stores", flush=True)
     probs = _problems()
-    batch = pa.TimeEvolutionBatch(probs, block_size=256).compute()
-    for p, rb in zip(probs, batch):
-        _same(rb, p.compute())
+    tb = pa.TimeEvolutionBatch(probs, block_size=256)
+    batch = tb.compute()
+    assert tb.solvers == ["lattice", "cholesky", "cholesky"]
+    for p, rb, sv in zip(probs, batch, tb.solvers):
+        _same(rb, _own(p, sv))
 
 
 def test_gpu_batch_autotune():
@@ -51,8 +65,8 @@ def test_gpu_batch_autotune():
     assert (tb.tuning["block_size"], tb.tuning["parts"]) in [(128, 1), (256, 1)] + \
         [(1024, q) for q in (2, 4, 8, 16)]
     assert 100 <= tb.tuning["steps_per_launch"] <= 4000
-    for p, rb in zip(probs, batch):
-        _same(rb, p.compute())
+    for p, rb, sv in zip(probs, batch, tb.solvers):
+        _same(rb, _own(p, sv))
 
 
 def test_gpu_separable_source_matches_oracle():
@@ -80,9 +94,11 @@ def test_gpu_batch_split_columns_match_individual_runs():
     global): the results of each problem's own compute()."""
     print("[batch] four parts per column", flush=True)
     probs = _problems()
-    batch = pa.TimeEvolutionBatch(probs, block_size=256, parts=4).compute()
-    for p, rb in zip(probs, batch):
-        _same(rb, p.compute())
+    tb = pa.TimeEvolutionBatch(probs, block_size=256, parts=4)
+    batch = tb.compute()
+    assert tb.solvers == ["cholesky"] * 3
+    for p, rb, sv in zip(probs, batch, tb.solvers):
+        _same(rb, _own(p, sv))
 
 
 def test_gpu_batch_balanced_parts_match_individual_runs():
@@ -92,8 +108,8 @@ def test_gpu_batch_balanced_parts_match_individual_runs():
     probs = _problems()
     tb = pa.TimeEvolutionBatch(probs, block_size=256, parts="balanced", max_parts=8)
     batch = tb.compute()
-    for p, rb in zip(probs, batch):
-        _same(rb, p.compute())
+    for p, rb, sv in zip(probs, batch, tb.solvers):
+        _same(rb, _own(p, sv))
 
 
 def test_gpu_batch_autotune_short_horizon():
@@ -108,8 +124,8 @@ def test_gpu_batch_autotune_short_horizon():
         probs.append(pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=12, external_flux=f,
                                              current_sources=Is, temperature=0.01, seed=4))
     tb = pa.TimeEvolutionBatch(probs, autotune=True, block_sizes=(64, 128), tune_steps=40)
-    for r, p in zip(tb.compute(), probs):
-        _same(r, p.compute())
+    for r, p, sv in zip(tb.compute(), probs, tb.solvers):
+        _same(r, _own(p, sv))
     eng = tb.make_engine()
     try:
         eng.advance(10)
@@ -133,6 +149,7 @@ def test_gpu_batch_autotune_keeps_reference_noise_stream():
         return pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=120, external_flux=0.2,
                                        current_sources=Is, temperature=0.02, noise="reference")
     np.random.seed(1234)
-    batch = pa.TimeEvolutionBatch([problem()], autotune=True, block_sizes=(64, 128), tune_steps=10).compute()
+    tb = pa.TimeEvolutionBatch([problem()], autotune=True, block_sizes=(64, 128), tune_steps=10)
+    batch = tb.compute()
     np.random.seed(1234)
-    _same(batch[0], problem().compute())
+    _same(batch[0], _own(problem(), tb.solvers[0]))

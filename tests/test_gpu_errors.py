@@ -12,10 +12,22 @@ from pyjjasim_amd.engine import Engine
 pytestmark = pytest.mark.gpu
 
 
+def _problem_f(*args, **kwargs):
+    """A problem solved with the factor program (the subject here), also where
+    the lattice solve would take it."""
+    kwargs.setdefault("solver", "cholesky")
+    return pa.TimeEvolutionProblem(*args, **kwargs)
+
+
+def _evolution_f(*args, **kwargs):
+    kwargs.setdefault("solver", "cholesky")
+    return pa.DifferentiableEvolution(*args, **kwargs)
+
+
 def _problem(W=2, Nt=40, parts=4, **kw):
     c = pa.SquareArray(12, 10)
     Is = c.current_base(0.0)[:, None, None] * np.linspace(0.5, 1.2, W)[None, :, None]
-    return pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.1,
+    return _problem_f(c, time_step=0.05, time_step_count=Nt, external_flux=0.1,
                                    current_sources=Is, store_time_steps=[Nt - 1], block_size=256,
                                    face_store="global", parts=parts, **kw)
 
@@ -82,7 +94,7 @@ def test_gpu_plan_busy_on_another_stream_is_refused():
     def problem():
         c = pa.SquareArray(60, 60)
         Is = c.current_base(0.0)[:, None, None] * np.linspace(0.5, 1.5, 64)[None, :, None]
-        return pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=4001, external_flux=0.2,
+        return _problem_f(c, time_step=0.05, time_step_count=4001, external_flux=0.2,
                                        current_sources=Is, store_time_steps=[4000], store_voltage=False,
                                        store_current=False)
     p = problem()
@@ -131,7 +143,7 @@ def test_gpu_split_differentiable_evolution_reports_give_up():
     Is = torch.tensor(c.current_base(0.0)[:, None] * np.array([0.6, 1.3])[None, :], device="cuda")
     grads = []
     for parts in (4, 1):
-        with pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W,
+        with _evolution_f(c, time_step=0.05, time_step_count=Nt, problem_count=W,
                                         store_time_steps=[Nt - 1], checkpoint_every=16, block_size=256,
                                         face_store="global", parts=parts) as evo:
             if parts > 1:

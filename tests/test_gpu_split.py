@@ -10,6 +10,18 @@ import pyjjasim_amd as pa
 pytestmark = pytest.mark.gpu
 
 
+def _problem_f(*args, **kwargs):
+    """A problem solved with the factor program (the subject here), also where
+    the lattice solve would take it."""
+    kwargs.setdefault("solver", "cholesky")
+    return pa.TimeEvolutionProblem(*args, **kwargs)
+
+
+def _evolution_f(*args, **kwargs):
+    kwargs.setdefault("solver", "cholesky")
+    return pa.DifferentiableEvolution(*args, **kwargs)
+
+
 def _circuit():
     c = pa.SquareArray(36, 28)
     rng = np.random.default_rng(21)
@@ -22,7 +34,7 @@ def _circuit():
 def _run(c, parts, T, Nt=150, store="global", local="auto"):
     W = 3
     Is = c.current_base(0.0)[:, None, None] * np.array([0.4, 0.9, 1.3])[None, :, None]
-    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
+    prob = _problem_f(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
                                    current_sources=Is, temperature=T, seed=5,
                                    store_time_steps=[Nt // 2, Nt - 1], block_size=256,
                                    face_store=store, parts=parts, steps_per_launch=60,
@@ -48,7 +60,7 @@ def _engine_run(c, parts, W, Nt=150):
     every part of their column on one XCD, over the run's launches."""
     from pyjjasim_amd.engine import Engine
     Is = c.current_base(0.0)[:, None, None] * np.linspace(0.3, 1.4, W)[None, :, None]
-    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
+    prob = _problem_f(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
                                    current_sources=Is, temperature=0.02, seed=9,
                                    store_time_steps=[Nt // 2, Nt - 1], block_size=256,
                                    face_store="global", parts=parts, steps_per_launch=50)
@@ -100,7 +112,7 @@ def test_gpu_wave_local_stages_bit_identical(store, parts, local):
 
 def _run_pairs(c, W, pairs, Nt=120):
     Is = c.current_base(0.0)[:, None, None] * np.linspace(0.3, 1.4, W)[None, :, None]
-    prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
+    prob = _problem_f(c, time_step=0.05, time_step_count=Nt, external_flux=0.17,
                                    current_sources=Is, temperature=0.02, seed=11,
                                    store_time_steps=[Nt // 2, Nt - 1], block_size=512,
                                    face_store="global", parts=2, steps_per_launch=50,
@@ -142,7 +154,7 @@ def test_gpu_split_adjoint_bit_identical():
     for parts, local, bs, pairs in ((1, 0, 256, False), (4, "auto", 256, False), (1, 99, 256, False),
                                     (2, "auto", 512, True)):
         print(f"[split] adjoint, {parts} parts, local_stages={local}, pairs={pairs}", flush=True)
-        evo = pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W,
+        evo = _evolution_f(c, time_step=0.05, time_step_count=Nt, problem_count=W,
                                          temperature=0.01, store_time_steps=store, checkpoint_every=30,
                                          block_size=bs, face_store="global", parts=parts, seed=3,
                                          local_stages=local, column_pairs=pairs)
@@ -167,7 +179,7 @@ def test_gpu_part_local_stages_bit_identical():
     Is = c.current_base(0.0)[:, None, None] * np.array([0.6, 1.2])[None, :, None]
 
     def run(parts, local):
-        prob = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.13,
+        prob = _problem_f(c, time_step=0.05, time_step_count=Nt, external_flux=0.13,
                                        current_sources=Is, temperature=0.02, seed=13,
                                        store_time_steps=[Nt // 2, Nt - 1], block_size=512,
                                        face_store="global", parts=parts, steps_per_launch=25,
@@ -200,7 +212,7 @@ def test_gpu_part_local_adjoint_bit_identical():
     grads = []
     for parts in (1, 2):
         print(f"[part-local] adjoint, {parts} parts", flush=True)
-        evo = pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W,
+        evo = _evolution_f(c, time_step=0.05, time_step_count=Nt, problem_count=W,
                                          temperature=0.01, store_time_steps=store, checkpoint_every=20,
                                          block_size=512, face_store="global", parts=parts, seed=5)
         if parts == 2:

@@ -10,6 +10,13 @@ import pyjjasim_amd as pa
 from pyjjasim_amd.solver_plan import CircuitPlan, cholesky, cycle_codes
 
 
+def _plan(*args, **kwargs):
+    """A plan with the staged factor program (the subject of these tests), also
+    for lattices the lattice solve would take."""
+    kwargs.setdefault("solver", "cholesky")
+    return CircuitPlan(*args, **kwargs)
+
+
 def _circuits():
     sqL = pa.SquareArray(12, 9)
     sqL.set_inductance(0.8)
@@ -24,7 +31,7 @@ def _circuits():
 @pytest.mark.parametrize("name", sorted(_circuits()))
 def test_staged_solve_matches_direct(name):
     c = _circuits()[name]
-    plan = CircuitPlan(c, 0.05, leaf_size=8)
+    plan = _plan(c, 0.05, leaf_size=8)
     K = scipy.sparse.csc_matrix(plan.K)
     b = np.random.default_rng(0).standard_normal((K.shape[0], 4))
     want = scipy.sparse.linalg.spsolve(K, b)
@@ -51,7 +58,7 @@ def test_cholesky_rejects_indefinite():
 
 def test_schedule_structure():
     c = pa.SquareArray(30, 30)
-    sp = CircuitPlan(c, 0.05, leaf_size=16).solve_plan
+    sp = _plan(c, 0.05, leaf_size=16).solve_plan
     n = sp.n
     assert sp.stage_ptr[0] == 0 and sp.stage_ptr[-1] == n
     # off-block rows only reach earlier stages (lower) or later stages (upper)
@@ -91,7 +98,7 @@ def test_sliced_layout_emulation(name, BS):
     """The exact face phase and staged solve the kernel runs, executed on the host
     from the sliced layouts, equals a direct solve of K J = A u."""
     c = _circuits()[name]
-    plan = CircuitPlan(c, 0.05, leaf_size=8, block_size=BS)
+    plan = _plan(c, 0.05, leaf_size=8, block_size=BS)
     u = np.random.default_rng(2).standard_normal(c.junction_count())
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
     got = plan.emulate_face_and_solve(u)
@@ -102,7 +109,7 @@ def test_sliced_layout_holed_array():
     full = pa.SquareArray(9, 9)
     x, y = full.get_node_coordinates()
     c = full.remove_nodes((x > 2) & (x < 6) & (y > 2) & (y < 6))
-    plan = CircuitPlan(c, 0.05, block_size=64)
+    plan = _plan(c, 0.05, block_size=64)
     u = np.random.default_rng(3).standard_normal(c.junction_count())
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
     np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12,
@@ -117,7 +124,7 @@ def test_plan_descriptors_pass_native_validation(name, BS, store):
     (the ones jja_plan_create runs before any upload); no device is touched."""
     import ctypes as C
     from pyjjasim_amd import _native
-    plan = CircuitPlan(_circuits()[name], 0.05, leaf_size=8, block_size=BS, face_store=store)
+    plan = _plan(_circuits()[name], 0.05, leaf_size=8, block_size=BS, face_store=store)
     assert plan.index_bytes == (4 if store == "global" else 2)
     d = plan.desc()
     NW = BS // 64
@@ -133,7 +140,7 @@ def test_native_validation_rejects_bad_programs():
     lib = _native.lib()
 
     def fresh():
-        return CircuitPlan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128)
+        return _plan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128)
     plan = fresh()
     k = next(i for i in range(len(plan.solve_slices)) if plan.solve_slices[i][4] > 0
              and not plan.solve_slices[i][5] & _native.SLICE_BLOCK)
@@ -141,7 +148,7 @@ def test_native_validation_rejects_bad_programs():
     assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
     assert b"offset" in lib.jja_last_error()
     # a global store's block-phase lane word (base | len << 28) past the store
-    plan = CircuitPlan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128, face_store="global")
+    plan = _plan(_circuits()["sq10"], 0.05, leaf_size=8, block_size=128, face_store="global")
     k = next(i for i in range(len(plan.solve_slices)) if plan.solve_slices[i][5] & _native.SLICE_BLOCK)
     plan.ent_off[plan.solve_slices[k][1]] = (plan.Nf + plan.scratch_rows) | (1 << 28)
     assert lib.jja_plan_validate(1, C.byref(plan.desc())) == _native.JJA_EINVAL
@@ -161,12 +168,12 @@ def test_large_circuit_plans_a_global_face_store():
     """A face vector beyond one workgroup's LDS is planned in global memory with
     32-bit indices, and the host emulation of that program still solves K J = A u."""
     c = pa.SquareArray(130, 130)
-    plan = CircuitPlan(c, 0.05, block_size=1024)
+    plan = _plan(c, 0.05, block_size=1024)
     assert plan.global_store and plan.index_bytes == 4 and plan.ent_off.dtype == np.uint32
     with pytest.raises(ValueError, match="LDS"):
-        CircuitPlan(c, 0.05, block_size=1024, face_store="lds")
+        _plan(c, 0.05, block_size=1024, face_store="lds")
     small = pa.SquareArray(8, 8)
-    p2 = CircuitPlan(small, 0.05, block_size=256, face_store="global")
+    p2 = _plan(small, 0.05, block_size=256, face_store="global")
     u = np.random.default_rng(4).standard_normal(small.junction_count())
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(p2.K), small.get_cycle_matrix() @ u)
     np.testing.assert_allclose(p2.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
@@ -180,7 +187,7 @@ def test_relayout_plan_validates_and_solves(bs):
     from pyjjasim_amd import _native
     a = pa.TriangularArray(9, 7)
     a.set_inductance(0.7)
-    plan = CircuitPlan(a, 0.05, block_size=1024)
+    plan = _plan(a, 0.05, block_size=1024)
     p2 = plan.relayout(bs)
     assert p2.block_size == bs
     assert _native.lib().jja_plan_validate(1, C.byref(p2.desc())) == 0, _native.lib().jja_last_error()
@@ -222,7 +229,7 @@ def test_split_column_plans_validate_and_solve(bs, parts):
     import ctypes as C
     from pyjjasim_amd import _native
     c = _circuits()["tri8"]
-    plan = CircuitPlan(c, 0.05, leaf_size=8, block_size=bs, parts=parts)
+    plan = _plan(c, 0.05, leaf_size=8, block_size=bs, parts=parts)
     assert plan.global_store and plan.column_threads == bs * parts
     d = plan.desc()
     assert d.block_size == bs * parts and d.parts == parts
@@ -230,7 +237,7 @@ def test_split_column_plans_validate_and_solve(bs, parts):
     u = np.random.default_rng(5).standard_normal(c.junction_count())
     want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(plan.K), c.get_cycle_matrix() @ u)
     np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=1e-12, atol=1e-12 * abs(want).max())
-    p1 = CircuitPlan(c, 0.05, leaf_size=8, block_size=bs).relayout(bs, parts)
+    p1 = _plan(c, 0.05, leaf_size=8, block_size=bs).relayout(bs, parts)
     assert p1.column_threads == bs * parts and p1.global_store
 
 
@@ -239,10 +246,10 @@ def test_split_column_plans_rejected_where_unsupported():
     from pyjjasim_amd import _native
     c = _circuits()["sq10"]
     with pytest.raises(ValueError, match="global face store"):
-        CircuitPlan(c, 0.05, block_size=256, parts=2, face_store="lds")
+        _plan(c, 0.05, block_size=256, parts=2, face_store="lds")
     with pytest.raises(ValueError, match="power of two"):
-        CircuitPlan(c, 0.05, block_size=256, parts=3)
-    plan = CircuitPlan(c, 0.05, block_size=256, parts=2)
+        _plan(c, 0.05, block_size=256, parts=3)
+    plan = _plan(c, 0.05, block_size=256, parts=2)
     d = plan.desc()
     d.index_bytes = 2                    # split plans need the global store
     assert _native.lib().jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
@@ -270,15 +277,15 @@ def test_mixed_part_plans_validate():
     size); once one is split, all need the global face store."""
     import ctypes as C
     from pyjjasim_amd import _native
-    a = CircuitPlan(_circuits()["sq10"], 0.05, block_size=256, parts=4)
-    b = CircuitPlan(_circuits()["hc7"], 0.05, block_size=256, face_store="global")
+    a = _plan(_circuits()["sq10"], 0.05, block_size=256, parts=4)
+    b = _plan(_circuits()["hc7"], 0.05, block_size=256, face_store="global")
     descs = (_native.ProblemDesc * 2)(a.desc(), b.desc())
     assert _native.lib().jja_plan_validate(2, descs) == 0, _native.lib().jja_last_error()
-    c = CircuitPlan(_circuits()["hc7"], 0.05, block_size=256)          # LDS store
+    c = _plan(_circuits()["hc7"], 0.05, block_size=256)          # LDS store
     assert not c.global_store
     descs = (_native.ProblemDesc * 2)(a.desc(), c.desc())
     assert _native.lib().jja_plan_validate(2, descs) == _native.JJA_EINVAL
-    d = CircuitPlan(_circuits()["hc7"], 0.05, block_size=128, face_store="global")
+    d = _plan(_circuits()["hc7"], 0.05, block_size=128, face_store="global")
     descs = (_native.ProblemDesc * 2)(a.desc(), d.desc())
     assert _native.lib().jja_plan_validate(2, descs) == _native.JJA_EINVAL
 
@@ -297,9 +304,9 @@ def test_wave_local_stages_emulation(name, BS, parts, store):
         pytest.skip("split columns use the global face store")
     c = _circuits()[name]
     u = np.random.default_rng(6).standard_normal(c.junction_count())
-    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(CircuitPlan(c, 0.05, leaf_size=8).K),
+    want = scipy.sparse.linalg.spsolve(scipy.sparse.csc_matrix(_plan(c, 0.05, leaf_size=8).K),
                                        c.get_cycle_matrix() @ u)
-    base = CircuitPlan(c, 0.05, leaf_size=8, block_size=BS, parts=parts, face_store=store, local_stages=0)
+    base = _plan(c, 0.05, leaf_size=8, block_size=BS, parts=parts, face_store=store, local_stages=0)
     for k in (1, 2, 99):
         plan = base.relayout(BS, local_stages=k)
         assert plan.local_stages == min(k, plan.solve_plan.n_stages)
@@ -315,7 +322,7 @@ def test_wave_local_barrier_count():
     """k wave-local stages replace their 4k - 1 phase barriers (the leaves'
     first phase is skipped) by one per pass."""
     c = pa.TriangularArray(14, 14)
-    base = CircuitPlan(c, 0.05, leaf_size=8, block_size=256, local_stages=0)
+    base = _plan(c, 0.05, leaf_size=8, block_size=256, local_stages=0)
 
     def barriers(plan):
         return sum(1 for r in plan.wave_program(0) if r[5] & 0x400)
@@ -351,12 +358,12 @@ def test_local_stage_choice_and_lds_fit():
     count whose scratch does not fit LDS moves the store to global memory, or
     is refused when LDS is forced."""
     c = _circuits()["tri8"]
-    assert CircuitPlan(c, 0.05, leaf_size=8, block_size=256).local_stages == 0
-    assert CircuitPlan(c, 0.05, leaf_size=8, block_size=256, parts=2).local_stages > 0
+    assert _plan(c, 0.05, leaf_size=8, block_size=256).local_stages == 0
+    assert _plan(c, 0.05, leaf_size=8, block_size=256, parts=2).local_stages > 0
     with pytest.raises(ValueError, match="local_stages"):
-        CircuitPlan(c, 0.05, local_stages=-1)
+        _plan(c, 0.05, local_stages=-1)
     big = pa.SquareArray(103, 103)          # 10404 faces: LDS holds the store only in step
-    p0 = CircuitPlan(big, 0.05, block_size=1024)
+    p0 = _plan(big, 0.05, block_size=1024)
     assert not p0.global_store and p0.local_stages == 0
     pall = p0.relayout(1024, local_stages=99)
     assert pall.global_store and pall.scratch_rows > p0.scratch_rows
@@ -371,7 +378,7 @@ def test_column_pair_plans_validate():
     import ctypes as C
     from pyjjasim_amd import _native
     c = _circuits()["tri8"]
-    split = CircuitPlan(c, 0.05, leaf_size=8, block_size=512, parts=2)
+    split = _plan(c, 0.05, leaf_size=8, block_size=512, parts=2)
     pair = split.paired()
     assert (pair.block_size, pair.parts, pair.column_threads) == (256, 4, split.column_threads)
     assert pair.solve_slices is split.solve_slices
@@ -380,14 +387,14 @@ def test_column_pair_plans_validate():
     assert _native.lib().jja_plan_validate(1, C.byref(d)) == 0, _native.lib().jja_last_error()
     d.column_pairs = 2
     assert _native.lib().jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
-    one = CircuitPlan(c, 0.05, leaf_size=8, block_size=256, face_store="global").desc()
+    one = _plan(c, 0.05, leaf_size=8, block_size=256, face_store="global").desc()
     one.column_pairs = 1                          # pairs need split columns
     assert _native.lib().jja_plan_validate(1, C.byref(one)) == _native.JJA_EINVAL
-    mixed = (_native.ProblemDesc * 2)(pair.desc(), CircuitPlan(c, 0.05, leaf_size=8, block_size=256,
+    mixed = (_native.ProblemDesc * 2)(pair.desc(), _plan(c, 0.05, leaf_size=8, block_size=256,
                                                                   parts=4).desc())
     assert _native.lib().jja_plan_validate(2, mixed) == _native.JJA_EINVAL
     with pytest.raises(ValueError, match="column pairs"):
-        CircuitPlan(c, 0.05, leaf_size=8, block_size=256).paired()
+        _plan(c, 0.05, leaf_size=8, block_size=256).paired()
     assert not split.relayout(512, 2).column_pairs
 
 
