@@ -102,14 +102,32 @@ def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events
     launch_ms = [(k, e0.elapsed_time(e1)) for k, e0, e1 in launches]
     th, _, nv = eng.results()
     sp = eng.host.solve_plan
+    lat = eng.host.lattice if eng.host.uses_lattice else None
     info = dict(Nj=c.junction_count(), Nf=c.face_count(), bytes_per_step=eng.bytes_per_step(),
                 bytes_per_stored_step=8 * c.junction_count() * W,
                 stored_in_window=int(th_mask[warmup:].sum()),
                 nnz_factor=int(len(sp.lo_val) + len(sp.inv_cm)), nnz_L=int(sp.nnz_L),
                 n_stages=sp.n_stages, block_size=eng.block_size, parts=eng.host.parts,
-                steps_per_launch=eng.steps_per_launch)
+                steps_per_launch=eng.steps_per_launch,
+                solver="lattice" if lat is not None else "cholesky",
+                lattice=None if lat is None else {"rows": lat.R, "cols": lat.C,
+                                                  "table_bytes": lat.table_bytes()})
     eng.close()
     return wall, launch_ms, th, nv, c, info
+
+
+def factor_path(pa, torch, n, W, steps):
+    """The same workload with the face system solved by the staged Cholesky
+    factor (the solver for every other circuit): one timed launch of ``steps``
+    steps after one untimed step."""
+    wall, launch_ms, _, _, _, info = run_gpu(pa, torch, n, W, steps, 1, steps, None, solver="cholesky")
+    k, ms = launch_ms[0]
+    achieved = (info["bytes_per_step"] * k + info["bytes_per_stored_step"] * info["stored_in_window"]) / \
+        (ms * 1e-3) / 1e9
+    return {"solver": "cholesky", "steps": k, "ms_per_step": ms / k,
+            "value": info["Nj"] * W * k / (ms * 1e-3), "unit": "junction-steps/s",
+            "bytes_per_step": info["bytes_per_step"], "frac": achieved / HBM_PEAK_GBS,
+            "solve_stages": info["n_stages"], "nnz_factor_per_pass": info["nnz_factor"]}
 
 
 def cpu_baseline(pa, n, W, budget_s):
@@ -401,8 +419,9 @@ TRAFFIC_PROFILE = os.path.join("profiles", "evolve_kernel_square100_W256_traffic
 def measured_traffic():
     """HBM bytes per step of evolve_kernel from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE of one dispatch, each scaled by the
-    copy-kernel factor, divided by the dispatch's steps) and the launch length
-    it was taken at; None when the summary is missing."""
+    copy-kernel factor, divided by the dispatch's steps), the launch length
+    and the face-system solver it was taken with; None when the summary is
+    missing."""
     path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if not os.path.exists(path):
         return None
@@ -413,7 +432,7 @@ def measured_traffic():
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        return float(vals["hbm_bytes_per_step"]), int(vals["steps_per_launch"])
+        return float(vals["hbm_bytes_per_step"]), int(vals["steps_per_launch"]), vals.get("solver", "cholesky")
     except (KeyError, ValueError):
         return None
 
@@ -438,9 +457,9 @@ def host_info():
 
 def plain_best(pa, torch, n, steps, amplitude=1.0, parts_options=(1, 2, 4, 8, 16)):
     """The plain loop: one problem (one bias point, W = 1) at a time, in its
-    fastest single-problem layout: one workgroup with the LDS face store, or the
-    column split over 2..16 workgroups with the global face store. Returns the
-    best rate and every layout's ms/step."""
+    fastest single-problem layout: one workgroup with the LDS face store (the
+    lattice solve), or the column split over 2..16 workgroups with the global
+    face store (the factor). Returns the best rate and every layout's ms/step."""
     per = {}
     for parts in parts_options:
         opts = {"parts": parts} if parts > 1 else {"parts": 1}
@@ -448,7 +467,7 @@ def plain_best(pa, torch, n, steps, amplitude=1.0, parts_options=(1, 2, 4, 8, 16
             opts["face_store"] = "global"
         wall, _, _, _, c, info = run_gpu(pa, torch, n, 1, steps, 100, None, None, events=False,
                                          amplitudes=[amplitude], **opts)
-        per[f"parts{parts}"] = wall / steps * 1e3
+        per[f"parts{parts}_{info['solver']}"] = wall / steps * 1e3
     best = min(per, key=per.get)
     Nj = 2 * n * (n - 1)
     return {"value": Nj / (per[best] * 1e-3), "unit": "junction-steps/s", "layout": best,
@@ -505,6 +524,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-plain-small", action="store_true")
+    ap.add_argument("--solver", choices=["auto", "lattice", "cholesky"], default="auto",
+                    help="face-system solver of the timed run")
+    ap.add_argument("--factor-steps", type=int, default=2000,
+                    help="steps of the same workload with the factor solver, beside a lattice-solve run (0: skip)")
     ap.add_argument("--workload", choices=["iv_sweep", "uneven_batch", "large"], default="iv_sweep",
                     help="iv_sweep: SquareArray(100,100) W=256 (default); uneven_batch: "
                          "64 problems of uneven size in one launch per chunk; large: "
@@ -550,7 +573,8 @@ def main():
 
     wall, launch_ms, th, nv, c, info = run_gpu(pa, torch, args.n, args.W, args.steps, args.warmup,
                                                args.steps_per_launch, args.block_size,
-                                               prelude=plain_loop if args.plain_steps > 0 else None)
+                                               prelude=plain_loop if args.plain_steps > 0 else None,
+                                               solver=args.solver)
     Nj, W, K = info["Nj"], args.W, args.steps
     value = Nj * W * K / wall
     # dominant kernel: evolve_kernel; one launch advances k steps of the whole
@@ -566,8 +590,8 @@ def main():
     traffic = traffic_note = None
     if (args.n, W) == (100, 256):
         mt = measured_traffic()
-        if mt is not None:
-            per_step, prof_len = mt
+        if mt is not None and mt[2] == info["solver"]:
+            per_step, prof_len, _ = mt
             traffic = per_step * tot_steps / n_launch
             traffic_note = (f"{TRAFFIC_PROFILE}: {per_step / 1e6:.1f} MB/step measured on a "
                             f"{prof_len}-step dispatch, times {tot_steps / n_launch:g} steps per launch")
@@ -595,15 +619,21 @@ def main():
                    "beta_C": 1.0, "f": 0.25, "Nt": args.warmup + K,
                    "bias": "SeparableSource(current_base(0), linspace(0, 2, W))",
                    "steps_per_launch": info["steps_per_launch"], "block_size": info["block_size"],
-                   "parts": info["parts"], "solve_stages": info["n_stages"]},
+                   "parts": info["parts"], "solver": info["solver"],
+                   "solve_stages": info["n_stages"] if info["solver"] == "cholesky" else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": "evolve_kernel", "bytes_per_step": info["bytes_per_step"],
                      "bytes_per_launch": bytes_per_launch, "stored_steps_in_window": info["stored_in_window"],
-                     "nnz_factor_per_pass": info["nnz_factor"], "nnz_L": info["nnz_L"],
                      "mean_launch_ms": mean_launch_ms, "launches": n_launch},
         "plain": plain,
     }
+    if info["solver"] == "cholesky":
+        out["roofline"].update({"nnz_factor_per_pass": info["nnz_factor"], "nnz_L": info["nnz_L"]})
+    else:
+        out["roofline"]["lattice"] = info["lattice"]
+    if info["solver"] == "lattice" and args.factor_steps > 0:
+        out["factor_path"] = factor_path(pa, torch, args.n, W, args.factor_steps)
     if not args.no_plain_small:
         out["plain_small"] = plain_small(pa, torch)
     if not args.no_adjoint:

@@ -410,11 +410,15 @@ class Engine:
         (off-block rows of L plus the dense diagonal-block inverses), each pass
         reading an 8-byte value and a 4-byte index; k_par = 5 coefficients per
         junction (Ic, c0, c1, c2, noise scale); sources: terms that vary over
-        the columns, 8 n W bytes each."""
-        sp = self.host.solve_plan
-        nnz = len(sp.lo_val) + len(sp.inv_cm)
+        the columns, 8 n W bytes each. With the lattice solve the factor term is
+        the solve's tables instead (half the sine matrix and the pivots)."""
+        if self.host.uses_lattice:
+            solve = self.host.lattice.table_bytes()
+        else:
+            sp = self.host.solve_plan
+            solve = 24 * (len(sp.lo_val) + len(sp.inv_cm))
         src = sum(s.bytes_per_step for k, s in self.srcs.items() if not s.src.zero)
-        return self.W * (24 * self.Nj + 48 * self.Nf) + 24 * nnz + 8 * self.Nj * 5 + src
+        return self.W * (24 * self.Nj + 48 * self.Nf) + solve + 8 * self.Nj * 5 + src
 
     def bytes_per_stored_step(self):
         """Output bytes of a step whose theta and current are both stored

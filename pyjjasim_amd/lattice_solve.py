@@ -129,9 +129,9 @@ class LatticeSolve:
         return (Nf & 1) + 2 * self.R * self.ES
 
     def table_bytes(self):
-        """Bytes of the tables one column reads per step: the sine table twice
-        and the pivot table once."""
-        return 8 * (2 * self.ES * self.CP + self.R * self.C)
+        """Bytes of the solve's tables (the tabulated half of the sine matrix
+        and the pivots), which every column of a step reads."""
+        return 8 * (self.H * self.C + self.R * self.C)
 
     def set_desc(self, d):
         P = _native.ptr
