@@ -15,8 +15,9 @@
 // history holds x between the junction passes, then the new phase, so the
 // state is stencil_width-1 vectors per column. The junction pass that finishes
 // step t also prepares step t+1. Arithmetic of the step expressions follows the
-// reference's operation order (no FMA contraction), so results agree with the
-// CPU scheme to the rounding of sin and of the linear solve.
+// reference's operation order (no FMA contraction; a division by c0 is a
+// product with the reciprocal 1/c0), so results agree with the CPU scheme to
+// the rounding of sin, of those quotients and of the linear solve.
 //
 // The solve is a linear program of slices whose addresses do not depend on any
 // data: each thread prefetches the entries of the next slice (also across the
@@ -124,6 +125,7 @@ struct DevProblem {
   const int32_t* face_fid;  // [n_slice_rows]: face index of a face record's row slot, or -1
   int32_t parts;            // workgroups per column (1, or more with a global face store)
   int32_t coef_si;          // junction stride of coef: 1, or 0 when every row is one value
+  const double* rc0;        // [Nj] 1 / c0 (stride coef_si)
   int32_t cpr_terms;        // 0: Ic sin(arg); else Ic * sum_k cpr_coef[k] sin((k+1) arg)
   double cpr_coef[JJA_CPR_MAX_TERMS];
   // lattice solve (lat_rows 0: the staged factor program solves K)
@@ -149,6 +151,11 @@ __device__ __forceinline__ double cpr_sum(const DevProblem& P, double arg) {
 // is read from one address).
 __device__ __forceinline__ double coef_at(const DevProblem& P, int k, int32_t j) {
   return gbl(P.coef)[static_cast<int64_t>(k) * P.Nj + static_cast<int64_t>(j) * P.coef_si];
+}
+// 1 / c0 of junction j: the step divides by c0 as a product with its
+// reciprocal (within one rounding of the quotient)
+__device__ __forceinline__ double rc0_at(const DevProblem& P, int32_t j) {
+  return gbl(P.rc0)[static_cast<int64_t>(j) * P.coef_si];
 }
 
 struct WorkItem {
@@ -608,8 +615,13 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
 // Mode k's tridiagonal solve along r, in place in the face vector, with the
 // inverse pivots staged in the (then free) folded-row area: forward
 // d_r = (y_r + g d_{r-1}) inv_r, backward x_r = d_r - ((-g) inv_r) x_{r+1}.
+// One thread per mode; rows go through registers kTriRows at a time, the next
+// block's loads issued before the current block's recurrence and stores.
+constexpr int kTriRows = 8;
+
 template <class Z>
 __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs) {
+  constexpr int TB = kTriRows;
   const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp;
   const int64_t t0 = P.lat_t0;
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
@@ -620,22 +632,51 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
   zs.barrier();
   const double g = P.lat_g, mg = -P.lat_g;
   for (int32_t k = zs.vt; k < C; k += zs.vbs) {
+    // rows of a block past either end read the nearest row and are not used
+    auto load = [&](int32_t r0, int32_t dir, double (&y)[TB], double (&iv)[TB]) {
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        const int32_t r = min(max(r0 + dir * q, 0), R - 1);
+        const int64_t e = static_cast<int64_t>(r) * C + k;
+        y[q] = zs.rd(e);
+        iv[q] = zs.rd(t0 + e);
+      }
+    };
+    double yA[TB], iA[TB], yB[TB] = {}, iB[TB] = {};
     double d = 0.0;
-    double y = zs.rd(k), iv = zs.rd(t0 + k);
-    for (int32_t r = 0; r < R; ++r) {
-      const int64_t q = static_cast<int64_t>(r) * C + k;
-      const bool more = r + 1 < R;
-      const double yn = more ? zs.rd(q + C) : 0.0, ivn = more ? zs.rd(t0 + q + C) : 0.0;
-      d = __builtin_fma(g, d, y) * iv;
-      zs.wr(q, d);
-      y = yn;
-      iv = ivn;
+    load(0, 1, yA, iA);
+    for (int32_t r0 = 0; r0 < R; r0 += TB) {
+      if (r0 + TB < R) load(r0 + TB, 1, yB, iB);
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        if (r0 + q < R) {
+          d = __builtin_fma(g, d, yA[q]) * iA[q];
+          zs.wr(static_cast<int64_t>(r0 + q) * C + k, d);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        yA[q] = yB[q];
+        iA[q] = iB[q];
+      }
     }
+    // backward from row R-2 down to 0, blocks of rows r1, r1-1, ...
     double x = d;
-    for (int32_t r = R - 2; r >= 0; --r) {
-      const int64_t q = static_cast<int64_t>(r) * C + k;
-      x = __builtin_fma(-(mg * zs.rd(t0 + q)), x, zs.rd(q));
-      zs.wr(q, x);
+    load(R - 2, -1, yA, iA);
+    for (int32_t r1 = R - 2; r1 >= 0; r1 -= TB) {
+      if (r1 - TB >= 0) load(r1 - TB, -1, yB, iB);
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        if (r1 - q >= 0) {
+          x = __builtin_fma(-(mg * iA[q]), x, yA[q]);
+          zs.wr(static_cast<int64_t>(r1 - q) * C + k, x);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        yA[q] = yB[q];
+        iA[q] = iB[q];
+      }
     }
   }
 }
@@ -864,7 +905,7 @@ struct JunctionStep {
 template <int SW, bool UPDATE, bool PREP, class Z>
 __device__ __forceinline__ void junction_one(const DevProblem& P, const JunctionStep& J,
                                              const Z& zs, int32_t j, double zj) {
-  const double c0 = coef_at(P, 0, j);
+  const double rc0 = rc0_at(P, j);
   vint2 code{-1, -1};
   double xo = 0.0, isn = 0.0, thsv = 0.0, vsv = 0.0;
   if (UPDATE) {
@@ -890,7 +931,7 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
     double y = 0.0;
     if (code.x >= 0) y = (code.x & 1) ? y - zs.rd(code.x >> 1) : y + zs.rd(code.x >> 1);
     if (code.y >= 0) y = (code.y & 1) ? y - zs.rd(code.y >> 1) : y + zs.rd(code.y >> 1);
-    t[0] = (y - xo) / c0;
+    t[0] = (y - xo) * rc0;
     J.th_n[j] = t[0];
     if (J.th_out) J.th_out[j] = t[0];
     if (J.I_out) J.I_out[j] = y + isn;
@@ -913,7 +954,7 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
     if (!J.t_zero) eta = (J.from_T ? sqrt(2.0 * sg * rv) : sg) * zj;
     const double x = J.is_zero ? eta + X : (eta - isx) + X;
     J.th_x[j] = x;
-    double uj = x / c0;
+    double uj = x * rc0;
     if (J.ths) uj = uj - thsv;
     J.u[j] = uj;
   }
@@ -978,13 +1019,13 @@ __device__ __forceinline__ void junction_loop(const DevProblem& P, const Junctio
 // while they are in flight.
 struct JFast {
   vint2 code;
-  double c0, xo, t2, c1, c2, Ic, isx, sg;
+  double rc0, xo, t2, c1, c2, Ic, isx, sg;
 };
 
 __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionStep& J, int32_t j,
                                            JFast& L) {
   L.code = gbl(reinterpret_cast<const vint2*>(P.junc_code))[j];
-  L.c0 = coef_at(P, 0, j);
+  L.rc0 = rc0_at(P, j);
   L.xo = J.th_n[j];
   L.t2 = J.th_k[1][j];
   L.c1 = coef_at(P, 1, j);
@@ -1000,14 +1041,14 @@ __device__ __forceinline__ void jfast_compute(const DevProblem& P, const Junctio
   double y = 0.0;
   if (L.code.x >= 0) y = (L.code.x & 1) ? y - zs.rd(L.code.x >> 1) : y + zs.rd(L.code.x >> 1);
   if (L.code.y >= 0) y = (L.code.y & 1) ? y - zs.rd(L.code.y >> 1) : y + zs.rd(L.code.y >> 1);
-  const double t1 = (y - L.xo) / L.c0;
+  const double t1 = (y - L.xo) * L.rc0;
   J.th_n[j] = t1;
   const double X = L.Ic * cpr_sum(P, 2.0 * t1 - L.t2) + L.c1 * t1 + L.c2 * L.t2;
   double eta = 0.0;
   if (!J.t_zero) eta = L.sg * z;
   const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
   J.th_x[j] = x;
-  J.u[j] = x / L.c0;
+  J.u[j] = x * L.rc0;
 }
 
 template <int BS, class Z>
@@ -1261,7 +1302,7 @@ __device__ __forceinline__ void adjoint_prime(const DevProblem& P, const jja_adj
   for (int32_t j = zs.vt; j < P.Nj; j += zs.vbs) {
     const double l = lk[j] + adj_g(A, C, k, j);
     lk[j] = l;
-    C.u[j] = l / coef_at(P, 0, j);
+    C.u[j] = l * rc0_at(P, j);
   }
 }
 
@@ -1282,14 +1323,14 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
   GAS double* __restrict__ gIs = gblw(A.grad_Is) + static_cast<size_t>(w) * Nj;
   for (int32_t j = zs.vt; j < Nj; j += zs.vbs) {
     const vint2 code = jcode[j];
-    const double c0 = coef_at(P, 0, j), c1 = coef_at(P, 1, j), c2 = coef_at(P, 2, j);
+    const double rc0 = rc0_at(P, j), c1 = coef_at(P, 1, j), c2 = coef_at(P, 2, j);
     const double lam = lk[j], lprev = lk1[j], t1 = h1[j], t2 = h2[j], Ic = C.Ic[j];
     const double gj = adj_g(A, C, k - 1, j);
     const double gi = gIc[j], gs = gIs[j];
     double y = 0.0;
     if (code.x >= 0) y = (code.x & 1) ? y - zs.rd(code.x >> 1) : y + zs.rd(code.x >> 1);
     if (code.y >= 0) y = (code.y & 1) ? y - zs.rd(code.y >> 1) : y + zs.rd(code.y >> 1);
-    const double mu = (y - lam) / c0;
+    const double mu = (y - lam) * rc0;
     double sn, cs;
     jja_sincos(2.0 * t1 - t2, &sn, &cs);
     const double a = 2.0 * Ic * cs + c1, b = c2 - Ic * cs;
@@ -1298,7 +1339,7 @@ __device__ __forceinline__ void adjoint_junctions(const DevProblem& P, const jja
     lk[j] = b * mu;  // slot k mod 2 now holds lambda_{k-2}'s partial sum
     const double l1 = (lprev + a * mu) + gj;
     lk1[j] = l1;
-    C.u[j] = l1 / c0;
+    C.u[j] = l1 * rc0;
   }
 }
 
@@ -2102,6 +2143,11 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
           dp.coef_si = 1;
           break;
         }
+    {
+      std::vector<double> rc0(Nj);
+      for (size_t j = 0; j < Nj; ++j) rc0[j] = 1.0 / d.coef[j];
+      if ((rc = upload(plan, rc0.data(), Nj, &dp.rc0))) break;
+    }
     if ((rc = upload(plan, d.junc_code, 2 * Nj, &dp.junc_code))) break;
     if (Nf) {
       if ((rc = upload(plan, d.face_of_pos, Nf, &dp.face_of_pos))) break;
