@@ -612,71 +612,59 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
   }
 }
 
-// Mode k's tridiagonal solve along r, in place in the face vector, with the
-// inverse pivots staged in the (then free) folded-row area: forward
-// d_r = (y_r + g d_{r-1}) inv_r, backward x_r = d_r - ((-g) inv_r) x_{r+1}.
-// One thread per mode; rows go through registers kTriRows at a time, the next
-// block's loads issued before the current block's recurrence and stores.
+// Mode k's tridiagonal solve along r, in place in the face vector: forward
+// d_r = (y_r + g d_{r-1}) inv_r, backward x_r = d_r - ((-g) inv_r) x_{r+1}. One
+// thread per mode; rows go kTriRows at a time: the block's face values are
+// read from the store together, and the inverse pivots of the next block are
+// loaded from the table while the current block's recurrence runs.
 constexpr int kTriRows = 8;
 
 template <class Z>
 __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs) {
   constexpr int TB = kTriRows;
   const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp;
-  const int64_t t0 = P.lat_t0;
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
-  for (int32_t e = zs.vt; e < R * C; e += zs.vbs) {
-    const int32_t r = e / C, k = e - r * C;
-    zs.wr(t0 + e, inv[static_cast<int64_t>(r) * CP + k]);
-  }
-  zs.barrier();
   const double g = P.lat_g, mg = -P.lat_g;
   for (int32_t k = zs.vt; k < C; k += zs.vbs) {
-    // rows of a block past either end read the nearest row and are not used
-    auto load = [&](int32_t r0, int32_t dir, double (&y)[TB], double (&iv)[TB]) {
+    // rows of a block past either end are clamped to the nearest row and unused
+    auto row = [&](int32_t r) { return min(max(r, 0), R - 1); };
+    auto load_iv = [&](int32_t r0, int32_t dir, double (&iv)[TB]) {
 #pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        const int32_t r = min(max(r0 + dir * q, 0), R - 1);
-        const int64_t e = static_cast<int64_t>(r) * C + k;
-        y[q] = zs.rd(e);
-        iv[q] = zs.rd(t0 + e);
-      }
+      for (int q = 0; q < TB; ++q) iv[q] = inv[static_cast<int64_t>(row(r0 + dir * q)) * CP + k];
     };
-    double yA[TB], iA[TB], yB[TB] = {}, iB[TB] = {};
+    double ivA[TB], ivB[TB] = {}, y[TB];
     double d = 0.0;
-    load(0, 1, yA, iA);
+    load_iv(0, 1, ivA);
     for (int32_t r0 = 0; r0 < R; r0 += TB) {
-      if (r0 + TB < R) load(r0 + TB, 1, yB, iB);
+      if (r0 + TB < R) load_iv(r0 + TB, 1, ivB);
+#pragma unroll
+      for (int q = 0; q < TB; ++q) y[q] = zs.rd(static_cast<int64_t>(row(r0 + q)) * C + k);
 #pragma unroll
       for (int q = 0; q < TB; ++q) {
         if (r0 + q < R) {
-          d = __builtin_fma(g, d, yA[q]) * iA[q];
+          d = __builtin_fma(g, d, y[q]) * ivA[q];
           zs.wr(static_cast<int64_t>(r0 + q) * C + k, d);
         }
       }
 #pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        yA[q] = yB[q];
-        iA[q] = iB[q];
-      }
+      for (int q = 0; q < TB; ++q) ivA[q] = ivB[q];
     }
-    // backward from row R-2 down to 0, blocks of rows r1, r1-1, ...
+    // backward from row R-2 down to 0: blocks of rows r1, r1-1, ...
     double x = d;
-    load(R - 2, -1, yA, iA);
+    load_iv(R - 2, -1, ivA);
     for (int32_t r1 = R - 2; r1 >= 0; r1 -= TB) {
-      if (r1 - TB >= 0) load(r1 - TB, -1, yB, iB);
+      if (r1 - TB >= 0) load_iv(r1 - TB, -1, ivB);
+#pragma unroll
+      for (int q = 0; q < TB; ++q) y[q] = zs.rd(static_cast<int64_t>(row(r1 - q)) * C + k);
 #pragma unroll
       for (int q = 0; q < TB; ++q) {
         if (r1 - q >= 0) {
-          x = __builtin_fma(-(mg * iA[q]), x, yA[q]);
+          x = __builtin_fma(-(mg * ivA[q]), x, y[q]);
           zs.wr(static_cast<int64_t>(r1 - q) * C + k, x);
         }
       }
 #pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        yA[q] = yB[q];
-        iA[q] = iB[q];
-      }
+      for (int q = 0; q < TB; ++q) ivA[q] = ivB[q];
     }
   }
 }
