@@ -122,6 +122,9 @@ const char* jja_factor_last_error(void);
 #define JJA_CPR_MAX_TERMS 8
 /* sine modes one wave of the lattice solve accumulates at once (even) */
 #define JJA_LATTICE_CHUNK 14
+/* the lattice solve cuts each mode's tridiagonal recurrence into
+ * min(R, JJA_LATTICE_TRI_THREADS / C) blocks */
+#define JJA_LATTICE_TRI_THREADS 1024
 
 typedef struct jja_slice {
   int64_t base;    /* first value (face slices: first code) */
@@ -201,7 +204,8 @@ typedef struct jja_problem_desc {
    * face_copy_rows 0). The reference factorises K with SuperLU for every
    * circuit (time_evolution.py:505-506); this is an exact direct solve of the
    * same K. Needs parts 1, R >= 2, C >= 2, R*C = n_faces and max_stage_rows >=
-   * (n_faces & 1) + 2*R*ES with H = (C+1)/2, ES = H + (H & 1). */
+   * (n_faces & 1) + max(2*R*ES, 3*B*C) with H = (C+1)/2, ES = H + (H & 1) and
+   * B = max(1, min(R, JJA_LATTICE_TRI_THREADS / C)) blocks per tridiagonal solve. */
   int32_t lattice_rows;        /* R (0: no lattice solve) */
   int32_t lattice_cols;        /* C */
   /* [ES][CP], CP = C rounded up to a multiple of JJA_LATTICE_CHUNK:

@@ -612,60 +612,112 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
   }
 }
 
-// Mode k's tridiagonal solve along r, in place in the face vector: forward
-// d_r = (y_r + g d_{r-1}) inv_r, backward x_r = d_r - ((-g) inv_r) x_{r+1}. One
-// thread per mode; rows go kTriRows at a time: the block's face values are
-// read from the store together, and the inverse pivots of the next block are
-// loaded from the table while the current block's recurrence runs.
-constexpr int kTriRows = 8;
+// Mode k's tridiagonal solve along r, in place in the face vector. Both
+// sweeps are first-order linear recurrences with the multiplier
+// a_r = g inv_r: forward d_r = (y_r + g d_{r-1}) inv_r = y_r inv_r + a_r d_{r-1},
+// backward x_r = d_r - ((-g) inv_r) x_{r+1} = d_r + a_r x_{r+1}. The rows of a
+// mode are cut into B blocks (B from R and C only, so results do not depend
+// on the block size): each block runs its recurrence from a zero start while
+// keeping the running product of its multipliers, one thread per mode chains
+// the B block ends, and each block adds its product times the carried value.
+constexpr int kTriRows = 8;  // rows of a block held in registers at a time
+constexpr int kTriThreads = JJA_LATTICE_TRI_THREADS;
 
 template <class Z>
 __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs) {
   constexpr int TB = kTriRows;
   const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp;
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
-  const double g = P.lat_g, mg = -P.lat_g;
+  const double g = P.lat_g;
+  const int32_t B = max(1, min(R, kTriThreads / C)), m = (R + B - 1) / B;
+  // block ends (value, product) and carries in the free folded-row area
+  const int64_t endv = P.lat_t0, endp = endv + static_cast<int64_t>(B) * C, carry = endp + static_cast<int64_t>(B) * C;
+  auto zi = [&](int32_t r, int32_t k) { return static_cast<int64_t>(r) * C + k; };
+  // one pass over rows [r0, r1) of mode k in direction dir (+1 up, -1 down):
+  // f(r, y, a) for each row, rows loaded TB at a time
+  auto sweep = [&](int32_t k, int32_t first, int32_t count, int32_t dir, auto&& f) {
+    for (int32_t c0 = 0; c0 < count; c0 += TB) {
+      double y[TB], a[TB];
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        const int32_t r = first + dir * min(c0 + q, count - 1);
+        y[q] = zs.rd(zi(r, k));
+        a[q] = inv[static_cast<int64_t>(r) * CP + k];
+      }
+#pragma unroll
+      for (int q = 0; q < TB; ++q)
+        if (c0 + q < count) f(first + dir * (c0 + q), y[q], a[q]);
+    }
+  };
+  // forward: local recurrence of every block from zero
+  for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
+    const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
+    if (n <= 0) continue;
+    double d = 0.0, pr = 1.0;
+    sweep(k, r0, n, 1, [&](int32_t r, double y, double iv) {
+      d = __builtin_fma(g, d, y) * iv;
+      pr = pr * (g * iv);
+      zs.wr(zi(r, k), d);
+    });
+    zs.wr(endv + e, d);
+    zs.wr(endp + e, pr);
+  }
+  zs.barrier();
+  // chain the block ends: carry[b] = d entering block b
   for (int32_t k = zs.vt; k < C; k += zs.vbs) {
-    // rows of a block past either end are clamped to the nearest row and unused
-    auto row = [&](int32_t r) { return min(max(r, 0), R - 1); };
-    auto load_iv = [&](int32_t r0, int32_t dir, double (&iv)[TB]) {
-#pragma unroll
-      for (int q = 0; q < TB; ++q) iv[q] = inv[static_cast<int64_t>(row(r0 + dir * q)) * CP + k];
-    };
-    double ivA[TB], ivB[TB] = {}, y[TB];
-    double d = 0.0;
-    load_iv(0, 1, ivA);
-    for (int32_t r0 = 0; r0 < R; r0 += TB) {
-      if (r0 + TB < R) load_iv(r0 + TB, 1, ivB);
-#pragma unroll
-      for (int q = 0; q < TB; ++q) y[q] = zs.rd(static_cast<int64_t>(row(r0 + q)) * C + k);
-#pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        if (r0 + q < R) {
-          d = __builtin_fma(g, d, y[q]) * ivA[q];
-          zs.wr(static_cast<int64_t>(r0 + q) * C + k, d);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < TB; ++q) ivA[q] = ivB[q];
+    double cin = 0.0;
+    zs.wr(carry + k, 0.0);
+    for (int32_t b = 1; b < B; ++b) {
+      cin = __builtin_fma(zs.rd(endp + (b - 1) * C + k), cin, zs.rd(endv + (b - 1) * C + k));
+      zs.wr(carry + static_cast<int64_t>(b) * C + k, cin);
     }
-    // backward from row R-2 down to 0: blocks of rows r1, r1-1, ...
-    double x = d;
-    load_iv(R - 2, -1, ivA);
-    for (int32_t r1 = R - 2; r1 >= 0; r1 -= TB) {
-      if (r1 - TB >= 0) load_iv(r1 - TB, -1, ivB);
-#pragma unroll
-      for (int q = 0; q < TB; ++q) y[q] = zs.rd(static_cast<int64_t>(row(r1 - q)) * C + k);
-#pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        if (r1 - q >= 0) {
-          x = __builtin_fma(-(mg * ivA[q]), x, y[q]);
-          zs.wr(static_cast<int64_t>(r1 - q) * C + k, x);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < TB; ++q) ivA[q] = ivB[q];
+  }
+  zs.barrier();
+  for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
+    const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
+    if (n <= 0 || b == 0) continue;
+    const double cin = zs.rd(carry + e);
+    double pr = 1.0;
+    sweep(k, r0, n, 1, [&](int32_t r, double y, double iv) {
+      pr = pr * (g * iv);
+      zs.wr(zi(r, k), __builtin_fma(pr, cin, y));
+    });
+  }
+  zs.barrier();
+  // backward: local recurrence of every block from zero, top row down
+  for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
+    const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
+    if (n <= 0) continue;
+    double x = 0.0, pr = 1.0;
+    sweep(k, r0 + n - 1, n, -1, [&](int32_t r, double dv, double iv) {
+      const double a = g * iv;
+      x = __builtin_fma(a, x, dv);
+      pr = pr * a;
+      zs.wr(zi(r, k), x);
+    });
+    zs.wr(endv + e, x);
+    zs.wr(endp + e, pr);
+  }
+  zs.barrier();
+  // chain from the last block: carry[b] = x of the row after block b
+  for (int32_t k = zs.vt; k < C; k += zs.vbs) {
+    double cin = 0.0;
+    zs.wr(carry + static_cast<int64_t>(B - 1) * C + k, 0.0);
+    for (int32_t b = B - 2; b >= 0; --b) {
+      cin = __builtin_fma(zs.rd(endp + (b + 1) * C + k), cin, zs.rd(endv + (b + 1) * C + k));
+      zs.wr(carry + static_cast<int64_t>(b) * C + k, cin);
     }
+  }
+  zs.barrier();
+  for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
+    const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
+    if (n <= 0 || b == B - 1) continue;
+    const double cin = zs.rd(carry + e);
+    double pr = 1.0;
+    sweep(k, r0 + n - 1, n, -1, [&](int32_t r, double x, double iv) {
+      pr = pr * (g * iv);
+      zs.wr(zi(r, k), __builtin_fma(pr, cin, x));
+    });
   }
 }
 
@@ -1724,7 +1776,9 @@ int validate(const jja_problem_desc& d) {
     if (!d.lattice_sine || !d.lattice_inv) return fail(JJA_EINVAL, "jja_plan_create: missing lattice tables");
     if (d.n_solve_slices != 0 || d.face_copy_rows != 0)
       return fail(JJA_EINVAL, "a lattice solve has no solve records and no face copy rows");
-    if (ms < (Nf & 1) + 2 * R * ES) return fail(JJA_EINVAL, "max_stage_rows too small for the lattice solve");
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(R, JJA_LATTICE_TRI_THREADS / C));
+    if (ms < (Nf & 1) + std::max(2 * R * ES, 3 * B * C))
+      return fail(JJA_EINVAL, "max_stage_rows too small for the lattice solve");
     if (!(d.lattice_g > 0.0) || !std::isfinite(d.lattice_g)) return fail(JJA_EINVAL, "lattice_g must be finite and > 0");
     return JJA_OK;
   }

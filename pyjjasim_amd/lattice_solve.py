@@ -28,7 +28,10 @@ pairs with itself, F = v there for even j, and S is 0 there for odd j). The
 tridiagonal blocks are factorised once here (Thomas algorithm) and applied per
 step: forward d_r = (y_r + g_r d_{r-1}) inv_r, backward x_r = d_r - up_r x_{r+1}
 with up_r = (-g_r) inv_r formed on the fly (the same product the factorisation
-forms), so only the pivots are tabulated.
+forms), so only the pivots are tabulated. Both sweeps are linear recurrences
+with the multiplier a_r = g_r inv_r; the device cuts each mode's rows into
+blocks that run from a zero start, chains the block ends, and adds each
+block's running product of multipliers times the carried value.
 
 The solve is exact up to rounding, like the reference's SuperLU solve, and is
 used only when the circuit's K equals the Kronecker form to within 8 ulp of its
@@ -44,6 +47,8 @@ from . import _native
 # sine modes one wave accumulates at once (even, so a mode's parity is that of
 # its slot); the sine table's rows are padded to a multiple of it
 CHUNK = 14
+# each mode's tridiagonal recurrence runs in min(R, TRI_THREADS // C) blocks
+TRI_THREADS = 1024
 
 
 def _kron_form(R, C, g_r, g_c):
@@ -124,9 +129,15 @@ class LatticeSolve:
         self.inv = np.ascontiguousarray(inv)
 
     def scratch_rows(self, Nf):
-        """Doubles after the face vector: the folded rows (E and O, R x ES each)
-        at an even offset (16-byte loads)."""
-        return (Nf & 1) + 2 * self.R * self.ES
+        """Doubles after the face vector, at an even offset (16-byte loads): the
+        folded rows (E and O, R x ES each), which the tridiagonal solves reuse
+        for their block ends and carries (3 B C)."""
+        return (Nf & 1) + max(2 * self.R * self.ES, 3 * self.tri_blocks * self.C)
+
+    @property
+    def tri_blocks(self):
+        """Blocks each mode's tridiagonal recurrence is cut into."""
+        return max(1, min(self.R, TRI_THREADS // self.C))
 
     def table_bytes(self):
         """Bytes of the solve's tables (the tabulated half of the sine matrix
@@ -160,6 +171,47 @@ class LatticeSolve:
         out[:, 1::2] = O @ self.sine[:, :self.C][:, 1::2]
         return out
 
+    def _tridiag(self, Y):
+        """Every mode's tridiagonal solve in place in Y (R x C), by the device's
+        blocked recurrences."""
+        R, C, g = self.R, self.C, self.g_r
+        inv = self.inv[:, :C]
+        B = self.tri_blocks
+        m = -(-R // B)
+        blocks = [(b * m, min(R, (b + 1) * m)) for b in range(B) if b * m < R]
+        ends = []
+        for r0, r1 in blocks:                       # forward, each block from zero
+            d, pr = np.zeros(C), np.ones(C)
+            for r in range(r0, r1):
+                d = (Y[r] + g * d) * inv[r]
+                pr = pr * (g * inv[r])
+                Y[r] = d
+            ends.append((d, pr))
+        cin = np.zeros(C)
+        for i, (r0, r1) in enumerate(blocks[1:], start=1):
+            cin = ends[i - 1][1] * cin + ends[i - 1][0]
+            pr = np.ones(C)
+            for r in range(r0, r1):
+                pr = pr * (g * inv[r])
+                Y[r] = pr * cin + Y[r]
+        ends = []
+        for r0, r1 in blocks:                       # backward, each block from zero
+            x, pr = np.zeros(C), np.ones(C)
+            for r in range(r1 - 1, r0 - 1, -1):
+                a = g * inv[r]
+                x = a * x + Y[r]
+                pr = pr * a
+                Y[r] = x
+            ends.append((x, pr))
+        cin = np.zeros(C)
+        for i in range(len(blocks) - 2, -1, -1):
+            cin = ends[i + 1][1] * cin + ends[i + 1][0]
+            r0, r1 = blocks[i]
+            pr = np.ones(C)
+            for r in range(r1 - 1, r0 - 1, -1):
+                pr = pr * (g * inv[r])
+                Y[r] = pr * cin + Y[r]
+
     def emulate(self, b):
         """K^-1 b for b (Nf,) or (Nf, m) in circuit face order, by the kernel's
         steps (its sums agree with these to rounding)."""
@@ -171,14 +223,7 @@ class LatticeSolve:
         for col in range(B.shape[1]):
             Z = B[self.face_of_pos, col].reshape(R, C)
             Y = self._transform(Z)
-            d = np.zeros(C)
-            for r in range(R):
-                d = (Y[r] + self.g_r * d) * self.inv[r, :C]
-                Y[r] = d
-            x = Y[R - 1].copy()
-            for r in range(R - 2, -1, -1):
-                x = Y[r] - ((-self.g_r) * self.inv[r, :C]) * x
-                Y[r] = x
+            self._tridiag(Y)
             X = self._transform(Y)
             out[self.face_of_pos, col] = X.ravel()
         return out[:, 0] if vec else out
