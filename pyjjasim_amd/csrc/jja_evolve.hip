@@ -623,6 +623,37 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
 constexpr int kTriRows = 8;  // rows of a block held in registers at a time
 constexpr int kTriThreads = JJA_LATTICE_TRI_THREADS;
 
+// Chain the block ends of every mode: dir +1, carry[b] = value entering block
+// b from below (carry[0] = 0, carry[b] = endp[b-1] carry[b-1] + endv[b-1]);
+// dir -1, carry[b] = value entering from above (carry[B-1] = 0, from block
+// b+1). The ends are read kTriRows at a time.
+template <class Z>
+__device__ __forceinline__ void lattice_chain(const Z& zs, int32_t C, int32_t B, int64_t endv, int64_t endp,
+                                              int64_t carry, int dir) {
+  constexpr int TB = kTriRows;
+  const int32_t b_first = dir > 0 ? 0 : B - 1;
+  for (int32_t k = zs.vt; k < C; k += zs.vbs) {
+    double cin = 0.0;
+    zs.wr(carry + static_cast<int64_t>(b_first) * C + k, 0.0);
+    for (int32_t i0 = 1; i0 < B; i0 += TB) {
+      double ev[TB], ep[TB];
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        const int32_t src = b_first + dir * (min(i0 + q, B - 1) - 1);
+        ev[q] = zs.rd(endv + static_cast<int64_t>(src) * C + k);
+        ep[q] = zs.rd(endp + static_cast<int64_t>(src) * C + k);
+      }
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        if (i0 + q < B) {
+          cin = __builtin_fma(ep[q], cin, ev[q]);
+          zs.wr(carry + static_cast<int64_t>(b_first + dir * (i0 + q)) * C + k, cin);
+        }
+      }
+    }
+  }
+}
+
 template <class Z>
 __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs) {
   constexpr int TB = kTriRows;
@@ -649,29 +680,22 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
         if (c0 + q < count) f(first + dir * (c0 + q), y[q], a[q]);
     }
   };
-  // forward: local recurrence of every block from zero
+  // forward: local recurrence of every block from zero (an empty block at the
+  // end passes its carry through: value 0, product 1)
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
-    if (n <= 0) continue;
     double d = 0.0, pr = 1.0;
-    sweep(k, r0, n, 1, [&](int32_t r, double y, double iv) {
-      d = __builtin_fma(g, d, y) * iv;
-      pr = pr * (g * iv);
-      zs.wr(zi(r, k), d);
-    });
+    if (n > 0)
+      sweep(k, r0, n, 1, [&](int32_t r, double y, double iv) {
+        d = __builtin_fma(g, d, y) * iv;
+        pr = pr * (g * iv);
+        zs.wr(zi(r, k), d);
+      });
     zs.wr(endv + e, d);
     zs.wr(endp + e, pr);
   }
   zs.barrier();
-  // chain the block ends: carry[b] = d entering block b
-  for (int32_t k = zs.vt; k < C; k += zs.vbs) {
-    double cin = 0.0;
-    zs.wr(carry + k, 0.0);
-    for (int32_t b = 1; b < B; ++b) {
-      cin = __builtin_fma(zs.rd(endp + (b - 1) * C + k), cin, zs.rd(endv + (b - 1) * C + k));
-      zs.wr(carry + static_cast<int64_t>(b) * C + k, cin);
-    }
-  }
+  lattice_chain(zs, C, B, endv, endp, carry, 1);
   zs.barrier();
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
@@ -687,27 +711,19 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
   // backward: local recurrence of every block from zero, top row down
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
-    if (n <= 0) continue;
     double x = 0.0, pr = 1.0;
-    sweep(k, r0 + n - 1, n, -1, [&](int32_t r, double dv, double iv) {
-      const double a = g * iv;
-      x = __builtin_fma(a, x, dv);
-      pr = pr * a;
-      zs.wr(zi(r, k), x);
-    });
+    if (n > 0)
+      sweep(k, r0 + n - 1, n, -1, [&](int32_t r, double dv, double iv) {
+        const double a = g * iv;
+        x = __builtin_fma(a, x, dv);
+        pr = pr * a;
+        zs.wr(zi(r, k), x);
+      });
     zs.wr(endv + e, x);
     zs.wr(endp + e, pr);
   }
   zs.barrier();
-  // chain from the last block: carry[b] = x of the row after block b
-  for (int32_t k = zs.vt; k < C; k += zs.vbs) {
-    double cin = 0.0;
-    zs.wr(carry + static_cast<int64_t>(B - 1) * C + k, 0.0);
-    for (int32_t b = B - 2; b >= 0; --b) {
-      cin = __builtin_fma(zs.rd(endp + (b + 1) * C + k), cin, zs.rd(endv + (b + 1) * C + k));
-      zs.wr(carry + static_cast<int64_t>(b) * C + k, cin);
-    }
-  }
+  lattice_chain(zs, C, B, endv, endp, carry, -1);
   zs.barrier();
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
@@ -721,13 +737,113 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
   }
 }
 
+// The same blocked solve when every (mode, block) pair has a thread and a
+// block has at most kTriCache rows: the thread's inverse pivots were loaded
+// into registers (ivc) before the first transform, and each sweep issues its
+// block's face-store reads at once.
+constexpr int kTriCache = 16;
+
+template <class Z>
+__device__ __forceinline__ void lattice_tridiag_cached(const DevProblem& P, const Z& zs,
+                                                       const double (&ivc)[kTriCache]) {
+  constexpr int TC = kTriCache;
+  const int32_t R = P.lat_rows, C = P.lat_cols;
+  const double g = P.lat_g;
+  const int32_t B = max(1, min(R, kTriThreads / C)), m = (R + B - 1) / B;
+  const int64_t endv = P.lat_t0, endp = endv + static_cast<int64_t>(B) * C, carry = endp + static_cast<int64_t>(B) * C;
+  const int32_t e = zs.vt;
+  const bool active = e < B * C;
+  const int32_t b = active ? e / C : 0, k = active ? e - b * C : 0, r0 = b * m, n = active ? min(m, R - r0) : 0;
+  auto zi = [&](int32_t q) { return static_cast<int64_t>(min(r0 + q, R - 1)) * C + k; };
+  double y[TC];
+  auto load = [&]() {
+#pragma unroll
+    for (int q = 0; q < TC; ++q)
+      if (q < n) y[q] = zs.rd(zi(q));
+  };
+  if (active) {
+    load();
+    double d = 0.0, pr = 1.0;
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+      if (q < n) {
+        d = __builtin_fma(g, d, y[q]) * ivc[q];
+        pr = pr * (g * ivc[q]);
+        zs.wr(zi(q), d);
+      }
+    }
+    zs.wr(endv + e, d);
+    zs.wr(endp + e, pr);
+  }
+  zs.barrier();
+  lattice_chain(zs, C, B, endv, endp, carry, 1);
+  zs.barrier();
+  if (active && b > 0) {
+    const double cin = zs.rd(carry + e);
+    load();
+    double pr = 1.0;
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+      if (q < n) {
+        pr = pr * (g * ivc[q]);
+        zs.wr(zi(q), __builtin_fma(pr, cin, y[q]));
+      }
+    }
+  }
+  zs.barrier();
+  if (active) {
+    load();
+    double x = 0.0, pr = 1.0;
+#pragma unroll
+    for (int q = TC - 1; q >= 0; --q) {
+      if (q < n) {
+        const double a = g * ivc[q];
+        x = __builtin_fma(a, x, y[q]);
+        pr = pr * a;
+        zs.wr(zi(q), x);
+      }
+    }
+    zs.wr(endv + e, x);
+    zs.wr(endp + e, pr);
+  }
+  zs.barrier();
+  lattice_chain(zs, C, B, endv, endp, carry, -1);
+  zs.barrier();
+  if (active && b < B - 1) {
+    const double cin = zs.rd(carry + e);
+    load();
+    double pr = 1.0;
+#pragma unroll
+    for (int q = TC - 1; q >= 0; --q) {
+      if (q < n) {
+        pr = pr * (g * ivc[q]);
+        zs.wr(zi(q), __builtin_fma(pr, cin, y[q]));
+      }
+    }
+  }
+}
+
 template <class Z>
 __device__ __forceinline__ void solve_lattice(const DevProblem& P, const Z& zs) {
+  const int32_t R = P.lat_rows, C = P.lat_cols;
+  const int32_t B = max(1, min(R, kTriThreads / C)), m = (R + B - 1) / B;
+  const bool cached = m <= kTriCache && B * C <= zs.vbs;
+  double ivc[kTriCache];
+  if (cached) {
+    // this thread's inverse pivots, loaded while the fold and transform run
+    const GAS double* __restrict__ inv = gbl(P.lat_inv);
+    const int32_t e = min(zs.vt, B * C - 1), b = e / C, k = e - b * C, r0 = b * m;
+#pragma unroll
+    for (int q = 0; q < kTriCache; ++q) ivc[q] = inv[static_cast<int64_t>(min(r0 + q, R - 1)) * P.lat_cp + k];
+  }
   lattice_fold(P, zs);
   zs.barrier();
   lattice_transform(P, zs);
   zs.barrier();
-  lattice_tridiag(P, zs);
+  if (cached)
+    lattice_tridiag_cached(P, zs, ivc);
+  else
+    lattice_tridiag(P, zs);
   zs.barrier();
   lattice_fold(P, zs);
   zs.barrier();
