@@ -737,113 +737,13 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
   }
 }
 
-// The same blocked solve when every (mode, block) pair has a thread and a
-// block has at most kTriCache rows: the thread's inverse pivots were loaded
-// into registers (ivc) before the first transform, and each sweep issues its
-// block's face-store reads at once.
-constexpr int kTriCache = 16;
-
-template <class Z>
-__device__ __forceinline__ void lattice_tridiag_cached(const DevProblem& P, const Z& zs,
-                                                       const double (&ivc)[kTriCache]) {
-  constexpr int TC = kTriCache;
-  const int32_t R = P.lat_rows, C = P.lat_cols;
-  const double g = P.lat_g;
-  const int32_t B = max(1, min(R, kTriThreads / C)), m = (R + B - 1) / B;
-  const int64_t endv = P.lat_t0, endp = endv + static_cast<int64_t>(B) * C, carry = endp + static_cast<int64_t>(B) * C;
-  const int32_t e = zs.vt;
-  const bool active = e < B * C;
-  const int32_t b = active ? e / C : 0, k = active ? e - b * C : 0, r0 = b * m, n = active ? min(m, R - r0) : 0;
-  auto zi = [&](int32_t q) { return static_cast<int64_t>(min(r0 + q, R - 1)) * C + k; };
-  double y[TC];
-  auto load = [&]() {
-#pragma unroll
-    for (int q = 0; q < TC; ++q)
-      if (q < n) y[q] = zs.rd(zi(q));
-  };
-  if (active) {
-    load();
-    double d = 0.0, pr = 1.0;
-#pragma unroll
-    for (int q = 0; q < TC; ++q) {
-      if (q < n) {
-        d = __builtin_fma(g, d, y[q]) * ivc[q];
-        pr = pr * (g * ivc[q]);
-        zs.wr(zi(q), d);
-      }
-    }
-    zs.wr(endv + e, d);
-    zs.wr(endp + e, pr);
-  }
-  zs.barrier();
-  lattice_chain(zs, C, B, endv, endp, carry, 1);
-  zs.barrier();
-  if (active && b > 0) {
-    const double cin = zs.rd(carry + e);
-    load();
-    double pr = 1.0;
-#pragma unroll
-    for (int q = 0; q < TC; ++q) {
-      if (q < n) {
-        pr = pr * (g * ivc[q]);
-        zs.wr(zi(q), __builtin_fma(pr, cin, y[q]));
-      }
-    }
-  }
-  zs.barrier();
-  if (active) {
-    load();
-    double x = 0.0, pr = 1.0;
-#pragma unroll
-    for (int q = TC - 1; q >= 0; --q) {
-      if (q < n) {
-        const double a = g * ivc[q];
-        x = __builtin_fma(a, x, y[q]);
-        pr = pr * a;
-        zs.wr(zi(q), x);
-      }
-    }
-    zs.wr(endv + e, x);
-    zs.wr(endp + e, pr);
-  }
-  zs.barrier();
-  lattice_chain(zs, C, B, endv, endp, carry, -1);
-  zs.barrier();
-  if (active && b < B - 1) {
-    const double cin = zs.rd(carry + e);
-    load();
-    double pr = 1.0;
-#pragma unroll
-    for (int q = TC - 1; q >= 0; --q) {
-      if (q < n) {
-        pr = pr * (g * ivc[q]);
-        zs.wr(zi(q), __builtin_fma(pr, cin, y[q]));
-      }
-    }
-  }
-}
-
 template <class Z>
 __device__ __forceinline__ void solve_lattice(const DevProblem& P, const Z& zs) {
-  const int32_t R = P.lat_rows, C = P.lat_cols;
-  const int32_t B = max(1, min(R, kTriThreads / C)), m = (R + B - 1) / B;
-  const bool cached = m <= kTriCache && B * C <= zs.vbs;
-  double ivc[kTriCache];
-  if (cached) {
-    // this thread's inverse pivots, loaded while the fold and transform run
-    const GAS double* __restrict__ inv = gbl(P.lat_inv);
-    const int32_t e = min(zs.vt, B * C - 1), b = e / C, k = e - b * C, r0 = b * m;
-#pragma unroll
-    for (int q = 0; q < kTriCache; ++q) ivc[q] = inv[static_cast<int64_t>(min(r0 + q, R - 1)) * P.lat_cp + k];
-  }
   lattice_fold(P, zs);
   zs.barrier();
   lattice_transform(P, zs);
   zs.barrier();
-  if (cached)
-    lattice_tridiag_cached(P, zs, ivc);
-  else
-    lattice_tridiag(P, zs);
+  lattice_tridiag(P, zs);
   zs.barrier();
   lattice_fold(P, zs);
   zs.barrier();
