@@ -562,19 +562,23 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    # the plain loop (one problem at a time) is measured between the batched
-    # run's host set-up and its warmup steps; its few seconds of GPU work also
-    # bring the device to its working clock right before the timed steps
-    plain = None
+    # the plain loop (one problem at a time) and the factor path are measured
+    # between the batched run's host set-up and its warmup steps; their GPU work
+    # also brings the device to its working clock right before the timed steps
+    plain = factor = None
 
-    def plain_loop():
-        nonlocal plain
-        plain = plain_best(pa, torch, args.n, args.plain_steps)
+    def prelude():
+        # the plain loop, then the same batch on the factor path (full width, so
+        # the device is at its loaded clock when the timed steps start)
+        nonlocal plain, factor
+        if args.plain_steps > 0:
+            plain = plain_best(pa, torch, args.n, args.plain_steps)
+        if args.solver != "cholesky" and args.factor_steps > 0:
+            factor = factor_path(pa, torch, args.n, args.W, args.factor_steps)
 
     wall, launch_ms, th, nv, c, info = run_gpu(pa, torch, args.n, args.W, args.steps, args.warmup,
                                                args.steps_per_launch, args.block_size,
-                                               prelude=plain_loop if args.plain_steps > 0 else None,
-                                               solver=args.solver)
+                                               prelude=prelude, solver=args.solver)
     Nj, W, K = info["Nj"], args.W, args.steps
     value = Nj * W * K / wall
     # dominant kernel: evolve_kernel; one launch advances k steps of the whole
@@ -632,8 +636,8 @@ def main():
         out["roofline"].update({"nnz_factor_per_pass": info["nnz_factor"], "nnz_L": info["nnz_L"]})
     else:
         out["roofline"]["lattice"] = info["lattice"]
-    if info["solver"] == "lattice" and args.factor_steps > 0:
-        out["factor_path"] = factor_path(pa, torch, args.n, W, args.factor_steps)
+    if info["solver"] == "lattice" and factor is not None:
+        out["factor_path"] = factor
     if not args.no_plain_small:
         out["plain_small"] = plain_small(pa, torch)
     if not args.no_adjoint:
