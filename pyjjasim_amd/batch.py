@@ -73,7 +73,8 @@ def _plan_one(args):
 def build_plans(problems, block_size, workers=None):
     """Host plans (ordering, factor, record layouts) of the problems' circuits,
     built in up to ``workers`` processes (default: the CPUs this process may run
-    on, at most 16; one process when the batch is small). Worker processes
+    on, at most 15, so that with this process at most 16 processes hold the
+    device runtime; one process when the batch is small). Worker processes
     only run host code; they are started fresh (spawn), never forked from a
     process that may hold a device context."""
     jobs = [(p.circuit, p._dt(), p.stencil_width, p.leaf_size, block_size,
@@ -81,7 +82,7 @@ def build_plans(problems, block_size, workers=None):
              getattr(p, "solver", "auto")) for p in problems]
     size = sum(p.circuit._Nj() for p in problems)
     if workers is None:
-        workers = min(16, len(os.sched_getaffinity(0)), len(jobs)) if size > 200_000 else 1
+        workers = min(15, len(os.sched_getaffinity(0)), len(jobs)) if size > 200_000 else 1
     if workers <= 1:
         return [_plan_one(j) for j in jobs]
     # largest first, so the pool's tail is short
