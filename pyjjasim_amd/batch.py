@@ -46,13 +46,21 @@ __all__ = ["TimeEvolutionBatch", "build_plans"]
 _THREAD_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "BLIS_NUM_THREADS")
 
 
+def _split_cost(h):
+    """Work of one column step of a problem in a split launch (which solves
+    with the factor, whatever the problem's own plan uses): the factor entries
+    both passes stream, plus the junction work."""
+    sp = h.solve_plan
+    return float(2 * (len(sp.lo_val) + len(sp.inv_cm)) + 8 * h.Nj)
+
+
 def balanced_parts(hosts, columns, n_cu, max_parts=16):
     """Parts per problem for one split launch: all start at one, and the problem
-    whose columns take longest (factor entries / parts) has its parts doubled
+    whose columns take longest (split cost / parts) has its parts doubled
     while every workgroup of the launch still fits on the device."""
     parts = [1] * len(hosts)
     used = sum(columns)
-    heap = [(-float(len(h.ent_val) + 8 * h.Nj), i) for i, h in enumerate(hosts)]
+    heap = [(-_split_cost(h), i) for i, h in enumerate(hosts)]
     heapq.heapify(heap)
     while heap:
         _, i = heapq.heappop(heap)
@@ -60,7 +68,7 @@ def balanced_parts(hosts, columns, n_cu, max_parts=16):
             break                      # the slowest problem cannot speed up further
         used += columns[i] * parts[i]
         parts[i] *= 2
-        heapq.heappush(heap, (-float(len(hosts[i].ent_val) + 8 * hosts[i].Nj) / parts[i], i))
+        heapq.heappush(heap, (-_split_cost(hosts[i]) / parts[i], i))
     return parts
 
 

@@ -155,3 +155,17 @@ def test_solver_choice():
         CircuitPlan(c, 0.05, solver="lattice", parts=2)
     with pytest.raises(ValueError):
         CircuitPlan(c, 0.05, solver="fft")
+
+
+def test_split_balancing_cost_does_not_depend_on_the_solver():
+    """A batch with split columns solves every problem with the factor, so the
+    part balancing prices a lattice problem by its factor, not by its lattice
+    plan (which streams no factor entries)."""
+    from pyjjasim_amd.batch import _split_cost, balanced_parts
+    c = pa.SquareArray(40, 40)
+    lat = CircuitPlan(c, 0.05, block_size=1024)
+    chol = CircuitPlan(c, 0.05, block_size=1024, solver="cholesky")
+    assert lat.uses_lattice and not chol.uses_lattice
+    assert _split_cost(lat) == _split_cost(chol)
+    hc = CircuitPlan(pa.HoneycombArray(30, 30), 0.05, block_size=1024)
+    assert balanced_parts([lat, hc], [1, 1], 256) == balanced_parts([chol, hc], [1, 1], 256)
