@@ -68,7 +68,7 @@ def make_problem(pa, n, W, Nt, T=0.01, seed=12345, steps_per_launch=None, block_
 
 
 def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events=True, amplitudes=None,
-            prelude=None, **engine_options):
+            prelude=None, download=True, **engine_options):
     """The timed batched run. ``prelude`` (GPU work of its own, such as the
     plain loop) runs after the host has built this run's plan and before the
     warmup steps, so that no host set-up lies between it and the timed steps."""
@@ -100,7 +100,7 @@ def run_gpu(pa, torch, n, W, steps, warmup, steps_per_launch, block_size, events
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     launch_ms = [(k, e0.elapsed_time(e1)) for k, e0, e1 in launches]
-    th, _, nv = eng.results()
+    th, _, nv = eng.results() if download else (None, None, None)
     sp = eng.host.solve_plan
     lat = eng.host.lattice if eng.host.uses_lattice else None
     info = dict(Nj=c.junction_count(), Nf=c.face_count(), bytes_per_step=eng.bytes_per_step(),
@@ -120,7 +120,8 @@ def factor_path(pa, torch, n, W, steps):
     """The same workload with the face system solved by the staged Cholesky
     factor (the solver for every other circuit): one timed launch of ``steps``
     steps after one untimed step."""
-    wall, launch_ms, _, _, _, info = run_gpu(pa, torch, n, W, steps, 1, steps, None, solver="cholesky")
+    wall, launch_ms, _, _, _, info = run_gpu(pa, torch, n, W, steps, 1, steps, None, solver="cholesky",
+                                             download=False)
     k, ms = launch_ms[0]
     achieved = (info["bytes_per_step"] * k + info["bytes_per_stored_step"] * info["stored_in_window"]) / \
         (ms * 1e-3) / 1e9
