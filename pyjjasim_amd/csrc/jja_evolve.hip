@@ -1070,9 +1070,9 @@ __device__ __forceinline__ void junction_loop(const DevProblem& P, const Junctio
 // ------------------------------------------------------------ common step
 // The usual step (stencil 3, no voltage sources, nothing stored at step t,
 // noise amplitude given per junction, Philox or no noise): the same
-// arithmetic as junction_one over the same junction order, with the loads of
-// both junctions of an iteration issued first and the Philox draws computed
-// while they are in flight.
+// arithmetic as junction_one, each lane taking a junction pair, with the loads
+// of both junctions issued first and the pair's Philox draw computed while
+// they are in flight.
 struct JFast {
   vint2 code;
   double rc0, xo, t2, c1, c2, Ic, isx, sg;
@@ -1113,17 +1113,19 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
   const int32_t tid = J.vt, vbs = J.vbs;
   const int32_t last = Nj - 1;
   const bool gauss = !J.t_zero;
-  for (int32_t k = 0; k * vbs < Nj; k += 2) {
-    const int32_t jA = k * vbs + tid, jB = jA + vbs;
-    // loads of junctions past the end read the last junction and are not used
+  // a lane takes the junction pair (2q, 2q+1): its own Philox draw gives both
+  // deviates (the pair's counter, as in the general loop)
+  for (int32_t q = tid; 2 * q < Nj; q += vbs) {
+    const int32_t jA = 2 * q, jB = jA + 1;
+    // a pair past the end (odd Nj) reads the last junction and does not use it
     JFast A, B;
-    jfast_load(P, J, min(jA, last), A);
+    jfast_load(P, J, jA, A);
     jfast_load(P, J, min(jB, last), B);
     __builtin_amdgcn_sched_barrier(0);
-    // the Philox draws run while the loads are in flight
+    // the Philox draw runs while the loads are in flight
     double zA = 0.0, zB = 0.0;
-    if (gauss) wave_normals<BS>(J, k, zA, zB);
-    if (jA < Nj) jfast_compute(P, J, zs, jA, A, zA);
+    if (gauss) normal_pair(J.seed, static_cast<uint32_t>(jA), J.noise_col, J.tx, &zA, &zB);
+    jfast_compute(P, J, zs, jA, A, zA);
     if (jB < Nj) jfast_compute(P, J, zs, jB, B, zB);
   }
 }
