@@ -263,7 +263,9 @@ def test_balanced_parts_allocation():
     workgroup; small problems stay at one part."""
     from types import SimpleNamespace
     from pyjjasim_amd.batch import balanced_parts
-    hosts = [SimpleNamespace(ent_val=np.zeros(n), Nj=0) for n in (64000, 16000, 1000, 1000)]
+    # split cost 2 * (off-block + block-inverse entries) of the factor
+    hosts = [SimpleNamespace(solve_plan=SimpleNamespace(lo_val=np.zeros(n // 2), inv_cm=np.zeros(0)), Nj=0)
+             for n in (64000, 16000, 1000, 1000)]
     parts = balanced_parts(hosts, [1, 1, 1, 1], n_cu=16, max_parts=16)
     # 1,1,1,1 (4 workgroups) -> first 2, 4, 8 for the largest (cost 64000 / parts),
     # then 2 for the second; doubling the largest again would need 20 > 16
