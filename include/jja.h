@@ -123,7 +123,7 @@ const char* jja_factor_last_error(void);
 /* sine modes one wave of the lattice solve accumulates at once (even) */
 #define JJA_LATTICE_CHUNK 14
 /* the lattice solve cuts each mode's tridiagonal recurrence into
- * min(R, JJA_LATTICE_TRI_THREADS / C) blocks */
+ * min(R, JJA_LATTICE_TRI_THREADS * parts / C) blocks */
 #define JJA_LATTICE_TRI_THREADS 1024
 
 typedef struct jja_slice {
@@ -203,9 +203,10 @@ typedef struct jja_problem_desc {
    * place of the staged factor program (which is then empty: n_solve_slices 0,
    * face_copy_rows 0). The reference factorises K with SuperLU for every
    * circuit (time_evolution.py:505-506); this is an exact direct solve of the
-   * same K. Needs parts 1, R >= 2, C >= 2, R*C = n_faces and max_stage_rows >=
-   * (n_faces & 1) + max(2*R*ES, 3*B*C) with H = (C+1)/2, ES = H + (H & 1) and
-   * B = max(1, min(R, JJA_LATTICE_TRI_THREADS / C)) blocks per tridiagonal solve. */
+   * same K. Needs column_pairs 0, R >= 2, C >= 2, R*C = n_faces and
+   * max_stage_rows >= (n_faces & 1) + max(2*R*ES, 3*B*C) with H = (C+1)/2,
+   * ES = H + (H & 1) and B = max(1, min(R, JJA_LATTICE_TRI_THREADS * parts / C))
+   * blocks per tridiagonal solve. */
   int32_t lattice_rows;        /* R (0: no lattice solve) */
   int32_t lattice_cols;        /* C */
   /* [ES][CP], CP = C rounded up to a multiple of JJA_LATTICE_CHUNK:

@@ -307,6 +307,9 @@ struct SplitStore {
   __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
   __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
+  __device__ __forceinline__ vdouble2 rd2(int64_t i) const {
+    return *reinterpret_cast<const GAS vdouble2*>(p + i);
+  }
   // Every wave waits for its own memory operations (stores reach the L2,
   // which the L1 writes through to), the workgroup meets, and thread 0 alone
   // arrives on the column's counter, polls it with relaxed loads (an acquire
@@ -538,12 +541,18 @@ __device__ __forceinline__ void solve(const DevProblem& P, const Z& zs) {
 // each mode's recurrence runs in one thread, so results depend neither on the
 // block size nor on the face store.
 
+// Stores the lattice solve runs on: one workgroup's (LDS or global) and a
+// split column's (global, column barriers)
 template <class Z>
-struct IsFaceStore {
+struct RunsLattice {
   static constexpr bool value = false;
 };
 template <bool G>
-struct IsFaceStore<FaceStore<G>> {
+struct RunsLattice<FaceStore<G>> {
+  static constexpr bool value = true;
+};
+template <>
+struct RunsLattice<SplitStore> {
   static constexpr bool value = true;
 };
 
@@ -621,7 +630,10 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
 // keeping the running product of its multipliers, one thread per mode chains
 // the B block ends, and each block adds its product times the carried value.
 constexpr int kTriRows = 8;  // rows of a block held in registers at a time
-constexpr int kTriThreads = JJA_LATTICE_TRI_THREADS;
+// blocks per mode: JJA_LATTICE_TRI_THREADS per part of the column, over C
+__device__ __forceinline__ int32_t lattice_blocks(const DevProblem& P) {
+  return max(1, min(P.lat_rows, JJA_LATTICE_TRI_THREADS * P.parts / P.lat_cols));
+}
 
 // Chain the block ends of every mode: dir +1, carry[b] = value entering block
 // b from below (carry[0] = 0, carry[b] = endp[b-1] carry[b-1] + endv[b-1]);
@@ -660,7 +672,7 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
   const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp;
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
   const double g = P.lat_g;
-  const int32_t B = max(1, min(R, kTriThreads / C)), m = (R + B - 1) / B;
+  const int32_t B = lattice_blocks(P), m = (R + B - 1) / B;
   // block ends (value, product) and carries in the free folded-row area
   const int64_t endv = P.lat_t0, endp = endv + static_cast<int64_t>(B) * C, carry = endp + static_cast<int64_t>(B) * C;
   auto zi = [&](int32_t r, int32_t k) { return static_cast<int64_t>(r) * C + k; };
@@ -754,7 +766,7 @@ __device__ __forceinline__ void solve_lattice(const DevProblem& P, const Z& zs) 
 // K^-1 b of the face vector in place: the lattice solve or the factor program.
 template <class Z>
 __device__ __forceinline__ void solve_face_system(const DevProblem& P, const Z& zs) {
-  if constexpr (IsFaceStore<Z>::value) {
+  if constexpr (RunsLattice<Z>::value) {
     if (P.lat_rows) {
       solve_lattice(P, zs);
       return;
@@ -1713,8 +1725,7 @@ int check_plan_shapes(int32_t n, const jja_problem_desc* descs) {
       return fail(JJA_EINVAL, "jja_plan_create: all problems of a plan need the same block_size/parts");
     if (split && d.index_bytes != 4)
       return fail(JJA_EINVAL, "jja_plan_create: with a split column, every problem needs the global face store");
-    if (split && d.lattice_rows != 0)
-      return fail(JJA_EINVAL, "jja_plan_create: a lattice solve cannot share a plan with split columns");
+
     if ((d.column_pairs != 0) != (descs[0].column_pairs != 0))
       return fail(JJA_EINVAL, "jja_plan_create: column_pairs must be the same for every problem of a plan");
   }
@@ -1790,11 +1801,11 @@ int validate(const jja_problem_desc& d) {
   if (d.lattice_rows != 0) {
     const int64_t R = d.lattice_rows, C = d.lattice_cols, H = (C + 1) / 2, ES = H + (H & 1);
     if (R < 2 || C < 2 || R * C != Nf) return fail(JJA_EINVAL, "lattice rows * columns must equal n_faces (>= 2 each)");
-    if (parts != 1) return fail(JJA_EINVAL, "a lattice solve needs one workgroup per column");
+    if (d.column_pairs) return fail(JJA_EINVAL, "a lattice solve does not pair columns");
     if (!d.lattice_sine || !d.lattice_inv) return fail(JJA_EINVAL, "jja_plan_create: missing lattice tables");
     if (d.n_solve_slices != 0 || d.face_copy_rows != 0)
       return fail(JJA_EINVAL, "a lattice solve has no solve records and no face copy rows");
-    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(R, JJA_LATTICE_TRI_THREADS / C));
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(R, int64_t{JJA_LATTICE_TRI_THREADS} * parts / C));
     if (ms < (Nf & 1) + std::max(2 * R * ES, 3 * B * C))
       return fail(JJA_EINVAL, "max_stage_rows too small for the lattice solve");
     if (!(d.lattice_g > 0.0) || !std::isfinite(d.lattice_g)) return fail(JJA_EINVAL, "lattice_g must be finite and > 0");

@@ -180,6 +180,8 @@ class Engine:
             if parts > 1:
                 self.host = self.host.relayout(self.block_size, parts)
             if getattr(problem, "column_pairs", False):
+                if self.host.uses_lattice and getattr(problem, "solver", "auto") == "auto":
+                    self.host = self.host.relayout(self.block_size, solver="cholesky")
                 self.host = self.host.paired()
                 self.block_size = self.host.block_size
             self._desc = self.host.desc(max_columns=W)

@@ -47,7 +47,7 @@ from . import _native
 # sine modes one wave accumulates at once (even, so a mode's parity is that of
 # its slot); the sine table's rows are padded to a multiple of it
 CHUNK = 14
-# each mode's tridiagonal recurrence runs in min(R, TRI_THREADS // C) blocks
+# each mode's tridiagonal recurrence runs in min(R, TRI_THREADS * parts // C) blocks
 TRI_THREADS = 1024
 
 
@@ -128,16 +128,16 @@ class LatticeSolve:
             cprev = (-g_r) * inv[r, :C]
         self.inv = np.ascontiguousarray(inv)
 
-    def scratch_rows(self, Nf):
+    def scratch_rows(self, Nf, parts=1):
         """Doubles after the face vector, at an even offset (16-byte loads): the
         folded rows (E and O, R x ES each), which the tridiagonal solves reuse
         for their block ends and carries (3 B C)."""
-        return (Nf & 1) + max(2 * self.R * self.ES, 3 * self.tri_blocks * self.C)
+        return (Nf & 1) + max(2 * self.R * self.ES, 3 * self.tri_blocks(parts) * self.C)
 
-    @property
-    def tri_blocks(self):
-        """Blocks each mode's tridiagonal recurrence is cut into."""
-        return max(1, min(self.R, TRI_THREADS // self.C))
+    def tri_blocks(self, parts=1):
+        """Blocks each mode's tridiagonal recurrence is cut into, for a column
+        worked by ``parts`` workgroups."""
+        return max(1, min(self.R, TRI_THREADS * parts // self.C))
 
     def table_bytes(self):
         """Bytes of the solve's tables (the tabulated half of the sine matrix
@@ -171,12 +171,12 @@ class LatticeSolve:
         out[:, 1::2] = O @ self.sine[:, :self.C][:, 1::2]
         return out
 
-    def _tridiag(self, Y):
+    def _tridiag(self, Y, parts=1):
         """Every mode's tridiagonal solve in place in Y (R x C), by the device's
         blocked recurrences."""
         R, C, g = self.R, self.C, self.g_r
         inv = self.inv[:, :C]
-        B = self.tri_blocks
+        B = self.tri_blocks(parts)
         m = -(-R // B)
         blocks = [(b * m, min(R, (b + 1) * m)) for b in range(B) if b * m < R]
         ends = []
@@ -212,9 +212,10 @@ class LatticeSolve:
                 pr = pr * (g * inv[r])
                 Y[r] = pr * cin + Y[r]
 
-    def emulate(self, b):
+    def emulate(self, b, parts=1):
         """K^-1 b for b (Nf,) or (Nf, m) in circuit face order, by the kernel's
-        steps (its sums agree with these to rounding)."""
+        steps for a column of ``parts`` workgroups (its sums agree with these
+        to rounding)."""
         b = np.asarray(b, dtype=np.float64)
         vec = b.ndim == 1
         B = b[:, None] if vec else b
@@ -223,7 +224,7 @@ class LatticeSolve:
         for col in range(B.shape[1]):
             Z = B[self.face_of_pos, col].reshape(R, C)
             Y = self._transform(Z)
-            self._tridiag(Y)
+            self._tridiag(Y, parts)
             X = self._transform(Y)
             out[self.face_of_pos, col] = X.ravel()
         return out[:, 0] if vec else out

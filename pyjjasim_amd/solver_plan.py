@@ -437,6 +437,8 @@ class CircuitPlan:
         record programs (nothing is rebuilt)."""
         if self.parts < 2 or self.block_size not in (512, 1024) or 2 * self.parts > 32:
             raise ValueError("column pairs need a split plan (parts 2..16) with workgroups of 512 or 1024 threads")
+        if self.uses_lattice:
+            raise ValueError("column pairs share factor records; the lattice solve has none (solver='cholesky')")
         other = copy.copy(self)
         other.block_size = self.block_size // 2
         other.parts = 2 * self.parts
@@ -585,10 +587,10 @@ class CircuitPlan:
         if Nf == 0:
             self._build_faceless()
             return
-        lattice_ok = self.lattice is not None and self.parts == 1 and self.solver != "cholesky"
+        lattice_ok = self.lattice is not None and self.solver != "cholesky"
         if self.solver == "lattice" and not lattice_ok:
             raise ValueError("the lattice solve needs a rectangular lattice with one junction weight per "
-                             "direction (no mutual inductance) and one workgroup per column")
+                             "direction (no mutual inductance)")
         if lattice_ok:
             self._build_lattice_layout()
             return
@@ -734,12 +736,12 @@ class CircuitPlan:
         """Face records by lattice position (r*C + c) and no solve program: the
         kernel solves K by the lattice solve (lattice_solve.py)."""
         lat, Nf, Nj, BS = self.lattice, self.Nf, self.Nj, self.column_threads
-        scratch = lat.scratch_rows(Nf)
+        scratch = lat.scratch_rows(Nf, self.parts)
         fits = Nf + scratch <= min(LDS_BYTES // 8, 65536)
         if self.face_store == "lds" and not fits:
             raise ValueError(f"circuit with {Nf} faces needs {8 * (Nf + scratch)} bytes of face store per "
                              f"column; more than one workgroup's LDS ({LDS_BYTES})")
-        self.global_store = self.face_store == "global" or not fits
+        self.global_store = self.face_store == "global" or not fits or self.parts > 1
         self.index_bytes = 4 if self.global_store else 2
         self.local_stages = self.part_stages = 0
         self.scratch_rows = scratch
@@ -1134,7 +1136,7 @@ class CircuitPlan:
                 zs[Nf + pos[cp]] = acc[cp]
                 acc = np.zeros(BS)
         if self.uses_lattice:
-            return self.lattice.emulate(zs[:Nf][self.lattice.pos_of_face])
+            return self.lattice.emulate(zs[:Nf][self.lattice.pos_of_face], self.parts)
         return self._emulate_records(zs)
 
     def emulate_solve(self, b):
@@ -1143,7 +1145,7 @@ class CircuitPlan:
         per-wave record programs run as in emulate_face_and_solve, and K^-1 b
         comes back in face order."""
         if self.uses_lattice:
-            return self.lattice.emulate(b)
+            return self.lattice.emulate(b, self.parts)
         Nf = self.Nf
         zs = np.zeros(Nf + max(self.scratch_rows, 1))
         pos = self.solve_plan.face_pos.astype(np.int64)

@@ -9,6 +9,7 @@ import pytest
 
 import pyjjasim_amd as pa
 from oracle import te_oracle as to
+from pyjjasim_amd.solver_plan import CircuitPlan
 
 pytestmark = pytest.mark.gpu
 
@@ -103,6 +104,61 @@ def test_gpu_lattice_adjoint_matches_factor_path():
         with pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W, temperature=0.01,
                                         store_time_steps=store, checkpoint_every=40, seed=6,
                                         solver=solver) as evo:
+            assert evo.eng.host.uses_lattice == (solver == "lattice")
+            leaves = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
+            (evo(*leaves) * G).sum().backward()
+            grads[solver] = [t.grad.cpu().numpy() for t in leaves]
+    for a, b in zip(grads["lattice"], grads["cholesky"]):
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()
+
+
+def test_gpu_split_lattice_matches_oracle_and_unsplit():
+    """A column split over 2 and 4 workgroups runs the lattice solve between
+    column barriers; each mode's recurrence takes more blocks (B grows with the
+    parts), which changes only the rounding: trajectories agree with the
+    unsplit lattice solve to 1e-12 and with the oracle to 1e-9, with exact
+    vortex counts."""
+    c = pa.SquareArray(60, 40)
+    Nt, kw = 300, _inputs(c, W=2)
+    ts = [149, Nt - 1]
+    ref = to.evolve(c, dt=0.05, Nt=Nt, f=0.21, Is=kw["current_sources"], store_time_steps=ts)
+    runs = {}
+    for parts in (1, 2, 4):
+        print(f"[lattice] split SquareArray(60,40), parts {parts}", flush=True)
+        prob = pa.TimeEvolutionProblem(c, time_step_count=Nt, store_time_steps=ts, solver="lattice",
+                                       block_size=512, face_store="global", parts=parts, **kw)
+        assert CircuitPlan(c, 0.05, block_size=512, face_store="global", parts=parts).uses_lattice
+        res = prob.compute()
+        runs[parts] = res
+        th, V = res.get_theta(), res.get_voltage()
+        assert np.abs(th - ref["theta"]).max() <= 1e-9 * np.abs(ref["theta"]).max()
+        assert np.abs(V - ref["voltage"]).max() <= 1e-9 * np.abs(ref["voltage"]).max()
+        np.testing.assert_array_equal(res.get_vortex_configuration(), to.vortex(c.get_cycle_matrix(), ref["theta"]))
+    for parts in (2, 4):
+        a, b = runs[parts].get_theta(), runs[1].get_theta()
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_gpu_split_lattice_adjoint_matches_factor_path():
+    """Gradients through a split column's lattice solve equal the split factor
+    path's to 1e-9."""
+    torch = pytest.importorskip("torch")
+    c = pa.SquareArray(40, 30)
+    Nj, Nf, W, Nt = c.junction_count(), c.face_count(), 2, 80
+    rng = np.random.default_rng(3)
+    Ic = torch.tensor(np.exp(0.1 * rng.standard_normal(Nj)), dtype=torch.float64, device="cuda")
+    f = torch.tensor(0.2 + 0.05 * rng.standard_normal((Nf, W)), dtype=torch.float64, device="cuda")
+    Is = torch.tensor(np.outer(c.current_base(0.0), [0.5, 1.2]), dtype=torch.float64, device="cuda")
+    store = np.zeros(Nt, dtype=bool)
+    store[[Nt // 2, Nt - 1]] = True
+    G = torch.tensor(rng.standard_normal((Nj, W, 2)), dtype=torch.float64, device="cuda")
+    grads = {}
+    for solver in ("lattice", "cholesky"):
+        print(f"[lattice] split adjoint, {solver}", flush=True)
+        with pa.DifferentiableEvolution(c, time_step=0.05, time_step_count=Nt, problem_count=W, temperature=0.01,
+                                        store_time_steps=store, checkpoint_every=40, seed=9, block_size=256,
+                                        face_store="global", parts=4, solver=solver) as evo:
+            assert evo.eng.host.parts == 4
             assert evo.eng.host.uses_lattice == (solver == "lattice")
             leaves = [t.clone().requires_grad_(True) for t in (Ic, f, Is)]
             (evo(*leaves) * G).sum().backward()

@@ -138,10 +138,17 @@ def test_lattice_descriptor_errors_are_refused():
     d = plan.desc()
     d.lattice_g = float("nan")
     assert lib.jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
+    # a split column's tridiagonal solves take more blocks, so more scratch
     split = CircuitPlan(c, 0.05, block_size=128, parts=2)
-    assert not split.uses_lattice
-    both = (_native.ProblemDesc * 2)(plan.relayout(128, face_store="global").desc(), split.desc())
-    assert lib.jja_plan_validate(2, both) == _native.JJA_EINVAL
+    assert split.uses_lattice and split.global_store
+    assert lib.jja_plan_validate(1, C.byref(split.desc())) == _native.JJA_OK
+    wide = pa.SquareArray(200, 20)
+    one, four = (CircuitPlan(wide, 0.05, block_size=1024, parts=p) for p in (1, 4))
+    assert four.lattice.tri_blocks(4) > one.lattice.tri_blocks(1)
+    d = four.desc()
+    d.max_stage_rows = one.relayout(1024, face_store="global").scratch_rows
+    assert d.max_stage_rows < four.scratch_rows
+    assert lib.jja_plan_validate(1, C.byref(d)) == _native.JJA_EINVAL
 
 
 def test_solver_choice():
@@ -150,11 +157,29 @@ def test_solver_choice():
     assert not CircuitPlan(c, 0.05, solver="cholesky").uses_lattice
     p = CircuitPlan(c, 0.05, solver="cholesky").relayout(1024, solver="auto")
     assert p.uses_lattice
-    assert not p.relayout(1024, parts=2, face_store="global").uses_lattice
+    sp = p.relayout(1024, parts=2, face_store="global")
+    assert sp.uses_lattice and sp.parts == 2
+    # column pairs share factor records, which a lattice plan does not have
     with pytest.raises(ValueError):
-        CircuitPlan(c, 0.05, solver="lattice", parts=2)
+        sp.paired()
+    assert not sp.relayout(1024, solver="cholesky").paired().uses_lattice
+    with pytest.raises(ValueError):
+        CircuitPlan(pa.HoneycombArray(4, 4), 0.05, solver="lattice")
     with pytest.raises(ValueError):
         CircuitPlan(c, 0.05, solver="fft")
+
+
+@pytest.mark.parametrize("parts", [2, 4])
+def test_split_lattice_emulation_matches_the_factor(parts):
+    """A split column cuts each mode's recurrence into more blocks: the solve
+    still equals the factor's to rounding."""
+    c = pa.SquareArray(200, 20)
+    plan = CircuitPlan(c, 0.05, block_size=256, parts=parts)
+    assert plan.uses_lattice and plan.lattice.tri_blocks(parts) > plan.lattice.tri_blocks(1)
+    ref = CircuitPlan(c, 0.05, block_size=256, solver="cholesky")
+    u = np.random.default_rng(parts).standard_normal(c.junction_count())
+    want = ref.emulate_face_and_solve(u)
+    np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
 def test_split_balancing_cost_does_not_depend_on_the_solver():

@@ -402,7 +402,7 @@ def test_column_pair_plans_validate():
 
 def _part_local_plan(cls=CircuitPlan):
     c = pa.SquareArray(64, 64)
-    plan = cls(c, 0.05, block_size=512, parts=2)
+    plan = cls(c, 0.05, block_size=512, parts=2, solver="cholesky")
     assert plan.part_stages > plan.local_stages > 0
     return c, plan
 
