@@ -359,7 +359,7 @@ def uneven_bench(pa, torch, steps, warmup, plain_steps, n_problems=64):
             "host_setup_s": setup_s, "plain": plain}
 
 
-def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000):
+def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000, solver="auto"):
     """Large workload: SquareArray(n, n), W bias points linspace(0.2, 1.2) *
     current_base(0), beta_C = 1, junction self-inductance beta_L = 0.5, per-face
     flux f = 0.1 + 0.02 z (seeded), T = 0.05, dt = 0.05; the forward pass and the
@@ -383,7 +383,7 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000):
     store[-1] = True
     evo = pa.DifferentiableEvolution(c, time_step=dt, time_step_count=steps, problem_count=W,
                                      temperature=0.05, store_time_steps=store,
-                                     checkpoint_every=min(checkpoint, steps), seed=7)
+                                     checkpoint_every=min(checkpoint, steps), seed=7, solver=solver)
     setup_s = time.perf_counter() - t0
     for rep in range(2):                          # the first pass warms up
         f = torch.tensor(f0, dtype=torch.float64, device=dev, requires_grad=True)
@@ -400,12 +400,15 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000):
     fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
     face_store = "global" if evo.eng.host.global_store else "lds"
     n_exec = int(evo.eng.host.n_solve_exec)
+    used = "lattice" if evo.eng.host.uses_lattice else "cholesky"
+    parts = int(evo.eng.host.parts)
     evo.close()
     return {"metric": "junction-steps/s (large array forward, FP64)", "value": Nj * W / fwd,
             "unit": "junction-steps/s", "steps": steps, "dtype": "f64",
             "config": {"workload": f"large_square{n}_W{W}", "junctions": Nj, "faces": Nf, "W": W,
                        "beta_L": 0.5, "T": 0.05, "checkpoint": min(checkpoint, steps),
-                       "face_store": face_store, "solve_records_per_column_step": n_exec},
+                       "face_store": face_store, "solver": used, "parts": parts,
+                       "solve_records_per_column_step": n_exec},
             "forward_ms_per_step": fwd * 1e3, "backward_ms_per_step": bwd * 1e3,
             "grad_over_fwd": bwd / fwd,
             "full_horizon_estimate_s": {"steps": full_steps, "forward": fwd * full_steps,
@@ -551,7 +554,7 @@ def main():
         copy_calibration(torch, args.copy_calibration)
         return
     if args.workload == "large":
-        out = large_bench(pa, torch, args.steps, args.checkpoint)
+        out = large_bench(pa, torch, args.steps, args.checkpoint, solver=args.solver)
         out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                     "data": "synthetic (seeded flux disorder, seeded in-kernel thermal noise)"})
         print(json.dumps(out), flush=True)
