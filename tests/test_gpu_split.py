@@ -174,7 +174,7 @@ def test_gpu_part_local_stages_bit_identical():
     from pyjjasim_amd.solver_plan import CircuitPlan
     c = pa.SquareArray(64, 64)
     c.set_capacitance(0.9)
-    assert CircuitPlan(c, 0.05, block_size=512, parts=2).part_stages > 0
+    assert CircuitPlan(c, 0.05, block_size=512, parts=2, solver="cholesky").part_stages > 0
     W, Nt = 2, 60
     Is = c.current_base(0.0)[:, None, None] * np.array([0.6, 1.2])[None, :, None]
 
