@@ -204,7 +204,7 @@ typedef struct jja_problem_desc {
    * face_copy_rows 0). The reference factorises K with SuperLU for every
    * circuit (time_evolution.py:505-506); this is an exact direct solve of the
    * same K. Needs column_pairs 0, R >= 2, C >= 2, R*C = n_faces and
-   * max_stage_rows >= (n_faces & 1) + max(2*R*ES, 3*B*C) with H = (C+1)/2,
+   * max_stage_rows >= (n_faces & 1) + max(2*R*ES, 4*B*C) with H = (C+1)/2,
    * ES = H + (H & 1) and B = max(1, min(R, JJA_LATTICE_TRI_THREADS * parts / C))
    * blocks per tridiagonal solve. */
   int32_t lattice_rows;        /* R (0: no lattice solve) */

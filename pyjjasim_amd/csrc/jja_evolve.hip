@@ -558,13 +558,24 @@ struct RunsLattice<SplitStore> {
 
 // E[r][i] = v[r][i] + v[r][C-1-i], O[r][i] = v[r][i] - v[r][C-1-i] for i < H
 // (the middle of an odd row pairs with itself: E = v, O = 0) and 0 for
-// H <= i < ES, at lat_t0 + r*ES + i and lat_t0 + (R + r)*ES + i.
+// H <= i < ES. An LDS store keeps them by row (E at lat_t0 + r*ES + i, O at
+// lat_t0 + (R + r)*ES + i: a lane reads 16 bytes of its own row per load); a
+// global store keeps them by column (E at lat_t0 + i*R + r, O at
+// lat_t0 + (ES + i)*R + r: a wave's 64 rows are one coalesced load per i).
+// Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*ES + e.
 template <class Z>
 __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, H = (C + 1) >> 1;
   const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * ES;
   for (int32_t e = zs.vt; e < R * ES; e += zs.vbs) {
-    const int32_t r = e / ES, i = e - r * ES;
+    int32_t r, i;
+    if constexpr (Z::kGlobal) {
+      i = e / R;
+      r = e - i * R;
+    } else {
+      r = e / ES;
+      i = e - r * ES;
+    }
     double a = 0.0, b = 0.0;
     if (i < H) {
       const int64_t row = static_cast<int64_t>(r) * C;
@@ -584,7 +595,10 @@ __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
 // out[r][j] = sum_{i<ES} (j even ? E : O)[r][i] S[i][j] into the face vector.
 // A wave takes 64 rows (one per lane) and JJA_LATTICE_CHUNK consecutive modes
 // from an even j0, so a mode's parity is its slot's; the table's values are
-// wave-uniform scalar loads.
+// wave-uniform scalar loads. Every output adds its terms in ascending i, so
+// both layouts give the same bits. The global layout's loads are issued
+// kFoldBatch values of i at a time (they come from L2, not LDS).
+constexpr int kFoldBatch = 8;
 template <class Z>
 __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& zs) {
   constexpr int MC = JJA_LATTICE_CHUNK;
@@ -597,19 +611,43 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
   for (int32_t task = wave; task < tasks; task += NW) {
     const int32_t rb = task % nrb, j0 = (task / nrb) * MC;
     const int32_t r = rb * kWave + lane;
-    const int64_t e = P.lat_t0 + static_cast<int64_t>(r < R ? r : R - 1) * ES;
-    const int64_t o = e + static_cast<int64_t>(R) * ES;
+    const int32_t rr = r < R ? r : R - 1;
     double acc[MC];
 #pragma unroll
     for (int q = 0; q < MC; ++q) acc[q] = 0.0;
-    for (int32_t i = 0; i < ES; i += 2) {
-      const vdouble2 ev = zs.rd2(e + i), ov = zs.rd2(o + i);
-      const CAS double* s0 = S + static_cast<int64_t>(i) * CP + j0;
-      const CAS double* s1 = s0 + CP;
+    if constexpr (Z::kGlobal) {
+      constexpr int KB = kFoldBatch;
+      const GAS double* __restrict__ ep = zs.p + P.lat_t0 + rr;
+      const GAS double* __restrict__ op = ep + static_cast<int64_t>(ES) * R;
+      auto term = [&](double e, double o, int32_t i) {
+        const CAS double* s = S + static_cast<int64_t>(i) * CP + j0;
 #pragma unroll
-      for (int q = 0; q < MC; ++q) {
-        acc[q] = __builtin_fma((q & 1) ? ov.x : ev.x, s0[q], acc[q]);
-        acc[q] = __builtin_fma((q & 1) ? ov.y : ev.y, s1[q], acc[q]);
+        for (int q = 0; q < MC; ++q) acc[q] = __builtin_fma((q & 1) ? o : e, s[q], acc[q]);
+      };
+      int32_t i = 0;
+      for (; i + KB <= ES; i += KB) {
+        double ev[KB], ov[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+          ev[q] = ep[static_cast<int64_t>(i + q) * R];
+          ov[q] = op[static_cast<int64_t>(i + q) * R];
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q) term(ev[q], ov[q], i + q);
+      }
+      for (; i < ES; ++i) term(ep[static_cast<int64_t>(i) * R], op[static_cast<int64_t>(i) * R], i);
+    } else {
+      const int64_t e = P.lat_t0 + static_cast<int64_t>(rr) * ES;
+      const int64_t o = e + static_cast<int64_t>(R) * ES;
+      for (int32_t i = 0; i < ES; i += 2) {
+        const vdouble2 ev = zs.rd2(e + i), ov = zs.rd2(o + i);
+        const CAS double* s0 = S + static_cast<int64_t>(i) * CP + j0;
+        const CAS double* s1 = s0 + CP;
+#pragma unroll
+        for (int q = 0; q < MC; ++q) {
+          acc[q] = __builtin_fma((q & 1) ? ov.x : ev.x, s0[q], acc[q]);
+          acc[q] = __builtin_fma((q & 1) ? ov.y : ev.y, s1[q], acc[q]);
+        }
       }
     }
     if (r < R) {
@@ -625,45 +663,42 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
 // sweeps are first-order linear recurrences with the multiplier
 // a_r = g inv_r: forward d_r = (y_r + g d_{r-1}) inv_r = y_r inv_r + a_r d_{r-1},
 // backward x_r = d_r - ((-g) inv_r) x_{r+1} = d_r + a_r x_{r+1}. The rows of a
-// mode are cut into B blocks (B from R and C only, so results do not depend
-// on the block size): each block runs its recurrence from a zero start while
-// keeping the running product of its multipliers, one thread per mode chains
-// the B block ends, and each block adds its product times the carried value.
-constexpr int kTriRows = 8;  // rows of a block held in registers at a time
+// mode are cut into B blocks (B from R, C and the part count only, so results
+// do not depend on the block size): each block runs its recurrence from a
+// zero start while keeping the running product of its multipliers, and then
+// adds its product times the value carried in from the blocks before it. A
+// block's thread chains those blocks' (value, product) ends itself, in block
+// order, so each carry is the same sum whoever forms it, and no phase or
+// barrier is spent on a separate chain. Three phases: forward from zero;
+// forward carries, then backward from zero on the same rows; backward carries.
+constexpr int kTriRows = 8;  // rows (or block ends) held in registers at a time
 // blocks per mode: JJA_LATTICE_TRI_THREADS per part of the column, over C
 __device__ __forceinline__ int32_t lattice_blocks(const DevProblem& P) {
   return max(1, min(P.lat_rows, JJA_LATTICE_TRI_THREADS * P.parts / P.lat_cols));
 }
 
-// Chain the block ends of every mode: dir +1, carry[b] = value entering block
-// b from below (carry[0] = 0, carry[b] = endp[b-1] carry[b-1] + endv[b-1]);
-// dir -1, carry[b] = value entering from above (carry[B-1] = 0, from block
-// b+1). The ends are read kTriRows at a time.
+// The value carried into block b of mode k: dir +1 chains the ends of blocks
+// 0 .. b-1 upward (cin = endp cin + endv from 0), dir -1 the ends of blocks
+// B-1 .. b+1 downward.
 template <class Z>
-__device__ __forceinline__ void lattice_chain(const Z& zs, int32_t C, int32_t B, int64_t endv, int64_t endp,
-                                              int64_t carry, int dir) {
+__device__ __forceinline__ double lattice_carry(const Z& zs, int32_t C, int32_t B, int64_t endv, int64_t endp,
+                                                int32_t b, int32_t k, int dir) {
   constexpr int TB = kTriRows;
-  const int32_t b_first = dir > 0 ? 0 : B - 1;
-  for (int32_t k = zs.vt; k < C; k += zs.vbs) {
-    double cin = 0.0;
-    zs.wr(carry + static_cast<int64_t>(b_first) * C + k, 0.0);
-    for (int32_t i0 = 1; i0 < B; i0 += TB) {
-      double ev[TB], ep[TB];
+  const int32_t n = dir > 0 ? b : B - 1 - b, first = dir > 0 ? 0 : B - 1;
+  double cin = 0.0;
+  for (int32_t i0 = 0; i0 < n; i0 += TB) {
+    double ev[TB], ep[TB];
 #pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        const int32_t src = b_first + dir * (min(i0 + q, B - 1) - 1);
-        ev[q] = zs.rd(endv + static_cast<int64_t>(src) * C + k);
-        ep[q] = zs.rd(endp + static_cast<int64_t>(src) * C + k);
-      }
-#pragma unroll
-      for (int q = 0; q < TB; ++q) {
-        if (i0 + q < B) {
-          cin = __builtin_fma(ep[q], cin, ev[q]);
-          zs.wr(carry + static_cast<int64_t>(b_first + dir * (i0 + q)) * C + k, cin);
-        }
-      }
+    for (int q = 0; q < TB; ++q) {
+      const int64_t at = static_cast<int64_t>(first + dir * min(i0 + q, n - 1)) * C + k;
+      ev[q] = zs.rd(endv + at);
+      ep[q] = zs.rd(endp + at);
     }
+#pragma unroll
+    for (int q = 0; q < TB; ++q)
+      if (i0 + q < n) cin = __builtin_fma(ep[q], cin, ev[q]);
   }
+  return cin;
 }
 
 template <class Z>
@@ -673,11 +708,14 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
   const double g = P.lat_g;
   const int32_t B = lattice_blocks(P), m = (R + B - 1) / B;
-  // block ends (value, product) and carries in the free folded-row area
-  const int64_t endv = P.lat_t0, endp = endv + static_cast<int64_t>(B) * C, carry = endp + static_cast<int64_t>(B) * C;
+  // forward and backward block ends (value, product) in the free folded-row
+  // area: the second phase reads forward ends while other blocks write their
+  // backward ends
+  const int64_t fv = P.lat_t0, fp = fv + static_cast<int64_t>(B) * C;
+  const int64_t bv = fp + static_cast<int64_t>(B) * C, bp = bv + static_cast<int64_t>(B) * C;
   auto zi = [&](int32_t r, int32_t k) { return static_cast<int64_t>(r) * C + k; };
-  // one pass over rows [r0, r1) of mode k in direction dir (+1 up, -1 down):
-  // f(r, y, a) for each row, rows loaded TB at a time
+  // one pass over `count` rows of mode k from `first` in direction dir (+1
+  // up, -1 down): f(r, y, a) for each row, rows loaded TB at a time
   auto sweep = [&](int32_t k, int32_t first, int32_t count, int32_t dir, auto&& f) {
     for (int32_t c0 = 0; c0 < count; c0 += TB) {
       double y[TB], a[TB];
@@ -692,8 +730,8 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
         if (c0 + q < count) f(first + dir * (c0 + q), y[q], a[q]);
     }
   };
-  // forward: local recurrence of every block from zero (an empty block at the
-  // end passes its carry through: value 0, product 1)
+  // forward from zero (an empty block at the end passes its carry through:
+  // value 0, product 1)
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
     double d = 0.0, pr = 1.0;
@@ -703,44 +741,40 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
         pr = pr * (g * iv);
         zs.wr(zi(r, k), d);
       });
-    zs.wr(endv + e, d);
-    zs.wr(endp + e, pr);
+    zs.wr(fv + e, d);
+    zs.wr(fp + e, pr);
   }
   zs.barrier();
-  lattice_chain(zs, C, B, endv, endp, carry, 1);
-  zs.barrier();
-  for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
-    const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
-    if (n <= 0 || b == 0) continue;
-    const double cin = zs.rd(carry + e);
-    double pr = 1.0;
-    sweep(k, r0, n, 1, [&](int32_t r, double y, double iv) {
-      pr = pr * (g * iv);
-      zs.wr(zi(r, k), __builtin_fma(pr, cin, y));
-    });
-  }
-  zs.barrier();
-  // backward: local recurrence of every block from zero, top row down
+  // forward carries into the rows, then backward from zero, top row down (the
+  // same thread wrote the rows it reads back)
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
     double x = 0.0, pr = 1.0;
-    if (n > 0)
+    if (n > 0) {
+      if (b > 0) {
+        const double cin = lattice_carry(zs, C, B, fv, fp, b, k, 1);
+        double pf = 1.0;
+        sweep(k, r0, n, 1, [&](int32_t r, double y, double iv) {
+          pf = pf * (g * iv);
+          zs.wr(zi(r, k), __builtin_fma(pf, cin, y));
+        });
+      }
       sweep(k, r0 + n - 1, n, -1, [&](int32_t r, double dv, double iv) {
         const double a = g * iv;
         x = __builtin_fma(a, x, dv);
         pr = pr * a;
         zs.wr(zi(r, k), x);
       });
-    zs.wr(endv + e, x);
-    zs.wr(endp + e, pr);
+    }
+    zs.wr(bv + e, x);
+    zs.wr(bp + e, pr);
   }
   zs.barrier();
-  lattice_chain(zs, C, B, endv, endp, carry, -1);
-  zs.barrier();
+  // backward carries into the rows
   for (int32_t e = zs.vt; e < B * C; e += zs.vbs) {
     const int32_t b = e / C, k = e - b * C, r0 = b * m, n = min(m, R - r0);
     if (n <= 0 || b == B - 1) continue;
-    const double cin = zs.rd(carry + e);
+    const double cin = lattice_carry(zs, C, B, bv, bp, b, k, -1);
     double pr = 1.0;
     sweep(k, r0 + n - 1, n, -1, [&](int32_t r, double x, double iv) {
       pr = pr * (g * iv);
@@ -1806,7 +1840,7 @@ int validate(const jja_problem_desc& d) {
     if (d.n_solve_slices != 0 || d.face_copy_rows != 0)
       return fail(JJA_EINVAL, "a lattice solve has no solve records and no face copy rows");
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(R, int64_t{JJA_LATTICE_TRI_THREADS} * parts / C));
-    if (ms < (Nf & 1) + std::max(2 * R * ES, 3 * B * C))
+    if (ms < (Nf & 1) + std::max(2 * R * ES, 4 * B * C))
       return fail(JJA_EINVAL, "max_stage_rows too small for the lattice solve");
     if (!(d.lattice_g > 0.0) || !std::isfinite(d.lattice_g)) return fail(JJA_EINVAL, "lattice_g must be finite and > 0");
     return JJA_OK;

@@ -131,8 +131,8 @@ class LatticeSolve:
     def scratch_rows(self, Nf, parts=1):
         """Doubles after the face vector, at an even offset (16-byte loads): the
         folded rows (E and O, R x ES each), which the tridiagonal solves reuse
-        for their block ends and carries (3 B C)."""
-        return (Nf & 1) + max(2 * self.R * self.ES, 3 * self.tri_blocks(parts) * self.C)
+        for their forward and backward block ends (values and products, 4 B C)."""
+        return (Nf & 1) + max(2 * self.R * self.ES, 4 * self.tri_blocks(parts) * self.C)
 
     def tri_blocks(self, parts=1):
         """Blocks each mode's tridiagonal recurrence is cut into, for a column
