@@ -85,6 +85,7 @@ __device__ __forceinline__ GAS T* gblw(T* p) {
 }
 
 typedef double vdouble2 __attribute__((ext_vector_type(2)));
+typedef double vdouble4 __attribute__((ext_vector_type(4)));
 typedef uint32_t vuint2 __attribute__((ext_vector_type(2)));
 typedef uint32_t vuint4 __attribute__((ext_vector_type(4)));
 typedef int32_t vint2 __attribute__((ext_vector_type(2)));
@@ -561,7 +562,7 @@ struct RunsLattice<SplitStore> {
 // H <= i < ES. An LDS store keeps them by row (E at lat_t0 + r*ES + i, O at
 // lat_t0 + (R + r)*ES + i: a lane reads 16 bytes of its own row per load); a
 // global store keeps them by column (E at lat_t0 + i*R + r, O at
-// lat_t0 + (ES + i)*R + r: a wave's 64 rows are one coalesced load per i).
+// lat_t0 + (ES + i)*R + r: the matrix-core operands are runs of 16 rows).
 // Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*ES + e.
 template <class Z>
 __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
@@ -592,15 +593,13 @@ __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
   }
 }
 
-// out[r][j] = sum_{i<ES} (j even ? E : O)[r][i] S[i][j] into the face vector.
-// A wave takes 64 rows (one per lane) and JJA_LATTICE_CHUNK consecutive modes
-// from an even j0, so a mode's parity is its slot's; the table's values are
-// wave-uniform scalar loads. Every output adds its terms in ascending i, so
-// both layouts give the same bits. The global layout's loads are issued
-// kFoldBatch values of i at a time (they come from L2, not LDS).
-constexpr int kFoldBatch = 8;
+// out[r][j] = sum_{i<ES} (j even ? E : O)[r][i] S[i][j] into the face vector
+// of an LDS store. A wave takes 64 rows (one per lane) and JJA_LATTICE_CHUNK
+// consecutive modes from an even j0, so a mode's parity is its slot's; the
+// table's values are wave-uniform scalar loads. Each output adds its terms in
+// ascending i with fused multiply-adds.
 template <class Z>
-__device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& zs) {
+__device__ __forceinline__ void lattice_transform_lds(const DevProblem& P, const Z& zs) {
   constexpr int MC = JJA_LATTICE_CHUNK;
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, CP = P.lat_cp;
   const int lane = zs.vt & (kWave - 1);
@@ -611,43 +610,19 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
   for (int32_t task = wave; task < tasks; task += NW) {
     const int32_t rb = task % nrb, j0 = (task / nrb) * MC;
     const int32_t r = rb * kWave + lane;
-    const int32_t rr = r < R ? r : R - 1;
+    const int64_t e = P.lat_t0 + static_cast<int64_t>(r < R ? r : R - 1) * ES;
+    const int64_t o = e + static_cast<int64_t>(R) * ES;
     double acc[MC];
 #pragma unroll
     for (int q = 0; q < MC; ++q) acc[q] = 0.0;
-    if constexpr (Z::kGlobal) {
-      constexpr int KB = kFoldBatch;
-      const GAS double* __restrict__ ep = zs.p + P.lat_t0 + rr;
-      const GAS double* __restrict__ op = ep + static_cast<int64_t>(ES) * R;
-      auto term = [&](double e, double o, int32_t i) {
-        const CAS double* s = S + static_cast<int64_t>(i) * CP + j0;
+    for (int32_t i = 0; i < ES; i += 2) {
+      const vdouble2 ev = zs.rd2(e + i), ov = zs.rd2(o + i);
+      const CAS double* s0 = S + static_cast<int64_t>(i) * CP + j0;
+      const CAS double* s1 = s0 + CP;
 #pragma unroll
-        for (int q = 0; q < MC; ++q) acc[q] = __builtin_fma((q & 1) ? o : e, s[q], acc[q]);
-      };
-      int32_t i = 0;
-      for (; i + KB <= ES; i += KB) {
-        double ev[KB], ov[KB];
-#pragma unroll
-        for (int q = 0; q < KB; ++q) {
-          ev[q] = ep[static_cast<int64_t>(i + q) * R];
-          ov[q] = op[static_cast<int64_t>(i + q) * R];
-        }
-#pragma unroll
-        for (int q = 0; q < KB; ++q) term(ev[q], ov[q], i + q);
-      }
-      for (; i < ES; ++i) term(ep[static_cast<int64_t>(i) * R], op[static_cast<int64_t>(i) * R], i);
-    } else {
-      const int64_t e = P.lat_t0 + static_cast<int64_t>(rr) * ES;
-      const int64_t o = e + static_cast<int64_t>(R) * ES;
-      for (int32_t i = 0; i < ES; i += 2) {
-        const vdouble2 ev = zs.rd2(e + i), ov = zs.rd2(o + i);
-        const CAS double* s0 = S + static_cast<int64_t>(i) * CP + j0;
-        const CAS double* s1 = s0 + CP;
-#pragma unroll
-        for (int q = 0; q < MC; ++q) {
-          acc[q] = __builtin_fma((q & 1) ? ov.x : ev.x, s0[q], acc[q]);
-          acc[q] = __builtin_fma((q & 1) ? ov.y : ev.y, s1[q], acc[q]);
-        }
+      for (int q = 0; q < MC; ++q) {
+        acc[q] = __builtin_fma((q & 1) ? ov.x : ev.x, s0[q], acc[q]);
+        acc[q] = __builtin_fma((q & 1) ? ov.y : ev.y, s1[q], acc[q]);
       }
     }
     if (r < R) {
@@ -657,6 +632,95 @@ __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& 
         if (j0 + q < C) zs.wr(row + j0 + q, acc[q]);
     }
   }
+}
+
+// The same sums for a global store, on the matrix cores: for each parity p,
+// out[r][2m+p] = sum_i F_p[r][i] S[i][2m+p] (F_0 = E, F_1 = O) is a product
+// of an R x ES and an ES x (modes of parity p) matrix. A wave takes a tile of
+// 32 rows x 32 modes of one parity as 2 x 2 blocks of 16 x 16 and runs
+// v_mfma_f64_16x16x4f64 over i in steps of 4; lane l holds A[i][k] =
+// F[r0 + l%16][i0 + l/16] and B[k][j] = S[i0 + l/16][2(m0 + l%16) + p], and
+// result register q of lane l is D[l/16 + 4q][l%16]. E and O are kept by
+// column (E[i][r] at lat_t0 + i*R + r), so each operand load is four
+// 128-byte rows. kMfmaBatch steps of loads are issued before their products.
+// Rows and modes past the lattice load clamped (finite) operands and are not
+// stored; terms i >= ES have A = 0. The matrix cores round differently from a
+// chain of fused multiply-adds: the two stores agree to rounding.
+constexpr int kMfmaBatch = 4;
+template <class Z>
+__device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, const Z& zs) {
+  constexpr int U = kMfmaBatch;
+  const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, CP = P.lat_cp;
+  const int lane = zs.vt & (kWave - 1);
+  const int32_t wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
+  const int32_t NW = zs.vbs / kWave;
+  const int32_t n_even = (C + 1) >> 1, n_odd = C >> 1;
+  const int32_t nrt = (R + 31) >> 5, nte = (n_even + 31) >> 5, nto = (n_odd + 31) >> 5;
+  const int32_t tasks = nrt * (nte + nto);
+  const int lr = lane & 15, lk = lane >> 4;
+  const GAS double* __restrict__ S = gbl(P.lat_sine);
+  for (int32_t task = wave; task < tasks; task += NW) {
+    const int32_t rt = task % nrt, mt = task / nrt;
+    const int par = mt < nte ? 0 : 1;
+    const int32_t m0 = 32 * (par ? mt - nte : mt);
+    const int32_t r0 = 32 * rt;
+    // operand addresses of this lane (clamped into the lattice)
+    const GAS double* fa[2];
+    const GAS double* sb[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int32_t r = min(r0 + 16 * b + lr, R - 1);
+      fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * ES * R + r;
+      const int32_t j = min(2 * (m0 + 16 * b + lr) + par, C - 1);
+      sb[b] = S + j;
+    }
+    vdouble4 acc[2][2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[b][c] = vdouble4{0.0, 0.0, 0.0, 0.0};
+    for (int32_t i0 = 0; i0 < ES; i0 += 4 * U) {
+      double av[U][2], bv[U][2];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = i0 + 4 * u + lk;
+        const int32_t ic = min(i, ES - 1);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          av[u][b] = fa[b][static_cast<int64_t>(ic) * R];
+          bv[u][b] = sb[b][static_cast<int64_t>(ic) * CP];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) av[u][b] = i < ES ? av[u][b] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            acc[b][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][b], bv[u][c], acc[b][c], 0, 0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int32_t j = 2 * (m0 + 16 * c + lr) + par;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int32_t r = r0 + 16 * b + lk + 4 * q;
+          if (r < R && j < C) zs.wr(static_cast<int64_t>(r) * C + j, acc[b][c][q]);
+        }
+      }
+  }
+}
+
+template <class Z>
+__device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& zs) {
+  if constexpr (Z::kGlobal)
+    lattice_transform_mfma(P, zs);
+  else
+    lattice_transform_lds(P, zs);
 }
 
 // Mode k's tridiagonal solve along r, in place in the face vector. Both

@@ -110,7 +110,8 @@ def test_gpu_rejects_custom_cpr():
 @pytest.mark.parametrize("T", [0.0, 0.02])
 def test_gpu_global_face_store_bit_identical_to_lds(T):
     """The global-memory face store runs the same record program and arithmetic as
-    the LDS one: identical bits."""
+    the LDS one: identical bits (the factor program; the lattice solve's two
+    stores agree to rounding, tested with the lattice solve)."""
     print(f"[gpu parity] global store T={T}", flush=True)
     c = pa.SquareArray(12, 9)
     rng = np.random.default_rng(8)
@@ -121,7 +122,7 @@ def test_gpu_global_face_store_bit_identical_to_lds(T):
     for store in ("lds", "global"):
         p = pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=400, external_flux=0.2,
                                     current_sources=Is, temperature=T, seed=5, block_size=128,
-                                    face_store=store)
+                                    face_store=store, solver="cholesky")
         r = p.compute()
         out[store] = (r.get_theta(), r.get_voltage(), r.get_vortex_configuration())
     for a, b in zip(out["lds"], out["global"]):
