@@ -557,6 +557,9 @@ struct RunsLattice<SplitStore> {
   static constexpr bool value = true;
 };
 
+// elements of the fold each thread loads before it computes
+constexpr int kFoldBatch = 4;
+
 // E[r][i] = v[r][i] + v[r][C-1-i], O[r][i] = v[r][i] - v[r][C-1-i] for i < H
 // (the middle of an odd row pairs with itself: E = v, O = 0) and 0 for
 // H <= i < ES. An LDS store keeps them by row (E at lat_t0 + r*ES + i, O at
@@ -566,30 +569,49 @@ struct RunsLattice<SplitStore> {
 // Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*ES + e.
 template <class Z>
 __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
+  constexpr int KB = kFoldBatch;
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, H = (C + 1) >> 1;
+  const int32_t n = R * ES;
   const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * ES;
-  for (int32_t e = zs.vt; e < R * ES; e += zs.vbs) {
-    int32_t r, i;
-    if constexpr (Z::kGlobal) {
-      i = e / R;
-      r = e - i * R;
-    } else {
-      r = e / ES;
-      i = e - r * ES;
-    }
-    double a = 0.0, b = 0.0;
-    if (i < H) {
-      const int64_t row = static_cast<int64_t>(r) * C;
-      const double x = zs.rd(row + i), y = zs.rd(row + (C - 1 - i));
-      if (2 * i + 1 == C) {
-        a = x;
+  // KB elements per thread at a time: all their loads are issued first
+  for (int32_t e = zs.vt; e < n; e += KB * zs.vbs) {
+    double x[KB], y[KB];
+    int32_t ii[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int32_t ek = min(e + k * zs.vbs, n - 1);
+      int32_t r, i;
+      if constexpr (Z::kGlobal) {
+        i = ek / R;
+        r = ek - i * R;
       } else {
-        a = x + y;
-        b = x - y;
+        r = ek / ES;
+        i = ek - r * ES;
       }
+      ii[k] = i;
+      // a padding slot (i >= H) reads the row's first element and drops it
+      const int64_t row = static_cast<int64_t>(r) * C;
+      const int32_t ic = i < H ? i : 0;
+      x[k] = zs.rd(row + ic);
+      y[k] = zs.rd(row + (C - 1 - ic));
     }
-    zs.wr(e0 + e, a);
-    zs.wr(o0 + e, b);
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int32_t ek = e + k * zs.vbs;
+      if (ek >= n) break;
+      const int32_t i = ii[k];
+      double a = 0.0, b = 0.0;
+      if (i < H) {
+        if (2 * i + 1 == C) {
+          a = x[k];
+        } else {
+          a = x[k] + y[k];
+          b = x[k] - y[k];
+        }
+      }
+      zs.wr(e0 + ek, a);
+      zs.wr(o0 + ek, b);
+    }
   }
 }
 
