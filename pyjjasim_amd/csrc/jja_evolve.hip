@@ -132,6 +132,7 @@ struct DevProblem {
   // lattice solve (lat_rows 0: the staged factor program solves K)
   int32_t lat_rows, lat_cols;  // R, C: faces at positions r*C + c
   int32_t lat_es, lat_cp;      // folded row length ES (even) and sine-table row stride CP
+  int32_t lat_ef, lat_pad_;    // folded rows stored per row: ES (LDS store), ES rounded up to 4 (global)
   int64_t lat_t0;              // store offset of the folded rows (even: 16-byte loads)
   const double* lat_sine;      // [ES][CP]
   const double* lat_inv;       // [R][CP] inverse Thomas pivots
@@ -562,17 +563,19 @@ constexpr int kFoldBatch = 4;
 
 // E[r][i] = v[r][i] + v[r][C-1-i], O[r][i] = v[r][i] - v[r][C-1-i] for i < H
 // (the middle of an odd row pairs with itself: E = v, O = 0) and 0 for
-// H <= i < ES. An LDS store keeps them by row (E at lat_t0 + r*ES + i, O at
-// lat_t0 + (R + r)*ES + i: a lane reads 16 bytes of its own row per load); a
-// global store keeps them by column (E at lat_t0 + i*R + r, O at
-// lat_t0 + (ES + i)*R + r: the matrix-core operands are runs of 16 rows).
-// Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*ES + e.
+// H <= i < EF (EF = lat_ef). An LDS store keeps them by row (EF = ES; E at
+// lat_t0 + r*ES + i, O at lat_t0 + (R + r)*ES + i: a lane reads 16 bytes of its
+// own row per load). A global store keeps them in blocks of four i by row
+// (EF = ES rounded up to 4; E at lat_t0 + ((i/4)*R + r)*4 + i%4, O at
+// lat_t0 + R*EF + the same): a matrix-core operand, 16 rows x 4 i, is 512
+// contiguous bytes, and a wave of the fold reads 32-byte runs of 16 rows.
+// Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*EF + e.
 template <class Z>
 __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
   constexpr int KB = kFoldBatch;
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, H = (C + 1) >> 1;
-  const int32_t n = R * ES;
-  const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * ES;
+  const int32_t n = R * P.lat_ef;
+  const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * P.lat_ef;
   // KB elements per thread at a time: all their loads are issued first
   for (int32_t e = zs.vt; e < n; e += KB * zs.vbs) {
     double x[KB], y[KB];
@@ -582,8 +585,9 @@ __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
       const int32_t ek = min(e + k * zs.vbs, n - 1);
       int32_t r, i;
       if constexpr (Z::kGlobal) {
-        i = ek / R;
-        r = ek - i * R;
+        const int32_t blk = (ek >> 2) / R;
+        r = (ek >> 2) - blk * R;
+        i = 4 * blk + (ek & 3);
       } else {
         r = ek / ES;
         i = ek - r * ES;
@@ -662,17 +666,18 @@ __device__ __forceinline__ void lattice_transform_lds(const DevProblem& P, const
 // 32 rows x 32 modes of one parity as 2 x 2 blocks of 16 x 16 and runs
 // v_mfma_f64_16x16x4f64 over i in steps of 4; lane l holds A[i][k] =
 // F[r0 + l%16][i0 + l/16] and B[k][j] = S[i0 + l/16][2(m0 + l%16) + p], and
-// result register q of lane l is D[l/16 + 4q][l%16]. E and O are kept by
-// column (E[i][r] at lat_t0 + i*R + r), so each operand load is four
-// 128-byte rows. kMfmaBatch steps of loads are issued before their products.
-// Rows and modes past the lattice load clamped (finite) operands and are not
-// stored; terms i >= ES have A = 0. The matrix cores round differently from a
-// chain of fused multiply-adds: the two stores agree to rounding.
+// result register q of lane l is D[l/16 + 4q][l%16]. E and O are kept in
+// blocks of four i by row, so each A load is 512 contiguous bytes.
+// kMfmaBatch steps of loads are issued before their products. Rows and modes
+// past the lattice load clamped (finite) operands and are not stored; the
+// folded rows are zero for i >= H, and table rows past ES are read clamped.
+// The matrix cores round differently from a chain of fused multiply-adds: the
+// two stores agree to rounding.
 constexpr int kMfmaBatch = 4;
 template <class Z>
 __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, const Z& zs) {
   constexpr int U = kMfmaBatch;
-  const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, CP = P.lat_cp;
+  const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, EF = P.lat_ef, CP = P.lat_cp;
   const int lane = zs.vt & (kWave - 1);
   const int32_t wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
   const int32_t NW = zs.vbs / kWave;
@@ -692,7 +697,7 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int32_t r = min(r0 + 16 * b + lr, R - 1);
-      fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * ES * R + r;
+      fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * EF * R + 4 * r + lk;
       const int32_t j = min(2 * (m0 + 16 * b + lr) + par, C - 1);
       sb[b] = S + j;
     }
@@ -701,19 +706,20 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int c = 0; c < 2; ++c) acc[b][c] = vdouble4{0.0, 0.0, 0.0, 0.0};
-    for (int32_t i0 = 0; i0 < ES; i0 += 4 * U) {
+    for (int32_t i0 = 0; i0 < EF; i0 += 4 * U) {
       double av[U][2], bv[U][2];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int32_t i = i0 + 4 * u + lk;
-        const int32_t ic = min(i, ES - 1);
+        const int32_t i4 = i0 + 4 * u;                     // this step's first i (a multiple of 4)
+        const int32_t blk = min(i4, EF - 4) >> 2;
+        const int32_t ic = min(i4 + lk, ES - 1);
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          av[u][b] = fa[b][static_cast<int64_t>(ic) * R];
+          av[u][b] = fa[b][static_cast<int64_t>(blk) * R * 4];
           bv[u][b] = sb[b][static_cast<int64_t>(ic) * CP];
         }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) av[u][b] = i < ES ? av[u][b] : 0.0;
+        for (int b = 0; b < 2; ++b) av[u][b] = i4 < EF ? av[u][b] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -1254,6 +1260,82 @@ __device__ __forceinline__ void jfast_compute(const DevProblem& P, const Junctio
   J.u[j] = x * L.rc0;
 }
 
+// The common step of a global store whose junction coefficients and noise
+// amplitude are one value each (uniform R and C, constant T): rc0, c1, c2 and
+// sigma are read once, and a lane takes two junction pairs per iteration (q
+// and q + vbs), so twice the loads are in flight per round trip. The same
+// arithmetic as jfast_compute.
+struct JUni {
+  vint2 code;
+  double za, zb, xo, t2, Ic, isx;
+};
+
+template <class Z>
+__device__ __forceinline__ void juni_load(const JunctionStep& J, const Z& zs, int32_t j, vint2 code, JUni& L) {
+  L.code = code;
+  L.za = code.x >= 0 ? zs.rd(code.x >> 1) : 0.0;
+  L.zb = code.y >= 0 ? zs.rd(code.y >> 1) : 0.0;
+  L.xo = J.th_n[j];
+  L.t2 = J.th_k[1][j];
+  L.Ic = J.Ic[j];
+  L.isx = term_at(J.Is_x, j);
+}
+
+__device__ __forceinline__ void juni_compute(const DevProblem& P, const JunctionStep& J, int32_t j, const JUni& L,
+                                             double rc0, double c1, double c2, double sg, double z) {
+  double y = 0.0;
+  if (L.code.x >= 0) y = (L.code.x & 1) ? y - L.za : y + L.za;
+  if (L.code.y >= 0) y = (L.code.y & 1) ? y - L.zb : y + L.zb;
+  const double t1 = (y - L.xo) * rc0;
+  J.th_n[j] = t1;
+  const double X = L.Ic * cpr_sum(P, 2.0 * t1 - L.t2) + c1 * t1 + c2 * L.t2;
+  double eta = 0.0;
+  if (!J.t_zero) eta = sg * z;
+  const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
+  J.th_x[j] = x;
+  J.u[j] = x * rc0;
+}
+
+template <class Z>
+__device__ __forceinline__ void junction_loop_uniform(const DevProblem& P, const JunctionStep& J, const Z& zs) {
+  const int32_t Nj = P.Nj;
+  const int32_t tid = J.vt, vbs = J.vbs;
+  const int32_t last = Nj - 1;
+  const bool gauss = !J.t_zero;
+  const GAS vint2* __restrict__ codes = gbl(reinterpret_cast<const vint2*>(P.junc_code));
+  const double rc0 = rc0_at(P, 0), c1 = coef_at(P, 1, 0), c2 = coef_at(P, 2, 0), sg = term_at(J.sig_x, 0);
+  // codes of the pairs q and q + vbs, loaded one iteration ahead (clamped
+  // into range; a pair past the end is loaded and not used)
+  auto code = [&](int32_t j) { return codes[min(j, last)]; };
+  vint2 n[4];
+  n[0] = code(2 * tid);
+  n[1] = code(2 * tid + 1);
+  n[2] = code(2 * (tid + vbs));
+  n[3] = code(2 * (tid + vbs) + 1);
+  for (int32_t q = tid; 2 * q < Nj; q += 2 * vbs) {
+    const int32_t q2 = q + vbs;
+    const int32_t j[4] = {2 * q, 2 * q + 1, 2 * q2, 2 * q2 + 1};
+    JUni L[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) juni_load(J, zs, min(j[k], last), n[k], L[k]);
+    const int32_t qn = q + 2 * vbs;
+    n[0] = code(2 * qn);
+    n[1] = code(2 * qn + 1);
+    n[2] = code(2 * (qn + vbs));
+    n[3] = code(2 * (qn + vbs) + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    // the Philox draws run while the loads are in flight
+    double z[4] = {0.0, 0.0, 0.0, 0.0};
+    if (gauss) {
+      normal_pair(J.seed, static_cast<uint32_t>(j[0]), J.noise_col, J.tx, &z[0], &z[1]);
+      if (j[2] < Nj) normal_pair(J.seed, static_cast<uint32_t>(j[2]), J.noise_col, J.tx, &z[2], &z[3]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (j[k] < Nj) juni_compute(P, J, j[k], L[k], rc0, c1, c2, sg, z[k]);
+  }
+}
+
 template <int BS, class Z>
 __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
@@ -1338,7 +1420,8 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   J.noise_col = noise_col;
   J.tx = static_cast<uint64_t>(tx);
   const bool fast = SW == 3 && update && prep && !ths && !J.th_out && !J.I_out && !J.noise && !J.from_T;
-  if (fast) junction_loop_fast<BS, Z>(P, J, zs);
+  if (fast && Z::kGlobal && P.coef_si == 0 && J.sig_x.si == 0) junction_loop_uniform<Z>(P, J, zs);
+  else if (fast) junction_loop_fast<BS, Z>(P, J, zs);
   else if (update && prep) junction_loop<BS, SW, true, true, Z>(P, J, zs);
   else if (update) junction_loop<BS, SW, true, false, Z>(P, J, zs);
   else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
@@ -1959,7 +2042,8 @@ int validate(const jja_problem_desc& d) {
     if (d.n_solve_slices != 0 || d.face_copy_rows != 0)
       return fail(JJA_EINVAL, "a lattice solve has no solve records and no face copy rows");
     const int64_t B = std::max<int64_t>(1, std::min<int64_t>(R, int64_t{JJA_LATTICE_TRI_THREADS} * parts / C));
-    if (ms < (Nf & 1) + std::max(2 * R * ES, 4 * B * C))
+    const int64_t EF = d.index_bytes == 4 ? (ES + 3) & ~int64_t{3} : ES;
+    if (ms < (Nf & 1) + std::max(2 * R * EF, 4 * B * C))
       return fail(JJA_EINVAL, "max_stage_rows too small for the lattice solve");
     if (!(d.lattice_g > 0.0) || !std::isfinite(d.lattice_g)) return fail(JJA_EINVAL, "lattice_g must be finite and > 0");
     return JJA_OK;
@@ -2350,6 +2434,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       dp.lat_rows = R;
       dp.lat_cols = C;
       dp.lat_es = H + (H & 1);
+      dp.lat_ef = d.index_bytes == 4 ? (dp.lat_es + 3) & ~3 : dp.lat_es;
       dp.lat_cp = (C + JJA_LATTICE_CHUNK - 1) / JJA_LATTICE_CHUNK * JJA_LATTICE_CHUNK;
       dp.lat_t0 = static_cast<int64_t>(d.n_faces) + (d.n_faces & 1);
       dp.lat_g = d.lattice_g;

@@ -128,11 +128,14 @@ class LatticeSolve:
             cprev = (-g_r) * inv[r, :C]
         self.inv = np.ascontiguousarray(inv)
 
-    def scratch_rows(self, Nf, parts=1):
+    def scratch_rows(self, Nf, parts=1, global_store=False):
         """Doubles after the face vector, at an even offset (16-byte loads): the
-        folded rows (E and O, R x ES each), which the tridiagonal solves reuse
-        for their forward and backward block ends (values and products, 4 B C)."""
-        return (Nf & 1) + max(2 * self.R * self.ES, 4 * self.tri_blocks(parts) * self.C)
+        folded rows (E and O, R x EF each; EF = ES in LDS, ES rounded up to 4 in
+        a global store, which keeps them in blocks of four), which the
+        tridiagonal solves reuse for their forward and backward block ends
+        (values and products, 4 B C)."""
+        EF = (self.ES + 3) & ~3 if global_store else self.ES
+        return (Nf & 1) + max(2 * self.R * EF, 4 * self.tri_blocks(parts) * self.C)
 
     def tri_blocks(self, parts=1):
         """Blocks each mode's tridiagonal recurrence is cut into, for a column

@@ -744,7 +744,7 @@ class CircuitPlan:
         self.global_store = self.face_store == "global" or not fits or self.parts > 1
         self.index_bytes = 4 if self.global_store else 2
         self.local_stages = self.part_stages = 0
-        self.scratch_rows = scratch
+        self.scratch_rows = lat.scratch_rows(Nf, self.parts, self.global_store)
         pos = lat.pos_of_face
         jc = self.junc_code.reshape(Nj, 2).astype(np.int64)
         jpos = np.where(jc >= 0, 2 * pos[np.maximum(jc, 0) >> 1] + (jc & 1), -1)
