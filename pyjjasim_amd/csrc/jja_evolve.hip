@@ -558,7 +558,7 @@ struct RunsLattice<SplitStore> {
   static constexpr bool value = true;
 };
 
-// elements of the fold each thread loads before it computes
+// elements of the fold each thread loads before it computes (global store)
 constexpr int kFoldBatch = 4;
 
 // E[r][i] = v[r][i] + v[r][C-1-i], O[r][i] = v[r][i] - v[r][C-1-i] for i < H
@@ -572,7 +572,7 @@ constexpr int kFoldBatch = 4;
 // Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*EF + e.
 template <class Z>
 __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
-  constexpr int KB = kFoldBatch;
+  constexpr int KB = Z::kGlobal ? kFoldBatch : 1;
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, H = (C + 1) >> 1;
   const int32_t n = R * P.lat_ef;
   const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * P.lat_ef;
@@ -1260,82 +1260,6 @@ __device__ __forceinline__ void jfast_compute(const DevProblem& P, const Junctio
   J.u[j] = x * L.rc0;
 }
 
-// The common step of a global store whose junction coefficients and noise
-// amplitude are one value each (uniform R and C, constant T): rc0, c1, c2 and
-// sigma are read once, and a lane takes two junction pairs per iteration (q
-// and q + vbs), so twice the loads are in flight per round trip. The same
-// arithmetic as jfast_compute.
-struct JUni {
-  vint2 code;
-  double za, zb, xo, t2, Ic, isx;
-};
-
-template <class Z>
-__device__ __forceinline__ void juni_load(const JunctionStep& J, const Z& zs, int32_t j, vint2 code, JUni& L) {
-  L.code = code;
-  L.za = code.x >= 0 ? zs.rd(code.x >> 1) : 0.0;
-  L.zb = code.y >= 0 ? zs.rd(code.y >> 1) : 0.0;
-  L.xo = J.th_n[j];
-  L.t2 = J.th_k[1][j];
-  L.Ic = J.Ic[j];
-  L.isx = term_at(J.Is_x, j);
-}
-
-__device__ __forceinline__ void juni_compute(const DevProblem& P, const JunctionStep& J, int32_t j, const JUni& L,
-                                             double rc0, double c1, double c2, double sg, double z) {
-  double y = 0.0;
-  if (L.code.x >= 0) y = (L.code.x & 1) ? y - L.za : y + L.za;
-  if (L.code.y >= 0) y = (L.code.y & 1) ? y - L.zb : y + L.zb;
-  const double t1 = (y - L.xo) * rc0;
-  J.th_n[j] = t1;
-  const double X = L.Ic * cpr_sum(P, 2.0 * t1 - L.t2) + c1 * t1 + c2 * L.t2;
-  double eta = 0.0;
-  if (!J.t_zero) eta = sg * z;
-  const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
-  J.th_x[j] = x;
-  J.u[j] = x * rc0;
-}
-
-template <class Z>
-__device__ __forceinline__ void junction_loop_uniform(const DevProblem& P, const JunctionStep& J, const Z& zs) {
-  const int32_t Nj = P.Nj;
-  const int32_t tid = J.vt, vbs = J.vbs;
-  const int32_t last = Nj - 1;
-  const bool gauss = !J.t_zero;
-  const GAS vint2* __restrict__ codes = gbl(reinterpret_cast<const vint2*>(P.junc_code));
-  const double rc0 = rc0_at(P, 0), c1 = coef_at(P, 1, 0), c2 = coef_at(P, 2, 0), sg = term_at(J.sig_x, 0);
-  // codes of the pairs q and q + vbs, loaded one iteration ahead (clamped
-  // into range; a pair past the end is loaded and not used)
-  auto code = [&](int32_t j) { return codes[min(j, last)]; };
-  vint2 n[4];
-  n[0] = code(2 * tid);
-  n[1] = code(2 * tid + 1);
-  n[2] = code(2 * (tid + vbs));
-  n[3] = code(2 * (tid + vbs) + 1);
-  for (int32_t q = tid; 2 * q < Nj; q += 2 * vbs) {
-    const int32_t q2 = q + vbs;
-    const int32_t j[4] = {2 * q, 2 * q + 1, 2 * q2, 2 * q2 + 1};
-    JUni L[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) juni_load(J, zs, min(j[k], last), n[k], L[k]);
-    const int32_t qn = q + 2 * vbs;
-    n[0] = code(2 * qn);
-    n[1] = code(2 * qn + 1);
-    n[2] = code(2 * (qn + vbs));
-    n[3] = code(2 * (qn + vbs) + 1);
-    __builtin_amdgcn_sched_barrier(0);
-    // the Philox draws run while the loads are in flight
-    double z[4] = {0.0, 0.0, 0.0, 0.0};
-    if (gauss) {
-      normal_pair(J.seed, static_cast<uint32_t>(j[0]), J.noise_col, J.tx, &z[0], &z[1]);
-      if (j[2] < Nj) normal_pair(J.seed, static_cast<uint32_t>(j[2]), J.noise_col, J.tx, &z[2], &z[3]);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (j[k] < Nj) juni_compute(P, J, j[k], L[k], rc0, c1, c2, sg, z[k]);
-  }
-}
-
 template <int BS, class Z>
 __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
@@ -1420,8 +1344,7 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   J.noise_col = noise_col;
   J.tx = static_cast<uint64_t>(tx);
   const bool fast = SW == 3 && update && prep && !ths && !J.th_out && !J.I_out && !J.noise && !J.from_T;
-  if (fast && Z::kGlobal && P.coef_si == 0 && J.sig_x.si == 0) junction_loop_uniform<Z>(P, J, zs);
-  else if (fast) junction_loop_fast<BS, Z>(P, J, zs);
+  if (fast) junction_loop_fast<BS, Z>(P, J, zs);
   else if (update && prep) junction_loop<BS, SW, true, true, Z>(P, J, zs);
   else if (update) junction_loop<BS, SW, true, false, Z>(P, J, zs);
   else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
