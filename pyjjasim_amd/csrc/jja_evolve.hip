@@ -32,6 +32,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/jja.h"
@@ -132,7 +133,10 @@ struct DevProblem {
   // lattice solve (lat_rows 0: the staged factor program solves K)
   int32_t lat_rows, lat_cols;  // R, C: faces at positions r*C + c
   int32_t lat_es, lat_cp;      // folded row length ES (even) and sine-table row stride CP
-  int32_t lat_ef, lat_pad_;    // folded rows stored per row: ES (LDS store), ES rounded up to 4 (global)
+  int32_t lat_ef;              // folded rows stored per row: ES (LDS store), ES rounded up to 4 (global)
+  int32_t lat_mp, lat_bp;      // split column: modes per part and blocks per mode of the part-local
+                               // tridiagonal solve in LDS (lat_mp 0: the column-wide one)
+  int32_t lat_pad_;
   int64_t lat_t0;              // store offset of the folded rows (even: 16-byte loads)
   const double* lat_sine;      // [ES][CP]
   const double* lat_inv;       // [R][CP] inverse Thomas pivots
@@ -281,6 +285,10 @@ struct FaceStore<true> {
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
+// Dynamic LDS a split workgroup may take for its part of a lattice solve (the
+// workgroup's 160 KiB less room for its static variables)
+constexpr int64_t kSplitLdsMax = 163840 - 256;
+
 // Default polls of one column barrier before a workgroup gives up (each poll
 // sleeps briefly; far longer than any phase of a step takes).
 constexpr uint32_t kWaitLimit = 1u << 21;
@@ -306,6 +314,7 @@ struct SplitStore {
   int32_t* stopped;      // this workgroup's flag (LDS): it no longer waits
   mutable uint32_t gen;  // arrivals to wait for (thread 0)
   mutable bool one_xcd;  // every part of the column shares this XCD's L2 (thread 0)
+  LAS double* lds;       // this workgroup's dynamic LDS (the part-local tridiagonal solve)
   __device__ __forceinline__ double rd(int64_t i) const { return p[i]; }
   __device__ __forceinline__ double gather(uint32_t i) const { return p[i]; }
   __device__ __forceinline__ void wr(int64_t i, double v) const { p[i] = v; }
@@ -793,8 +802,118 @@ __device__ __forceinline__ double lattice_carry(const Z& zs, int32_t C, int32_t 
   return cin;
 }
 
+// The tridiagonal solves of a split column with each part's modes in its own
+// LDS: part p takes modes [p*MP, p*MP + MP) (MP = lat_mp) with all R rows
+// (Y[r][kk], R*MP doubles), cuts each mode into BP = lat_bp blocks and runs
+// the same blocked recurrences as lattice_tridiag between workgroup barriers;
+// the block ends (forward and backward, value and product, 4*BP*MP doubles)
+// stay in LDS, so no column barrier is needed inside, and the solved rows are
+// written back to the face vector.
+__device__ __forceinline__ void lattice_tridiag_parts(const DevProblem& P, const SplitStore& zs) {
+  constexpr int TB = kTriRows;
+  const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp, MP = P.lat_mp, B = P.lat_bp;
+  const int32_t nt = static_cast<int32_t>(blockDim.x), tid = static_cast<int32_t>(threadIdx.x);
+  const int32_t part = __builtin_amdgcn_readfirstlane(zs.vt / nt);
+  const int32_t k0 = part * MP, nm = min(MP, C - k0);
+  if (nm <= 0) return;  // a part past the last mode (the whole workgroup)
+  const GAS double* __restrict__ inv = gbl(P.lat_inv);
+  const double g = P.lat_g;
+  const int32_t m = (R + B - 1) / B;
+  LAS double* Y = zs.lds;
+  LAS double* fv = Y + static_cast<int32_t>(R) * MP;
+  LAS double* fp = fv + B * MP;
+  LAS double* bv = fp + B * MP;
+  LAS double* bp = bv + B * MP;
+  for (int32_t e = tid; e < R * nm; e += nt) {
+    const int32_t r = e / nm, kk = e - r * nm;
+    Y[r * MP + kk] = zs.rd(static_cast<int64_t>(r) * C + k0 + kk);
+  }
+  __syncthreads();
+  auto sweep = [&](int32_t kk, int32_t first, int32_t count, int32_t dir, auto&& f) {
+    for (int32_t c0 = 0; c0 < count; c0 += TB) {
+      double y[TB], a[TB];
+#pragma unroll
+      for (int q = 0; q < TB; ++q) {
+        const int32_t r = first + dir * min(c0 + q, count - 1);
+        y[q] = Y[r * MP + kk];
+        a[q] = inv[static_cast<int64_t>(r) * CP + k0 + kk];
+      }
+#pragma unroll
+      for (int q = 0; q < TB; ++q)
+        if (c0 + q < count) f(first + dir * (c0 + q), y[q], a[q]);
+    }
+  };
+  // the value carried into block b (dir +1: from blocks 0..b-1, upward; dir
+  // -1: from blocks B-1..b+1, downward), chained in block order
+  auto carry = [&](const LAS double* ev, const LAS double* ep, int32_t b, int32_t kk, int dir) {
+    const int32_t n = dir > 0 ? b : B - 1 - b, first = dir > 0 ? 0 : B - 1;
+    double cin = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+      const int32_t at = (first + dir * i) * MP + kk;
+      cin = __builtin_fma(ep[at], cin, ev[at]);
+    }
+    return cin;
+  };
+  // forward from zero
+  for (int32_t e = tid; e < B * nm; e += nt) {
+    const int32_t b = e / nm, kk = e - b * nm, r0 = b * m, n = min(m, R - r0);
+    double d = 0.0, pr = 1.0;
+    if (n > 0)
+      sweep(kk, r0, n, 1, [&](int32_t r, double y, double iv) {
+        d = __builtin_fma(g, d, y) * iv;
+        pr = pr * (g * iv);
+        Y[r * MP + kk] = d;
+      });
+    fv[b * MP + kk] = d;
+    fp[b * MP + kk] = pr;
+  }
+  __syncthreads();
+  // forward carries, then backward from zero (the same thread's rows)
+  for (int32_t e = tid; e < B * nm; e += nt) {
+    const int32_t b = e / nm, kk = e - b * nm, r0 = b * m, n = min(m, R - r0);
+    double x = 0.0, pr = 1.0;
+    if (n > 0) {
+      if (b > 0) {
+        const double cin = carry(fv, fp, b, kk, 1);
+        double pf = 1.0;
+        sweep(kk, r0, n, 1, [&](int32_t r, double y, double iv) {
+          pf = pf * (g * iv);
+          Y[r * MP + kk] = __builtin_fma(pf, cin, y);
+        });
+      }
+      sweep(kk, r0 + n - 1, n, -1, [&](int32_t r, double dv, double iv) {
+        const double a = g * iv;
+        x = __builtin_fma(a, x, dv);
+        pr = pr * a;
+        Y[r * MP + kk] = x;
+      });
+    }
+    bv[b * MP + kk] = x;
+    bp[b * MP + kk] = pr;
+  }
+  __syncthreads();
+  // backward carries, and the rows back to the face vector
+  for (int32_t e = tid; e < B * nm; e += nt) {
+    const int32_t b = e / nm, kk = e - b * nm, r0 = b * m, n = min(m, R - r0);
+    if (n <= 0) continue;
+    const bool last = b == B - 1;
+    const double cin = last ? 0.0 : carry(bv, bp, b, kk, -1);
+    double pr = 1.0;
+    sweep(kk, r0 + n - 1, n, -1, [&](int32_t r, double x, double iv) {
+      pr = pr * (g * iv);
+      zs.wr(static_cast<int64_t>(r) * C + k0 + kk, last ? x : __builtin_fma(pr, cin, x));
+    });
+  }
+}
+
 template <class Z>
 __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs) {
+  if constexpr (std::is_same<Z, SplitStore>::value) {
+    if (P.lat_mp) {
+      lattice_tridiag_parts(P, zs);
+      return;
+    }
+  }
   constexpr int TB = kTriRows;
   const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp;
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
@@ -1436,7 +1555,7 @@ __device__ __forceinline__ void evolve_column(const DevProblem& P, const jja_pro
 __device__ __forceinline__ SplitStore split_store(const DevProblem& P, GAS double* zbuf, int64_t w,
                                                   int32_t part, uint32_t* ctrs, const WorkItem& wi,
                                                   int32_t* err, uint32_t wait_limit, int32_t* stopped,
-                                                  int BS) {
+                                                  int BS, LAS double* lds) {
   SplitStore zs;
   zs.p = zbuf + static_cast<size_t>(w) * (static_cast<size_t>(P.Nf) + P.max_stage_rows);
   zs.vt = part * BS + static_cast<int32_t>(threadIdx.x);
@@ -1449,6 +1568,7 @@ __device__ __forceinline__ SplitStore split_store(const DevProblem& P, GAS doubl
   zs.stopped = stopped;
   zs.gen = 0;
   zs.one_xcd = false;
+  zs.lds = lds;
   if (threadIdx.x == 0 && P.parts > 1)
     __hip_atomic_fetch_or(zs.xcds, 1u << xcd_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return zs;
@@ -1462,6 +1582,7 @@ __global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __re
                                                           const WorkItem* __restrict__ work,
                                                           int32_t n_steps, int64_t step0, uint32_t* ctrs,
                                                           int32_t* err, uint32_t wait_limit) {
+  extern __shared__ double split_lds[];
   __shared__ int32_t stopped;
   if (threadIdx.x == 0) stopped = 0;
   __syncthreads();
@@ -1473,12 +1594,14 @@ __global__ __launch_bounds__(BS) void evolve_split_kernel(const DevProblem* __re
   if constexpr (PAIR) {
     const int64_t wa = 2 * w;
     const bool two = wa + 1 < R.W;
-    const SplitStore za = split_store(P, gblw(R.zbuf), wa, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    const SplitStore za = split_store(P, gblw(R.zbuf), wa, part, ctrs, wi, err, wait_limit, &stopped, BS,
+                                        (LAS double*)split_lds);
     SplitStore zb = za;
     if (two) zb.p = za.p + (static_cast<size_t>(P.Nf) + P.max_stage_rows);
     evolve_steps_pair<BS, SW>(P, R, wa, two, za, zb, n_steps, step0);
   } else {
-    const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    const SplitStore zs = split_store(P, gblw(R.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS,
+                                        (LAS double*)split_lds);
     evolve_steps<BS, SW>(P, R, w, zs, n_steps, step0);
   }
 }
@@ -1670,6 +1793,7 @@ __global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __r
                                                            const WorkItem* __restrict__ work, int64_t k_hi,
                                                            int64_t k_lo, uint32_t* ctrs, int32_t* err,
                                                            uint32_t wait_limit) {
+  extern __shared__ double split_lds[];
   __shared__ int32_t stopped;
   if (threadIdx.x == 0) stopped = 0;
   __syncthreads();
@@ -1681,12 +1805,14 @@ __global__ __launch_bounds__(BS) void adjoint_split_kernel(const DevProblem* __r
   if constexpr (PAIR) {
     const int64_t wa = 2 * w;
     const bool two = wa + 1 < A.W;
-    const SplitStore za = split_store(P, gblw(A.zbuf), wa, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    const SplitStore za = split_store(P, gblw(A.zbuf), wa, part, ctrs, wi, err, wait_limit, &stopped, BS,
+                                        (LAS double*)split_lds);
     SplitStore zb = za;
     if (two) zb.p = za.p + (static_cast<size_t>(P.Nf) + P.max_stage_rows);
     adjoint_steps_pair<BS>(P, A, wa, two, za, zb, k_hi, k_lo);
   } else {
-    const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS);
+    const SplitStore zs = split_store(P, gblw(A.zbuf), w, part, ctrs, wi, err, wait_limit, &stopped, BS,
+                                        (LAS double*)split_lds);
     adjoint_steps<BS>(P, A, w, zs, k_hi, k_lo);
   }
 }
@@ -1780,6 +1906,7 @@ struct Plan {
                                      // [1] workgroups whose column ran on one XCD
   int32_t n_cu = 0;                  // compute units of the device
   uint32_t wait_limit = kWaitLimit;  // split launches: polls before a column barrier gives up
+  int64_t split_lds = 0;             // split launches: dynamic LDS bytes (part-local lattice solves)
   // what the device copies of the launch descriptors hold: a launch uploads
   // only what changed since the previous one
   std::vector<jja_problem_run> h_runs;
@@ -2190,21 +2317,24 @@ int reset_counters(Plan* plan, size_t n, hipStream_t stream) {
 }
 
 // Cooperative launch of a split kernel: all n workgroups resident at once (the
-// column barriers need every part of a column running), or an error.
+// column barriers need every part of a column running), or an error. Each
+// workgroup gets the plan's split_lds bytes of dynamic LDS.
 template <typename... Args>
 int launch_cooperative(Plan* plan, const void* kern, int BS, int32_t n, hipStream_t stream, Args... args) {
+  const int lds = static_cast<int>(plan->split_lds);
   int per_cu = -1;
   for (const auto& a : plan->per_cu)
     if (a.first == kern) per_cu = a.second;
   if (per_cu < 0) {
-    JJA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, 0));
+    if (lds > 0) JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    JJA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, static_cast<size_t>(lds)));
     plan->per_cu.push_back({kern, per_cu});
   }
   if (static_cast<int64_t>(per_cu) * plan->n_cu < n)
     return fail(JJA_EUNSUPPORTED, "split launch of " + std::to_string(n) + " workgroups; the device holds " +
                                       std::to_string(static_cast<int64_t>(per_cu) * plan->n_cu) + " at once");
   void* argv[] = {static_cast<void*>(&args)...};
-  JJA_HIP(hipLaunchCooperativeKernel(kern, dim3(n), dim3(BS), argv, 0, stream));
+  JJA_HIP(hipLaunchCooperativeKernel(kern, dim3(n), dim3(BS), argv, static_cast<size_t>(lds), stream));
   return JJA_OK;
 }
 
@@ -2358,6 +2488,19 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       dp.lat_cols = C;
       dp.lat_es = H + (H & 1);
       dp.lat_ef = d.index_bytes == 4 ? (dp.lat_es + 3) & ~3 : dp.lat_es;
+      // a split column solves its tridiagonal systems part by part in LDS
+      // when a part's modes and block ends fit
+      const int32_t np = parts_of(d);
+      if (np > 1) {
+        const int32_t mp = (C + np - 1) / np;
+        const int32_t bp = std::max(1, std::min(R, JJA_LATTICE_TRI_THREADS / mp));
+        const int64_t need = (static_cast<int64_t>(R) * mp + 4 * static_cast<int64_t>(bp) * mp) * 8;
+        if (need <= kSplitLdsMax) {
+          dp.lat_mp = mp;
+          dp.lat_bp = bp;
+          plan->split_lds = std::max(plan->split_lds, need);
+        }
+      }
       dp.lat_cp = (C + JJA_LATTICE_CHUNK - 1) / JJA_LATTICE_CHUNK * JJA_LATTICE_CHUNK;
       dp.lat_t0 = static_cast<int64_t>(d.n_faces) + (d.n_faces & 1);
       dp.lat_g = d.lattice_g;

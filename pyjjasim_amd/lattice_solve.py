@@ -47,8 +47,10 @@ from . import _native
 # sine modes one wave accumulates at once (even, so a mode's parity is that of
 # its slot); the sine table's rows are padded to a multiple of it
 CHUNK = 14
-# each mode's tridiagonal recurrence runs in min(R, TRI_THREADS * parts // C) blocks
+# each mode's tridiagonal recurrence runs in min(R, TRI_THREADS * parts // C) blocks (column-wide)
 TRI_THREADS = 1024
+# dynamic LDS a split workgroup may take for its part-local tridiagonal solve
+SPLIT_LDS_MAX = 163840 - 256
 
 
 def _kron_form(R, C, g_r, g_c):
@@ -135,12 +137,35 @@ class LatticeSolve:
         tridiagonal solves reuse for their forward and backward block ends
         (values and products, 4 B C)."""
         EF = (self.ES + 3) & ~3 if global_store else self.ES
-        return (Nf & 1) + max(2 * self.R * EF, 4 * self.tri_blocks(parts) * self.C)
+        return (Nf & 1) + max(2 * self.R * EF, 4 * self.column_tri_blocks(parts) * self.C)
+
+    def column_tri_blocks(self, parts=1):
+        """Blocks per mode of the column-wide tridiagonal solve (block ends in
+        the face store): JJA_LATTICE_TRI_THREADS per part, over C."""
+        return max(1, min(self.R, TRI_THREADS * parts // self.C))
+
+    def part_local(self, parts):
+        """(modes per part, blocks per mode) of the part-local tridiagonal
+        solve a split column runs in each workgroup's LDS (all R rows of
+        ceil(C / parts) modes and 4 block-end arrays), or None when it does
+        not fit (the column-wide solve then runs)."""
+        if parts <= 1:
+            return None
+        mp = -(-self.C // parts)
+        bp = max(1, min(self.R, TRI_THREADS // mp))
+        need = 8 * (self.R * mp + 4 * bp * mp)
+        return (mp, bp) if need <= SPLIT_LDS_MAX else None
+
+    def split_lds_bytes(self, parts):
+        """Dynamic LDS per workgroup of a split column's part-local solve (0 if none)."""
+        pl = self.part_local(parts)
+        return 0 if pl is None else 8 * (self.R * pl[0] + 4 * pl[1] * pl[0])
 
     def tri_blocks(self, parts=1):
         """Blocks each mode's tridiagonal recurrence is cut into, for a column
-        worked by ``parts`` workgroups."""
-        return max(1, min(self.R, TRI_THREADS * parts // self.C))
+        worked by ``parts`` workgroups (the part-local solve's where it runs)."""
+        pl = self.part_local(parts)
+        return pl[1] if pl is not None else self.column_tri_blocks(parts)
 
     def table_bytes(self):
         """Bytes of the solve's tables (the tabulated half of the sine matrix
