@@ -282,7 +282,8 @@ typedef struct jja_problem_run {
   double dt;
   double* theta;         /* [stencil_width-1][W][Nj]; theta at step m in slot m mod (sw-1) */
   double* theta_s;       /* [W][Nj] voltage-source phase; NULL when Vs.zero */
-  double* u;             /* [W][Nj] scratch: x/c0 - theta_s of the step being solved */
+  double* u;             /* unused, may be NULL: the face phase forms x/c0 - theta_s of the step
+                            being solved from its history slot */
   jja_source Is, Vs, f, T;
   /* noise amplitude sqrt(2 T Rv) when T is constant in time (sigma.base != NULL);
    * otherwise the kernel forms it from T */

@@ -1149,10 +1149,25 @@ __device__ __forceinline__ void face_rec_load(const DevProblem& P, int32_t k, bo
   r.fid = rowed ? gbl(P.face_fid)[m.rowbase + tid] : -1;
 }
 
+// The junction values u_j the face phase sums: u itself (the adjoint), or
+// formed from the step's x as the junction pass would have: x_j / c0 (a
+// product with the reciprocal), less theta_s with voltage sources.
+struct JuncSource {
+  const GAS double* v;
+  const GAS double* ths;
+  bool scaled;
+};
+
+__device__ __forceinline__ double junc_value(const DevProblem& P, const JuncSource& s, int32_t j) {
+  if (!s.scaled) return s.v[j];
+  double u = s.v[j] * rc0_at(P, j);
+  if (s.ths) u = u - s.ths[j];
+  return u;
+}
+
 template <int BS, class Z>
-__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f,
-                                           const GAS double* __restrict__ u, const Z& zs,
-                                           int64_t w, int64_t t) {
+__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f, const JuncSource& us,
+                                           const Z& zs, int64_t w, int64_t t) {
   const int tid = zs.vt;
   // flux of face q at this step is fv.p[q * fv.si] * fv.amp; an absent flux
   // reads an exact zero, and acc - 2 pi * 0 leaves acc unchanged
@@ -1168,7 +1183,7 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source
 #pragma unroll
     for (int g = 0; g < kFaceGroup; ++g) {
 #pragma unroll
-      for (int i = 0; i < kFaceIter; ++i) uu[g][i] = r[g].cd[i] >= 0 ? u[r[g].cd[i] >> 1] : 0.0;
+      for (int i = 0; i < kFaceIter; ++i) uu[g][i] = r[g].cd[i] >= 0 ? junc_value(P, us, r[g].cd[i] >> 1) : 0.0;
       fq[g] = r[g].fid >= 0 ? term_at(fv, r[g].fid) : 0.0;
     }
 #pragma unroll
@@ -1194,7 +1209,6 @@ struct JunctionStep {
   const GAS double* th_k[4];   // theta_{tx-1-k}, k = 0..S-1 (k = 0 unused with UPDATE)
   GAS double* th_x;            // slot of step tx: receives x_tx
   GAS double* ths;             // voltage-source phase (NULL without Vs)
-  GAS double* u;
   GAS double* th_out;          // stored theta_t row or NULL
   GAS double* I_out;           // stored current row or NULL
   TermView Is_t, Vs_t;         // step t (update)
@@ -1213,8 +1227,8 @@ struct JunctionStep {
 // One junction, all options: with UPDATE finish step t (time_evolution.py:
 // 560-580): y = A^T J from the face vector, theta_t = (y - x)/c0, stored
 // outputs, theta_s += Vs dt; with PREP form x of step tx (time_evolution.py:
-// 533-558) and write x to the history slot and u = x/c0 - theta_s to the
-// scratch. All loads are issued before any arithmetic.
+// 533-558) and write x to the history slot (the next face phase forms
+// u = x/c0 - theta_s from it). All loads are issued before any arithmetic.
 template <int SW, bool UPDATE, bool PREP, class Z>
 __device__ __forceinline__ void junction_one(const DevProblem& P, const JunctionStep& J,
                                              const Z& zs, int32_t j, double zj) {
@@ -1267,9 +1281,6 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
     if (!J.t_zero) eta = (J.from_T ? sqrt(2.0 * sg * rv) : sg) * zj;
     const double x = J.is_zero ? eta + X : (eta - isx) + X;
     J.th_x[j] = x;
-    double uj = x * rc0;
-    if (J.ths) uj = uj - thsv;
-    J.u[j] = uj;
   }
 }
 
@@ -1376,7 +1387,6 @@ __device__ __forceinline__ void jfast_compute(const DevProblem& P, const Junctio
   if (!J.t_zero) eta = L.sg * z;
   const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
   J.th_x[j] = x;
-  J.u[j] = x * L.rc0;
 }
 
 template <int BS, class Z>
@@ -1425,7 +1435,7 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
 template <int BS, int SW, class Z>
 __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_problem_run& R,
                                               GAS double* __restrict__ th, GAS double* __restrict__ ths,
-                                              GAS double* __restrict__ u, const Z& zs,
+                                              const Z& zs,
                                               int64_t w, size_t slot_stride, int64_t t, bool update,
                                               int64_t tx, bool prep, uint32_t noise_col) {
   constexpr int32_t S = SW - 1;
@@ -1441,7 +1451,6 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   for (int k = 0; k < 4; ++k) J.th_k[k] = th + (k < S ? slot(tx - 1 - k) : 0);
   J.th_x = th + slot(tx);
   J.ths = ths;
-  J.u = u;
   const size_t col = static_cast<size_t>(w) * P.Nj;
   const int32_t tho = (update && R.th_slot) ? gbl(R.th_slot)[t - R.slot_t0] : -1;
   const int32_t Io = (update && R.I_slot) ? gbl(R.I_slot)[t - R.slot_t0] : -1;
@@ -1469,6 +1478,13 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
 }
 
+// The face phase's junction values of step t: x_t, in history slot t mod
+// (SW - 1), written by the junction pass before it.
+__device__ __forceinline__ JuncSource step_source(const GAS double* th, const GAS double* ths,
+                                                 size_t slot_stride, int SW, int64_t t) {
+  return JuncSource{th + static_cast<size_t>(t % (SW - 1)) * slot_stride, ths, true};
+}
+
 // n_steps steps of column w of one problem, by the threads the store names.
 template <int BS, int SW, class Z>
 __device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_problem_run& R, int64_t w,
@@ -1476,20 +1492,19 @@ __device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_prob
   const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
   GAS double* __restrict__ th = gblw(R.theta) + static_cast<size_t>(w) * P.Nj;
   GAS double* __restrict__ ths = R.theta_s ? gblw(R.theta_s) + static_cast<size_t>(w) * P.Nj : nullptr;
-  GAS double* __restrict__ u = gblw(R.u) + static_cast<size_t>(w) * P.Nj;
   const uint32_t noise_col = static_cast<uint32_t>(R.column_offset + w);
 
-  junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, step0, false, step0, true, noise_col);
+  junction_pass<BS, SW>(P, R, th, ths, zs, w, slot_stride, step0, false, step0, true, noise_col);
   zs.sync();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
     if (P.Nf > 0) {  // a circuit without faces has nothing to solve (K is 0 x 0)
-      face_phase<BS>(P, &R.f, u, zs, w, t);
+      face_phase<BS>(P, &R.f, step_source(th, ths, slot_stride, SW, t), zs, w, t);
       zs.barrier();
       solve_face_system(P, zs);
     }
     // finish step t and prepare step t+1 in one pass
-    junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
+    junction_pass<BS, SW>(P, R, th, ths, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     zs.sync();
   }
 }
@@ -1509,18 +1524,16 @@ __device__ __forceinline__ void evolve_steps_pair(const DevProblem& P, const jja
   GAS double* thb = gblw(R.theta) + cb;
   GAS double* thsa = R.theta_s ? gblw(R.theta_s) + ca : nullptr;
   GAS double* thsb = R.theta_s ? gblw(R.theta_s) + cb : nullptr;
-  GAS double* ua = gblw(R.u) + ca;
-  GAS double* ub = gblw(R.u) + cb;
   const uint32_t na = static_cast<uint32_t>(R.column_offset + wa), nb = static_cast<uint32_t>(R.column_offset + wb);
 
-  junction_pass<BS, SW>(P, R, tha, thsa, ua, za, wa, slot_stride, step0, false, step0, true, na);
-  if (two) junction_pass<BS, SW>(P, R, thb, thsb, ub, zb, wb, slot_stride, step0, false, step0, true, nb);
+  junction_pass<BS, SW>(P, R, tha, thsa, za, wa, slot_stride, step0, false, step0, true, na);
+  if (two) junction_pass<BS, SW>(P, R, thb, thsb, zb, wb, slot_stride, step0, false, step0, true, nb);
   za.sync();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
     if (P.Nf > 0) {
-      face_phase<BS>(P, &R.f, ua, za, wa, t);
-      if (two) face_phase<BS>(P, &R.f, ub, zb, wb, t);
+      face_phase<BS>(P, &R.f, step_source(tha, thsa, slot_stride, SW, t), za, wa, t);
+      if (two) face_phase<BS>(P, &R.f, step_source(thb, thsb, slot_stride, SW, t), zb, wb, t);
       za.barrier();
       if (two)
         solve_pair(P, za, zb);
@@ -1528,8 +1541,8 @@ __device__ __forceinline__ void evolve_steps_pair(const DevProblem& P, const jja
         solve(P, za);
     }
     const bool prep = n + 1 < n_steps;
-    junction_pass<BS, SW>(P, R, tha, thsa, ua, za, wa, slot_stride, t, true, t + 1, prep, na);
-    if (two) junction_pass<BS, SW>(P, R, thb, thsb, ub, zb, wb, slot_stride, t, true, t + 1, prep, nb);
+    junction_pass<BS, SW>(P, R, tha, thsa, za, wa, slot_stride, t, true, t + 1, prep, na);
+    if (two) junction_pass<BS, SW>(P, R, thb, thsb, zb, wb, slot_stride, t, true, t + 1, prep, nb);
     za.sync();
   }
 }
@@ -1721,7 +1734,7 @@ __device__ __forceinline__ void adjoint_steps(const DevProblem& P, const jja_adj
   zs.sync();
   for (int64_t k = k_hi; k >= k_lo; --k) {
     if (P.Nf > 0) {
-      face_phase<BS>(P, nullptr, C.u, zs, w, k);
+      face_phase<BS>(P, nullptr, JuncSource{C.u, nullptr, false}, zs, w, k);
       zs.barrier();
       solve_face_system(P, zs);
     }
@@ -1750,8 +1763,8 @@ __device__ __forceinline__ void adjoint_steps_pair(const DevProblem& P, const jj
   za.sync();
   for (int64_t k = k_hi; k >= k_lo; --k) {
     if (P.Nf > 0) {
-      face_phase<BS>(P, nullptr, Ca.u, za, wa, k);
-      if (two) face_phase<BS>(P, nullptr, Cb.u, zb, wb, k);
+      face_phase<BS>(P, nullptr, JuncSource{Ca.u, nullptr, false}, za, wa, k);
+      if (two) face_phase<BS>(P, nullptr, JuncSource{Cb.u, nullptr, false}, zb, wb, k);
       za.barrier();
       if (two)
         solve_pair(P, za, zb);
@@ -2676,7 +2689,7 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
     const jja_problem_run& r = runs[i];
     if (plan->problems[i].Nj == 0)
       return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " is a matrix plan without junctions");
-    if (r.W <= 0 || !r.theta || !r.u) return fail(JJA_EINVAL, "jja_evolve: run needs W > 0, theta and u");
+    if (r.W <= 0 || !r.theta) return fail(JJA_EINVAL, "jja_evolve: run needs W > 0 and theta");
     if (r.W > plan->max_cols[i])
       return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " has more columns than its max_columns");
     if (r.dt <= 0) return fail(JJA_EINVAL, "jja_evolve: dt must be positive");

@@ -223,7 +223,6 @@ class Engine:
             self.srcs[name] = DeviceSource(torch, problem._raw[name], n, W, Nt, zero)
         self.theta_s = torch.zeros((W, Nj), dtype=f64, device=dev) \
             if not self.srcs["Vs"].src.zero else None
-        self.u = torch.zeros((W, Nj), dtype=f64, device=dev)
         # noise amplitude sqrt(2 T Rv), formed once when T is constant in time
         # (the same expression as time_evolution.py:538, correctly rounded sqrt)
         self.sigma = None
@@ -268,7 +267,6 @@ class Engine:
         run.dt = dt
         run.theta = self.theta.data_ptr()
         run.theta_s = self.theta_s.data_ptr() if self.theta_s is not None else None
-        run.u = self.u.data_ptr()
         if self.sigma is not None:
             run.sigma = self.sigma.src
         run.th_slot = self.th_slot.data_ptr()
