@@ -1151,23 +1151,28 @@ __device__ __forceinline__ void face_rec_load(const DevProblem& P, int32_t k, bo
 
 // The junction values u_j the face phase sums: u itself (the adjoint), or
 // formed from the step's x as the junction pass would have: x_j / c0 (a
-// product with the reciprocal), less theta_s with voltage sources.
+// product with the reciprocal), less theta_s with voltage sources. rc0u is
+// 1/c0 when every junction has the same (read once).
 struct JuncSource {
   const GAS double* v;
   const GAS double* ths;
   bool scaled;
+  double rc0u;
 };
 
+// MODE 0: u itself; 1: x / c0 with one c0; 2: x / c0 per junction; 3: x / c0
+// per junction less theta_s
+template <int MODE>
 __device__ __forceinline__ double junc_value(const DevProblem& P, const JuncSource& s, int32_t j) {
-  if (!s.scaled) return s.v[j];
-  double u = s.v[j] * rc0_at(P, j);
-  if (s.ths) u = u - s.ths[j];
-  return u;
+  if constexpr (MODE == 0) return s.v[j];
+  if constexpr (MODE == 1) return s.v[j] * s.rc0u;
+  if constexpr (MODE == 2) return s.v[j] * rc0_at(P, j);
+  return s.v[j] * rc0_at(P, j) - s.ths[j];
 }
 
-template <int BS, class Z>
-__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f, const JuncSource& us,
-                                           const Z& zs, int64_t w, int64_t t) {
+template <int BS, int MODE, class Z>
+__device__ __forceinline__ void face_phase_impl(const DevProblem& P, const jja_source* f, const JuncSource& us,
+                                                const Z& zs, int64_t w, int64_t t) {
   const int tid = zs.vt;
   // flux of face q at this step is fv.p[q * fv.si] * fv.amp; an absent flux
   // reads an exact zero, and acc - 2 pi * 0 leaves acc unchanged
@@ -1183,7 +1188,8 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source
 #pragma unroll
     for (int g = 0; g < kFaceGroup; ++g) {
 #pragma unroll
-      for (int i = 0; i < kFaceIter; ++i) uu[g][i] = r[g].cd[i] >= 0 ? junc_value(P, us, r[g].cd[i] >> 1) : 0.0;
+      for (int i = 0; i < kFaceIter; ++i)
+        uu[g][i] = r[g].cd[i] >= 0 ? junc_value<MODE>(P, us, r[g].cd[i] >> 1) : 0.0;
       fq[g] = r[g].fid >= 0 ? term_at(fv, r[g].fid) : 0.0;
     }
 #pragma unroll
@@ -1200,6 +1206,21 @@ __device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source
         acc = 0.0;
       }
     }
+  }
+}
+
+template <int BS, class Z>
+__device__ __forceinline__ void face_phase(const DevProblem& P, const jja_source* f, JuncSource us, const Z& zs,
+                                           int64_t w, int64_t t) {
+  if (!us.scaled) {
+    face_phase_impl<BS, 0>(P, f, us, zs, w, t);
+  } else if (us.ths) {
+    face_phase_impl<BS, 3>(P, f, us, zs, w, t);
+  } else if (P.coef_si == 0) {
+    us.rc0u = rc0_at(P, 0);
+    face_phase_impl<BS, 1>(P, f, us, zs, w, t);
+  } else {
+    face_phase_impl<BS, 2>(P, f, us, zs, w, t);
   }
 }
 
@@ -1482,7 +1503,7 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
 // (SW - 1), written by the junction pass before it.
 __device__ __forceinline__ JuncSource step_source(const GAS double* th, const GAS double* ths,
                                                  size_t slot_stride, int SW, int64_t t) {
-  return JuncSource{th + static_cast<size_t>(t % (SW - 1)) * slot_stride, ths, true};
+  return JuncSource{th + static_cast<size_t>(t % (SW - 1)) * slot_stride, ths, true, 0.0};
 }
 
 // n_steps steps of column w of one problem, by the threads the store names.
@@ -1734,7 +1755,7 @@ __device__ __forceinline__ void adjoint_steps(const DevProblem& P, const jja_adj
   zs.sync();
   for (int64_t k = k_hi; k >= k_lo; --k) {
     if (P.Nf > 0) {
-      face_phase<BS>(P, nullptr, JuncSource{C.u, nullptr, false}, zs, w, k);
+      face_phase<BS>(P, nullptr, JuncSource{C.u, nullptr, false, 0.0}, zs, w, k);
       zs.barrier();
       solve_face_system(P, zs);
     }
@@ -1763,8 +1784,8 @@ __device__ __forceinline__ void adjoint_steps_pair(const DevProblem& P, const jj
   za.sync();
   for (int64_t k = k_hi; k >= k_lo; --k) {
     if (P.Nf > 0) {
-      face_phase<BS>(P, nullptr, JuncSource{Ca.u, nullptr, false}, za, wa, k);
-      if (two) face_phase<BS>(P, nullptr, JuncSource{Cb.u, nullptr, false}, zb, wb, k);
+      face_phase<BS>(P, nullptr, JuncSource{Ca.u, nullptr, false, 0.0}, za, wa, k);
+      if (two) face_phase<BS>(P, nullptr, JuncSource{Cb.u, nullptr, false, 0.0}, zb, wb, k);
       za.barrier();
       if (two)
         solve_pair(P, za, zb);
