@@ -282,8 +282,10 @@ typedef struct jja_problem_run {
   double dt;
   double* theta;         /* [stencil_width-1][W][Nj]; theta at step m in slot m mod (sw-1) */
   double* theta_s;       /* [W][Nj] voltage-source phase; NULL when Vs.zero */
-  double* u;             /* unused, may be NULL: the face phase forms x/c0 - theta_s of the step
-                            being solved from its history slot */
+  double* u;             /* [W][Nj] scratch: x/c0 - theta_s of the step being solved, for a
+                            problem with the LDS face store (index_bytes 2); may be NULL when
+                            every problem has the global face store (its face phase forms
+                            x/c0 - theta_s from the history slot) */
   jja_source Is, Vs, f, T;
   /* noise amplitude sqrt(2 T Rv) when T is constant in time (sigma.base != NULL);
    * otherwise the kernel forms it from T */

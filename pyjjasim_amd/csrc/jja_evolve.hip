@@ -1149,10 +1149,12 @@ __device__ __forceinline__ void face_rec_load(const DevProblem& P, int32_t k, bo
   r.fid = rowed ? gbl(P.face_fid)[m.rowbase + tid] : -1;
 }
 
-// The junction values u_j the face phase sums: u itself (the adjoint), or
-// formed from the step's x as the junction pass would have: x_j / c0 (a
-// product with the reciprocal), less theta_s with voltage sources. rc0u is
-// 1/c0 when every junction has the same (read once).
+// The junction values u_j the face phase sums: u itself (an LDS store's
+// evolution, whose junction pass writes u, and the adjoint), or, for a global
+// store, formed from the step's x as the junction pass would have: x_j / c0 (a
+// product with the reciprocal), less theta_s with voltage sources; one write
+// stream fewer per junction. rc0u is 1/c0 when every junction has the same
+// (read once).
 struct JuncSource {
   const GAS double* v;
   const GAS double* ths;
@@ -1230,6 +1232,7 @@ struct JunctionStep {
   const GAS double* th_k[4];   // theta_{tx-1-k}, k = 0..S-1 (k = 0 unused with UPDATE)
   GAS double* th_x;            // slot of step tx: receives x_tx
   GAS double* ths;             // voltage-source phase (NULL without Vs)
+  GAS double* u;               // LDS store: receives u = x_tx/c0 - theta_s (global store: NULL)
   GAS double* th_out;          // stored theta_t row or NULL
   GAS double* I_out;           // stored current row or NULL
   TermView Is_t, Vs_t;         // step t (update)
@@ -1248,8 +1251,9 @@ struct JunctionStep {
 // One junction, all options: with UPDATE finish step t (time_evolution.py:
 // 560-580): y = A^T J from the face vector, theta_t = (y - x)/c0, stored
 // outputs, theta_s += Vs dt; with PREP form x of step tx (time_evolution.py:
-// 533-558) and write x to the history slot (the next face phase forms
-// u = x/c0 - theta_s from it). All loads are issued before any arithmetic.
+// 533-558) and write x to the history slot, and for an LDS store u = x/c0 -
+// theta_s to the scratch (a global store's face phase forms u from x). All
+// loads are issued before any arithmetic.
 template <int SW, bool UPDATE, bool PREP, class Z>
 __device__ __forceinline__ void junction_one(const DevProblem& P, const JunctionStep& J,
                                              const Z& zs, int32_t j, double zj) {
@@ -1302,6 +1306,11 @@ __device__ __forceinline__ void junction_one(const DevProblem& P, const Junction
     if (!J.t_zero) eta = (J.from_T ? sqrt(2.0 * sg * rv) : sg) * zj;
     const double x = J.is_zero ? eta + X : (eta - isx) + X;
     J.th_x[j] = x;
+    if constexpr (!Z::kGlobal) {
+      double uj = x * rc0;
+      if (J.ths) uj = uj - thsv;
+      J.u[j] = uj;
+    }
   }
 }
 
@@ -1408,6 +1417,7 @@ __device__ __forceinline__ void jfast_compute(const DevProblem& P, const Junctio
   if (!J.t_zero) eta = L.sg * z;
   const double x = J.is_zero ? eta + X : (eta - L.isx) + X;
   J.th_x[j] = x;
+  if constexpr (!Z::kGlobal) J.u[j] = x * L.rc0;
 }
 
 template <int BS, class Z>
@@ -1456,7 +1466,7 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
 template <int BS, int SW, class Z>
 __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_problem_run& R,
                                               GAS double* __restrict__ th, GAS double* __restrict__ ths,
-                                              const Z& zs,
+                                              GAS double* __restrict__ u, const Z& zs,
                                               int64_t w, size_t slot_stride, int64_t t, bool update,
                                               int64_t tx, bool prep, uint32_t noise_col) {
   constexpr int32_t S = SW - 1;
@@ -1472,6 +1482,7 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   for (int k = 0; k < 4; ++k) J.th_k[k] = th + (k < S ? slot(tx - 1 - k) : 0);
   J.th_x = th + slot(tx);
   J.ths = ths;
+  J.u = u;
   const size_t col = static_cast<size_t>(w) * P.Nj;
   const int32_t tho = (update && R.th_slot) ? gbl(R.th_slot)[t - R.slot_t0] : -1;
   const int32_t Io = (update && R.I_slot) ? gbl(R.I_slot)[t - R.slot_t0] : -1;
@@ -1513,19 +1524,21 @@ __device__ __forceinline__ void evolve_steps(const DevProblem& P, const jja_prob
   const size_t slot_stride = static_cast<size_t>(R.W) * P.Nj;
   GAS double* __restrict__ th = gblw(R.theta) + static_cast<size_t>(w) * P.Nj;
   GAS double* __restrict__ ths = R.theta_s ? gblw(R.theta_s) + static_cast<size_t>(w) * P.Nj : nullptr;
+  GAS double* __restrict__ u = Z::kGlobal ? nullptr : gblw(R.u) + static_cast<size_t>(w) * P.Nj;
   const uint32_t noise_col = static_cast<uint32_t>(R.column_offset + w);
 
-  junction_pass<BS, SW>(P, R, th, ths, zs, w, slot_stride, step0, false, step0, true, noise_col);
+  junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, step0, false, step0, true, noise_col);
   zs.sync();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
     if (P.Nf > 0) {  // a circuit without faces has nothing to solve (K is 0 x 0)
-      face_phase<BS>(P, &R.f, step_source(th, ths, slot_stride, SW, t), zs, w, t);
+      face_phase<BS>(P, &R.f, Z::kGlobal ? step_source(th, ths, slot_stride, SW, t) : JuncSource{u, nullptr, false, 0.0},
+                     zs, w, t);
       zs.barrier();
       solve_face_system(P, zs);
     }
     // finish step t and prepare step t+1 in one pass
-    junction_pass<BS, SW>(P, R, th, ths, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
+    junction_pass<BS, SW>(P, R, th, ths, u, zs, w, slot_stride, t, true, t + 1, n + 1 < n_steps, noise_col);
     zs.sync();
   }
 }
@@ -1547,8 +1560,8 @@ __device__ __forceinline__ void evolve_steps_pair(const DevProblem& P, const jja
   GAS double* thsb = R.theta_s ? gblw(R.theta_s) + cb : nullptr;
   const uint32_t na = static_cast<uint32_t>(R.column_offset + wa), nb = static_cast<uint32_t>(R.column_offset + wb);
 
-  junction_pass<BS, SW>(P, R, tha, thsa, za, wa, slot_stride, step0, false, step0, true, na);
-  if (two) junction_pass<BS, SW>(P, R, thb, thsb, zb, wb, slot_stride, step0, false, step0, true, nb);
+  junction_pass<BS, SW>(P, R, tha, thsa, nullptr, za, wa, slot_stride, step0, false, step0, true, na);
+  if (two) junction_pass<BS, SW>(P, R, thb, thsb, nullptr, zb, wb, slot_stride, step0, false, step0, true, nb);
   za.sync();
   for (int32_t n = 0; n < n_steps; ++n) {
     const int64_t t = step0 + n;
@@ -1562,8 +1575,8 @@ __device__ __forceinline__ void evolve_steps_pair(const DevProblem& P, const jja
         solve(P, za);
     }
     const bool prep = n + 1 < n_steps;
-    junction_pass<BS, SW>(P, R, tha, thsa, za, wa, slot_stride, t, true, t + 1, prep, na);
-    if (two) junction_pass<BS, SW>(P, R, thb, thsb, zb, wb, slot_stride, t, true, t + 1, prep, nb);
+    junction_pass<BS, SW>(P, R, tha, thsa, nullptr, za, wa, slot_stride, t, true, t + 1, prep, na);
+    if (two) junction_pass<BS, SW>(P, R, thb, thsb, nullptr, zb, wb, slot_stride, t, true, t + 1, prep, nb);
     za.sync();
   }
 }
@@ -2720,6 +2733,8 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
     const bool global = plan->zbuf_doubles[i] > 0;
     if (global && !r.zbuf)
       return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " needs a global face store (zbuf)");
+    if (!global && !r.u)
+      return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " (LDS face store) needs u");
     if (!global && plan->lds_bytes[i] > plan->lds_limit)
       return fail(JJA_EUNSUPPORTED, "jja_evolve: face vector of problem " + std::to_string(i) +
                                         " needs " + std::to_string(plan->lds_bytes[i]) +

@@ -235,6 +235,9 @@ class Engine:
             elif not np.ptp(sig, axis=1).any():
                 sig = sig[:, :1]
             self.sigma = DeviceSource(torch, sig[:, :, None], Nj, W, Nt, False)
+        # u = x/c0 - theta_s between the junction pass and the face phase: an
+        # LDS face store's scratch (a global store's face phase forms it from x)
+        self.u = None if self.host.global_store else torch.zeros((W, Nj), dtype=f64, device=dev)
         self.th_mask, self.I_mask = th_mask, I_mask
         self.n_th, self.n_I = int(th_mask.sum()), int(I_mask.sum())
         # stored steps: all resident on the device, or (when they exceed the
@@ -267,6 +270,7 @@ class Engine:
         run.dt = dt
         run.theta = self.theta.data_ptr()
         run.theta_s = self.theta_s.data_ptr() if self.theta_s is not None else None
+        run.u = self.u.data_ptr() if self.u is not None else None
         if self.sigma is not None:
             run.sigma = self.sigma.src
         run.th_slot = self.th_slot.data_ptr()
