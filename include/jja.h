@@ -283,9 +283,9 @@ typedef struct jja_problem_run {
   double* theta;         /* [stencil_width-1][W][Nj]; theta at step m in slot m mod (sw-1) */
   double* theta_s;       /* [W][Nj] voltage-source phase; NULL when Vs.zero */
   double* u;             /* [W][Nj] scratch: x/c0 - theta_s of the step being solved, for a
-                            problem with the LDS face store (index_bytes 2); may be NULL when
-                            every problem has the global face store (its face phase forms
-                            x/c0 - theta_s from the history slot) */
+                            problem with the LDS face store (index_bytes 2) in a plan without
+                            split columns; may be NULL otherwise (a global store's face phase
+                            forms x/c0 - theta_s from the history slot) */
   jja_source Is, Vs, f, T;
   /* noise amplitude sqrt(2 T Rv) when T is constant in time (sigma.base != NULL);
    * otherwise the kernel forms it from T */

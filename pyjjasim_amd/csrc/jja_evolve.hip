@@ -2733,7 +2733,9 @@ int jja_evolve(jja_plan* p, const jja_problem_run* runs, int32_t n_steps, int64_
     const bool global = plan->zbuf_doubles[i] > 0;
     if (global && !r.zbuf)
       return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " needs a global face store (zbuf)");
-    if (!global && !r.u)
+    // an LDS face store's junction pass writes u for its face phase (a split
+    // plan's columns all use the global store, faceless ones included)
+    if (!global && plan->parts == 1 && !r.u)
       return fail(JJA_EINVAL, "jja_evolve: problem " + std::to_string(i) + " (LDS face store) needs u");
     if (!global && plan->lds_bytes[i] > plan->lds_limit)
       return fail(JJA_EUNSUPPORTED, "jja_evolve: face vector of problem " + std::to_string(i) +
