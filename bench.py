@@ -402,6 +402,11 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000, sol
     n_exec = int(evo.eng.host.n_solve_exec)
     used = "lattice" if evo.eng.host.uses_lattice else "cholesky"
     parts = int(evo.eng.host.parts)
+    # forward pass against the HBM roofline: the step's algorithmic bytes
+    # (W (24 Nj + 48 Nf) + solve tables or factor + 40 Nj + the dense Is)
+    # over the measured time per step (the launch includes checkpoint copies)
+    bps = int(evo.eng.bytes_per_step())
+    achieved = bps / fwd / 1e9
     evo.close()
     return {"metric": "junction-steps/s (large array forward, FP64)", "value": Nj * W / fwd,
             "unit": "junction-steps/s", "steps": steps, "dtype": "f64",
@@ -410,6 +415,8 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000, sol
                        "face_store": face_store, "solver": used, "parts": parts,
                        "solve_records_per_column_step": n_exec},
             "forward_ms_per_step": fwd * 1e3, "backward_ms_per_step": bwd * 1e3,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_step": bps, "pass": "forward"},
             "grad_over_fwd": bwd / fwd,
             "full_horizon_estimate_s": {"steps": full_steps, "forward": fwd * full_steps,
                                         "backward": bwd * full_steps},
