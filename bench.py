@@ -403,9 +403,10 @@ def large_bench(pa, torch, steps, checkpoint, n=512, W=16, full_steps=50000, sol
     used = "lattice" if evo.eng.host.uses_lattice else "cholesky"
     parts = int(evo.eng.host.parts)
     # forward pass against the HBM roofline: the step's algorithmic bytes
-    # (W (24 Nj + 48 Nf) + solve tables or factor + 40 Nj + the dense Is)
-    # over the measured time per step (the launch includes checkpoint copies)
-    bps = int(evo.eng.bytes_per_step())
+    # (W (24 Nj + 48 Nf) + solve tables or factor + 40 Nj, plus the dense bias
+    # Is, 8 Nj W, passed as a tensor and read every step) over the measured
+    # time per step (the launches include the checkpoint copies)
+    bps = int(evo.eng.bytes_per_step()) + 8 * Nj * W
     achieved = bps / fwd / 1e9
     evo.close()
     return {"metric": "junction-steps/s (large array forward, FP64)", "value": Nj * W / fwd,
