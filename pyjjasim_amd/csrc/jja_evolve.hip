@@ -700,18 +700,13 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
     const int par = mt < nte ? 0 : 1;
     const int32_t m0 = 32 * (par ? mt - nte : mt);
     const int32_t r0 = 32 * rt;
-    // operand addresses of this lane (clamped into the lattice): a global
-    // store's folded rows in blocks of four i, an LDS store's by row
+    // operand addresses of this lane (clamped into the lattice)
     const GAS double* fa[2];
-    uint32_t la[2];
     const GAS double* sb[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int32_t r = min(r0 + 16 * b + lr, R - 1);
-      if constexpr (Z::kGlobal)
-        fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * EF * R + 4 * r + lk;
-      else
-        la[b] = static_cast<uint32_t>(P.lat_t0 + static_cast<int64_t>(par) * R * ES + static_cast<int64_t>(r) * ES);
+      fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * EF * R + 4 * r + lk;
       const int32_t j = min(2 * (m0 + 16 * b + lr) + par, C - 1);
       sb[b] = S + j;
     }
@@ -729,17 +724,11 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
         const int32_t ic = min(i4 + lk, ES - 1);
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          if constexpr (Z::kGlobal)
-            av[u][b] = fa[b][static_cast<int64_t>(blk) * R * 4];
-          else
-            av[u][b] = zs.gather(la[b] + static_cast<uint32_t>(ic));
+          av[u][b] = fa[b][static_cast<int64_t>(blk) * R * 4];
           bv[u][b] = sb[b][static_cast<int64_t>(ic) * CP];
         }
-        // terms past the folded rows are zero (a global store holds zeros up to
-        // EF; an LDS store's rows end at ES)
-        const bool live = Z::kGlobal ? i4 < EF : i4 + lk < ES;
 #pragma unroll
-        for (int b = 0; b < 2; ++b) av[u][b] = live ? av[u][b] : 0.0;
+        for (int b = 0; b < 2; ++b) av[u][b] = i4 < EF ? av[u][b] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -765,7 +754,10 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
 
 template <class Z>
 __device__ __forceinline__ void lattice_transform(const DevProblem& P, const Z& zs) {
-  lattice_transform_mfma(P, zs);
+  if constexpr (Z::kGlobal)
+    lattice_transform_mfma(P, zs);
+  else
+    lattice_transform_lds(P, zs);
 }
 
 // Mode k's tridiagonal solve along r, in place in the face vector. Both
