@@ -1,7 +1,10 @@
 """Many time-evolution problems in one launch per chunk of steps.
 
 ``TimeEvolutionBatch(problems).compute()`` returns one TimeEvolutionResult per
-problem, each identical to ``problem.compute()``: the problems (any circuits,
+problem, each identical to ``problem.compute()`` run with the layout the batch
+gave it (``layouts``: workgroup size, parts per column, face store; the factor
+gives the same bits in any layout, while the lattice solve's rounding depends
+on the face store and the part count): the problems (any circuits,
 sizes and column counts; one stencil width and step count) share one libjja
 plan, and every launch advances the columns of all of them. By default a
 column is one workgroup: circuits whose face vector fits in LDS keep it there,
@@ -47,9 +50,15 @@ _THREAD_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "B
 
 
 def _split_cost(h):
-    """Work of one column step of a problem in a split launch (which solves
-    with the factor, whatever the problem's own plan uses): the factor entries
-    both passes stream, plus the junction work."""
+    """Work of one column step of a problem in a split launch, in factor-entry
+    units: the factor entries both passes stream, or for a lattice plan its
+    transforms' multiply-adds (on the matrix cores, about twelve per entry's
+    time) and passes over the faces; plus the junction work. (SquareArray(512,
+    512): lattice 11.9M, factor 27.4M units; measured 0.336 and 0.966 ms per
+    step.)"""
+    if h.uses_lattice:
+        lat = h.lattice
+        return float(lat.R * lat.C * lat.ES / 12.0 + 8 * h.Nf + 8 * h.Nj)
     sp = h.solve_plan
     return float(2 * (len(sp.lo_val) + len(sp.inv_cm)) + 8 * h.Nj)
 
@@ -140,6 +149,7 @@ class TimeEvolutionBatch:
         self.max_parts = int(max_parts)
         self.tuning = None
         self.solvers = None
+        self.layouts = None
 
     # --------------------------------------------------------------- tuning
     def _engine(self, hosts, masks):
@@ -166,7 +176,8 @@ class TimeEvolutionBatch:
     def _hosts_for(self, base, bs, parts):
         """Plans laid out for workgroups of bs threads and the given parts: an
         int for every problem, or "balanced" (balanced_parts). With any split
-        column, every problem uses the global face store."""
+        column, every problem uses the global face store (lattice plans keep the
+        lattice solve)."""
         if parts == "balanced":
             cols = [p.problem_count for p in self.problems]
             per = balanced_parts(base, cols, device_compute_units(), self.max_parts)
@@ -174,9 +185,8 @@ class TimeEvolutionBatch:
             per = [int(parts)] * len(base)
         split = max(per) > 1
         if split:
-            # a plan with split columns solves every problem with the factor
-            return [h if (h.block_size, h.parts) == (bs, q) and h.global_store and not h.uses_lattice
-                    else h.relayout(bs, q, "global", solver="cholesky") for h, q in zip(base, per)]
+            return [h if (h.block_size, h.parts) == (bs, q) and h.global_store
+                    else h.relayout(bs, q, "global") for h, q in zip(base, per)]
         return [h if (h.block_size, h.parts) == (bs, q) else h.relayout(bs, q) for h, q in zip(base, per)]
 
     def _tune(self, masks):
@@ -253,9 +263,10 @@ class TimeEvolutionBatch:
             if self.parts == "balanced" or self.parts > 1:
                 hosts = self._hosts_for(hosts, self.block_size, self.parts)
         np.random.set_state(rng_state)
-        # how each problem's face system is solved in this batch (a plan with
-        # split columns solves every problem with the factor)
+        # how each problem's face system is solved in this batch, and its
+        # layout (workgroup size, parts per column, face store)
         self.solvers = ["lattice" if h.uses_lattice else "cholesky" for h in hosts]
+        self.layouts = [(h.block_size, h.parts, "global" if h.global_store else "lds") for h in hosts]
         eng = BatchEngine(self.problems, masks, block_size=self.block_size, hosts=hosts)
         try:
             if self.autotune and not self.steps_per_launch:

@@ -1,5 +1,6 @@
 """Batched problems (TimeEvolutionBatch) and separable time-dependent inputs on
-the GPU: a batch returns exactly what each problem's own compute() returns."""
+the GPU: a batch returns exactly what each problem's own compute() returns (run
+with the batch's solver, and for the lattice solve the batch's layout)."""
 import numpy as np
 import pytest
 
@@ -29,16 +30,19 @@ def _problems():
     return [pa_, pb, pc]
 
 
-def _own(p, solver):
-    """The problem's own compute() with the face-system solver the batch used
-    for it (a batch with split columns solves every problem with the factor;
-    the lattice solve and the factor agree to rounding, not bit for bit)."""
-    saved = p.solver
-    p.solver = solver
+def _own(p, tb, i):
+    """Problem i's own compute() with the face-system solver the batch used for
+    it, and for the lattice solve also its layout (the face store and the part
+    count set the lattice solve's rounding; the factor's bits do not depend on
+    the layout)."""
+    saved = (p.solver, p.block_size, p.parts, p.face_store)
+    p.solver = tb.solvers[i]
+    if tb.solvers[i] == "lattice":
+        p.block_size, p.parts, p.face_store = tb.layouts[i]
     try:
         return p.compute()
     finally:
-        p.solver = saved
+        p.solver, p.block_size, p.parts, p.face_store = saved
 
 
 def _same(r1, r2):
@@ -53,8 +57,8 @@ def test_gpu_batch_matches_individual_runs():
     tb = pa.TimeEvolutionBatch(probs, block_size=256)
     batch = tb.compute()
     assert tb.solvers == ["lattice", "cholesky", "cholesky"]
-    for p, rb, sv in zip(probs, batch, tb.solvers):
-        _same(rb, _own(p, sv))
+    for i, (p, rb) in enumerate(zip(probs, batch)):
+        _same(rb, _own(p, tb, i))
 
 
 def test_gpu_batch_autotune():
@@ -65,8 +69,8 @@ def test_gpu_batch_autotune():
     assert (tb.tuning["block_size"], tb.tuning["parts"]) in [(128, 1), (256, 1)] + \
         [(1024, q) for q in (2, 4, 8, 16)]
     assert 100 <= tb.tuning["steps_per_launch"] <= 4000
-    for p, rb, sv in zip(probs, batch, tb.solvers):
-        _same(rb, _own(p, sv))
+    for i, (p, rb) in enumerate(zip(probs, batch)):
+        _same(rb, _own(p, tb, i))
 
 
 def test_gpu_separable_source_matches_oracle():
@@ -96,9 +100,10 @@ def test_gpu_batch_split_columns_match_individual_runs():
     probs = _problems()
     tb = pa.TimeEvolutionBatch(probs, block_size=256, parts=4)
     batch = tb.compute()
-    assert tb.solvers == ["cholesky"] * 3
-    for p, rb, sv in zip(probs, batch, tb.solvers):
-        _same(rb, _own(p, sv))
+    assert tb.solvers == ["lattice", "cholesky", "cholesky"]
+    assert all(lay[1] == 4 and lay[2] == "global" for lay in tb.layouts)
+    for i, (p, rb) in enumerate(zip(probs, batch)):
+        _same(rb, _own(p, tb, i))
 
 
 def test_gpu_batch_balanced_parts_match_individual_runs():
@@ -108,8 +113,8 @@ def test_gpu_batch_balanced_parts_match_individual_runs():
     probs = _problems()
     tb = pa.TimeEvolutionBatch(probs, block_size=256, parts="balanced", max_parts=8)
     batch = tb.compute()
-    for p, rb, sv in zip(probs, batch, tb.solvers):
-        _same(rb, _own(p, sv))
+    for i, (p, rb) in enumerate(zip(probs, batch)):
+        _same(rb, _own(p, tb, i))
 
 
 def test_gpu_batch_autotune_short_horizon():
@@ -124,8 +129,8 @@ def test_gpu_batch_autotune_short_horizon():
         probs.append(pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=12, external_flux=f,
                                              current_sources=Is, temperature=0.01, seed=4))
     tb = pa.TimeEvolutionBatch(probs, autotune=True, block_sizes=(64, 128), tune_steps=40)
-    for r, p, sv in zip(tb.compute(), probs, tb.solvers):
-        _same(r, _own(p, sv))
+    for i, (r, p) in enumerate(zip(tb.compute(), probs)):
+        _same(r, _own(p, tb, i))
     eng = tb.make_engine()
     try:
         eng.advance(10)
@@ -152,4 +157,4 @@ def test_gpu_batch_autotune_keeps_reference_noise_stream():
     tb = pa.TimeEvolutionBatch([problem()], autotune=True, block_sizes=(64, 128), tune_steps=10)
     batch = tb.compute()
     np.random.seed(1234)
-    _same(batch[0], _own(problem(), tb.solvers[0]))
+    _same(batch[0], _own(problem(), tb, 0))

@@ -182,15 +182,16 @@ def test_split_lattice_emulation_matches_the_factor(parts):
     np.testing.assert_allclose(plan.emulate_face_and_solve(u), want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
-def test_split_balancing_cost_does_not_depend_on_the_solver():
-    """A batch with split columns solves every problem with the factor, so the
-    part balancing prices a lattice problem by its factor, not by its lattice
-    plan (which streams no factor entries)."""
+def test_split_balancing_prices_lattice_plans_by_their_transforms():
+    """A split launch keeps the lattice solve, so the part balancing prices a
+    lattice problem by its transforms and face passes (less than its factor
+    for a large lattice) and gives it no more parts than the factor would."""
     from pyjjasim_amd.batch import _split_cost, balanced_parts
-    c = pa.SquareArray(40, 40)
+    c = pa.SquareArray(120, 120)
     lat = CircuitPlan(c, 0.05, block_size=1024)
     chol = CircuitPlan(c, 0.05, block_size=1024, solver="cholesky")
     assert lat.uses_lattice and not chol.uses_lattice
-    assert _split_cost(lat) == _split_cost(chol)
-    hc = CircuitPlan(pa.HoneycombArray(30, 30), 0.05, block_size=1024)
-    assert balanced_parts([lat, hc], [1, 1], 256) == balanced_parts([chol, hc], [1, 1], 256)
+    assert 8 * lat.Nj < _split_cost(lat) < _split_cost(chol)
+    hc = CircuitPlan(pa.HoneycombArray(60, 60), 0.05, block_size=1024)
+    p_lat, p_chol = (balanced_parts([h, hc], [1, 1], 16) for h in (lat, chol))
+    assert p_lat[0] <= p_chol[0] and p_lat[1] >= p_chol[1]

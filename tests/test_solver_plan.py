@@ -264,7 +264,8 @@ def test_balanced_parts_allocation():
     from types import SimpleNamespace
     from pyjjasim_amd.batch import balanced_parts
     # split cost 2 * (off-block + block-inverse entries) of the factor
-    hosts = [SimpleNamespace(solve_plan=SimpleNamespace(lo_val=np.zeros(n // 2), inv_cm=np.zeros(0)), Nj=0)
+    hosts = [SimpleNamespace(solve_plan=SimpleNamespace(lo_val=np.zeros(n // 2), inv_cm=np.zeros(0)), Nj=0,
+                             uses_lattice=False)
              for n in (64000, 16000, 1000, 1000)]
     parts = balanced_parts(hosts, [1, 1, 1, 1], n_cu=16, max_parts=16)
     # 1,1,1,1 (4 workgroups) -> first 2, 4, 8 for the largest (cost 64000 / parts),
