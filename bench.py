@@ -328,6 +328,7 @@ def uneven_bench(pa, torch, steps, warmup, plain_steps, n_problems=64):
         eng.close()
     value = float(Njs.sum()) * steps / wall
     plain = None
+    plain_steps = min(plain_steps, Nt - 5)   # each plain run starts with 5 untimed steps
     if plain_steps > 0:
         per_step = 0.0
         for i, p in enumerate(probs):
