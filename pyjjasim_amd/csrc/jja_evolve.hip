@@ -184,6 +184,11 @@ __device__ __forceinline__ void normal_pair(uint64_t seed, uint32_t j, uint32_t 
                                             double* z0, double* z1) {
   uint32_t c0 = j >> 1, c1 = w, c2 = static_cast<uint32_t>(step), c3 = 0u;
   uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+  // an empty statement that the compiler must assume changes the key: the
+  // round keys are then formed here (two scalar adds per round) instead of
+  // being hoisted out of the junction loop as 20 live scalar registers, which
+  // spilled to vector lanes and cost 22 v_readlane per pair
+  __asm__ volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
