@@ -553,9 +553,11 @@ __device__ __forceinline__ void solve(const DevProblem& P, const Z& zs) {
 // (1965) 95-113). Per step: fold every row (S[C-1-i][j] = (-1)^j S[i][j]),
 // transform (out[r][j] = sum_{i<H} F_j[r][i] S[i][j], H = ceil(C/2)), one
 // tridiagonal solve per mode along r, then fold and transform again. Each
-// output is summed by one thread in ascending i with fused multiply-adds, and
-// each mode's recurrence runs in one thread, so results depend neither on the
-// block size nor on the face store.
+// transform output is summed in a fixed order (one thread in ascending i with
+// fused multiply-adds in LDS; one matrix-core chain in a global store) and each
+// block of a mode's recurrence runs in one thread with a blocking fixed by R, C
+// and the part count, so results do not depend on the block size or the launch
+// length; the two face stores agree to rounding.
 
 // Stores the lattice solve runs on: one workgroup's (LDS or global) and a
 // split column's (global, column barriers)
