@@ -1388,7 +1388,9 @@ struct JFast {
 // GATHER: read the face-store values now (global store: the codes came one
 // iteration ahead), else when the pair is computed (LDS store: the gathers
 // are short, and holding the codes ahead would cost registers)
-template <bool GATHER, class Z>
+// UNI: 1/c0, c1, c2 and sigma are one value each (uniform R and C, constant
+// T): read once before the loop (scalar registers), not per junction
+template <bool GATHER, bool UNI, class Z>
 __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionStep& J, const Z& zs, int32_t j,
                                            vint2 code, JFast& L) {
   L.code = code;
@@ -1396,14 +1398,16 @@ __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionSt
     L.za = code.x >= 0 ? zs.rd(code.x >> 1) : 0.0;
     L.zb = code.y >= 0 ? zs.rd(code.y >> 1) : 0.0;
   }
-  L.rc0 = rc0_at(P, j);
   L.xo = J.th_n[j];
   L.t2 = J.th_k[1][j];
-  L.c1 = coef_at(P, 1, j);
-  L.c2 = coef_at(P, 2, j);
   L.Ic = J.Ic[j];
   L.isx = term_at(J.Is_x, j);
-  L.sg = term_at(J.sig_x, j);
+  if constexpr (!UNI) {
+    L.rc0 = rc0_at(P, j);
+    L.c1 = coef_at(P, 1, j);
+    L.c2 = coef_at(P, 2, j);
+    L.sg = term_at(J.sig_x, j);
+  }
 }
 
 template <bool GATHER, class Z>
@@ -1427,9 +1431,16 @@ __device__ __forceinline__ void jfast_compute(const DevProblem& P, const Junctio
   if constexpr (!Z::kGlobal) J.u[j] = x * L.rc0;
 }
 
-template <int BS, class Z>
+template <int BS, bool UNI, class Z>
 __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const JunctionStep& J, const Z& zs) {
   const int32_t Nj = P.Nj;
+  double u_rc0 = 0.0, u_c1 = 0.0, u_c2 = 0.0, u_sg = 0.0;
+  if constexpr (UNI) {
+    u_rc0 = rc0_at(P, 0);
+    u_c1 = coef_at(P, 1, 0);
+    u_c2 = coef_at(P, 2, 0);
+    u_sg = term_at(J.sig_x, 0);
+  }
   const int32_t tid = J.vt, vbs = J.vbs;
   const int32_t last = Nj - 1;
   const bool gauss = !J.t_zero;
@@ -1450,8 +1461,14 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
       nB = codes[min(jB, last)];
     }
     JFast A, B;
-    jfast_load<G>(P, J, zs, jA, nA, A);
-    jfast_load<G>(P, J, zs, min(jB, last), nB, B);
+    jfast_load<G, UNI>(P, J, zs, jA, nA, A);
+    jfast_load<G, UNI>(P, J, zs, min(jB, last), nB, B);
+    if constexpr (UNI) {
+      A.rc0 = B.rc0 = u_rc0;
+      A.c1 = B.c1 = u_c1;
+      A.c2 = B.c2 = u_c2;
+      A.sg = B.sg = u_sg;
+    }
     if constexpr (G) {
       const int32_t qn = q + vbs;
       if (2 * qn < Nj) {
@@ -1511,7 +1528,8 @@ __device__ __forceinline__ void junction_pass(const DevProblem& P, const jja_pro
   J.noise_col = noise_col;
   J.tx = static_cast<uint64_t>(tx);
   const bool fast = SW == 3 && update && prep && !ths && !J.th_out && !J.I_out && !J.noise && !J.from_T;
-  if (fast) junction_loop_fast<BS, Z>(P, J, zs);
+  if (fast && P.coef_si == 0 && J.sig_x.si == 0) junction_loop_fast<BS, true, Z>(P, J, zs);
+  else if (fast) junction_loop_fast<BS, false, Z>(P, J, zs);
   else if (update && prep) junction_loop<BS, SW, true, true, Z>(P, J, zs);
   else if (update) junction_loop<BS, SW, true, false, Z>(P, J, zs);
   else if (prep) junction_loop<BS, SW, false, true, Z>(P, J, zs);
