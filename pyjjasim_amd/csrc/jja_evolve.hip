@@ -1375,29 +1375,21 @@ __device__ __forceinline__ void junction_loop(const DevProblem& P, const Junctio
 // ------------------------------------------------------------ common step
 // The usual step (stencil 3, no voltage sources, nothing stored at step t,
 // noise amplitude given per junction, Philox or no noise): the same
-// arithmetic as junction_one, each lane taking a junction pair. A pair's face
-// codes are loaded one iteration ahead, so its face-store gathers go out with
-// its other loads (one memory round trip per pair, not two), and the pair's
-// Philox draw is computed while they are in flight.
+// arithmetic as junction_one, each lane taking a junction pair, with the loads
+// of both junctions issued first and the pair's Philox draw computed while
+// they are in flight. (Loading the next pair's face codes an iteration ahead,
+// so that the face-store gathers could go out with the other loads, made the
+// split kernel spill and was 3.6% slower on SquareArray(512,512).)
 struct JFast {
   vint2 code;
-  double za, zb;  // face-store values at code.x and code.y (0 where absent)
   double rc0, xo, t2, c1, c2, Ic, isx, sg;
 };
 
-// GATHER: read the face-store values now (global store: the codes came one
-// iteration ahead), else when the pair is computed (LDS store: the gathers
-// are short, and holding the codes ahead would cost registers)
 // UNI: 1/c0, c1, c2 and sigma are one value each (uniform R and C, constant
 // T): read once before the loop (scalar registers), not per junction
-template <bool GATHER, bool UNI, class Z>
-__device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionStep& J, const Z& zs, int32_t j,
-                                           vint2 code, JFast& L) {
-  L.code = code;
-  if constexpr (GATHER) {
-    L.za = code.x >= 0 ? zs.rd(code.x >> 1) : 0.0;
-    L.zb = code.y >= 0 ? zs.rd(code.y >> 1) : 0.0;
-  }
+template <bool UNI>
+__device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionStep& J, int32_t j, JFast& L) {
+  L.code = gbl(reinterpret_cast<const vint2*>(P.junc_code))[j];
   L.xo = J.th_n[j];
   L.t2 = J.th_k[1][j];
   L.Ic = J.Ic[j];
@@ -1410,17 +1402,12 @@ __device__ __forceinline__ void jfast_load(const DevProblem& P, const JunctionSt
   }
 }
 
-template <bool GATHER, class Z>
+template <class Z>
 __device__ __forceinline__ void jfast_compute(const DevProblem& P, const JunctionStep& J, const Z& zs, int32_t j,
                                               const JFast& L, double z) {
-  double za = L.za, zb = L.zb;
-  if constexpr (!GATHER) {
-    za = L.code.x >= 0 ? zs.rd(L.code.x >> 1) : 0.0;
-    zb = L.code.y >= 0 ? zs.rd(L.code.y >> 1) : 0.0;
-  }
   double y = 0.0;
-  if (L.code.x >= 0) y = (L.code.x & 1) ? y - za : y + za;
-  if (L.code.y >= 0) y = (L.code.y & 1) ? y - zb : y + zb;
+  if (L.code.x >= 0) y = (L.code.x & 1) ? y - zs.rd(L.code.x >> 1) : y + zs.rd(L.code.x >> 1);
+  if (L.code.y >= 0) y = (L.code.y & 1) ? y - zs.rd(L.code.y >> 1) : y + zs.rd(L.code.y >> 1);
   const double t1 = (y - L.xo) * L.rc0;
   J.th_n[j] = t1;
   const double X = L.Ic * cpr_sum(P, 2.0 * t1 - L.t2) + L.c1 * t1 + L.c2 * L.t2;
@@ -1444,44 +1431,26 @@ __device__ __forceinline__ void junction_loop_fast(const DevProblem& P, const Ju
   const int32_t tid = J.vt, vbs = J.vbs;
   const int32_t last = Nj - 1;
   const bool gauss = !J.t_zero;
-  const GAS vint2* __restrict__ codes = gbl(reinterpret_cast<const vint2*>(P.junc_code));
   // a lane takes the junction pair (2q, 2q+1): its own Philox draw gives both
   // deviates (the pair's counter, as in the general loop); a pair past the
   // end (odd Nj) reads the last junction and does not use it
-  constexpr bool G = Z::kGlobal;
-  vint2 nA{-1, -1}, nB{-1, -1};
-  if (G && 2 * tid < Nj) {
-    nA = codes[2 * tid];
-    nB = codes[min(2 * tid + 1, last)];
-  }
   for (int32_t q = tid; 2 * q < Nj; q += vbs) {
     const int32_t jA = 2 * q, jB = jA + 1;
-    if constexpr (!G) {
-      nA = codes[jA];
-      nB = codes[min(jB, last)];
-    }
     JFast A, B;
-    jfast_load<G, UNI>(P, J, zs, jA, nA, A);
-    jfast_load<G, UNI>(P, J, zs, min(jB, last), nB, B);
+    jfast_load<UNI>(P, J, jA, A);
+    jfast_load<UNI>(P, J, min(jB, last), B);
     if constexpr (UNI) {
       A.rc0 = B.rc0 = u_rc0;
       A.c1 = B.c1 = u_c1;
       A.c2 = B.c2 = u_c2;
       A.sg = B.sg = u_sg;
     }
-    if constexpr (G) {
-      const int32_t qn = q + vbs;
-      if (2 * qn < Nj) {
-        nA = codes[2 * qn];
-        nB = codes[min(2 * qn + 1, last)];
-      }
-    }
     __builtin_amdgcn_sched_barrier(0);
     // the Philox draw runs while the loads are in flight
     double zA = 0.0, zB = 0.0;
     if (gauss) normal_pair(J.seed, static_cast<uint32_t>(jA), J.noise_col, J.tx, &zA, &zB);
-    jfast_compute<G>(P, J, zs, jA, A, zA);
-    if (jB < Nj) jfast_compute<G>(P, J, zs, jB, B, zB);
+    jfast_compute(P, J, zs, jA, A, zA);
+    if (jB < Nj) jfast_compute(P, J, zs, jB, B, zB);
   }
 }
 
