@@ -810,19 +810,36 @@ __device__ __forceinline__ double lattice_carry(const Z& zs, int32_t C, int32_t 
 }
 
 // The tridiagonal solves of a split column with each part's modes in its own
-// LDS: part p takes modes [p*MP, p*MP + MP) (MP = lat_mp) with all R rows
-// (Y[r][kk], R*MP doubles), cuts each mode into BP = lat_bp blocks and runs
-// the same blocked recurrences as lattice_tridiag between workgroup barriers;
-// the block ends (forward and backward, value and product, 4*BP*MP doubles)
-// stay in LDS, so no column barrier is needed inside, and the solved rows are
-// written back to the face vector.
+// LDS, followed by the second fold. Part p takes the mode pairs (i, C-1-i) for
+// i in [p*MH, p*MH + MH) (MH = lat_mp / 2; the middle mode of an odd C has no
+// partner) with all R rows (Y[r][slot]: the lower modes in slots 0..nh-1, their
+// partners after them; R*lat_mp doubles), cuts each mode into BP = lat_bp
+// blocks and runs the same blocked recurrences as lattice_tridiag between
+// workgroup barriers; the block ends (forward and backward, value and product,
+// 4*BP*lat_mp doubles) stay in LDS, so no column barrier is needed inside.
+// Holding both modes of a pair, the part then writes their folded rows
+// (E = x_i + x_{C-1-i}, O = x_i - x_{C-1-i}, the same sums as lattice_fold) in
+// the global store's blocked layout, in place of a separate fold pass.
 __device__ __forceinline__ void lattice_tridiag_parts(const DevProblem& P, const SplitStore& zs) {
   constexpr int TB = kTriRows;
   const int32_t R = P.lat_rows, C = P.lat_cols, CP = P.lat_cp, MP = P.lat_mp, B = P.lat_bp;
+  const int32_t H = (C + 1) >> 1, EF = P.lat_ef, MH = MP >> 1;
   const int32_t nt = static_cast<int32_t>(blockDim.x), tid = static_cast<int32_t>(threadIdx.x);
   const int32_t part = __builtin_amdgcn_readfirstlane(zs.vt / nt);
-  const int32_t k0 = part * MP, nm = min(MP, C - k0);
-  if (nm <= 0) return;  // a part past the last mode (the whole workgroup)
+  const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * EF;
+  // folded-row padding (H <= i < EF) is zero; part 0 writes it
+  if (part == 0)
+    for (int32_t e = tid; e < R * (EF - H); e += nt) {
+      const int32_t i = H + e / R, r = e - (e / R) * R;
+      const int64_t at = (static_cast<int64_t>(i >> 2) * R + r) * 4 + (i & 3);
+      zs.wr(e0 + at, 0.0);
+      zs.wr(o0 + at, 0.0);
+    }
+  const int32_t i0 = part * MH, nh = min(MH, H - i0);
+  if (nh <= 0) return;  // a part past the last pair (the whole workgroup)
+  const int32_t nu = (2 * (i0 + nh - 1) + 1 == C) ? nh - 1 : nh;  // partners (the middle has none)
+  const int32_t nm = nh + nu;
+  auto mode = [&](int32_t kk) { return kk < nh ? i0 + kk : C - 1 - (i0 + kk - nh); };
   const GAS double* __restrict__ inv = gbl(P.lat_inv);
   const double g = P.lat_g;
   const int32_t m = (R + B - 1) / B;
@@ -833,17 +850,18 @@ __device__ __forceinline__ void lattice_tridiag_parts(const DevProblem& P, const
   LAS double* bp = bv + B * MP;
   for (int32_t e = tid; e < R * nm; e += nt) {
     const int32_t r = e / nm, kk = e - r * nm;
-    Y[r * MP + kk] = zs.rd(static_cast<int64_t>(r) * C + k0 + kk);
+    Y[r * MP + kk] = zs.rd(static_cast<int64_t>(r) * C + mode(kk));
   }
   __syncthreads();
   auto sweep = [&](int32_t kk, int32_t first, int32_t count, int32_t dir, auto&& f) {
+    const int32_t k = mode(kk);
     for (int32_t c0 = 0; c0 < count; c0 += TB) {
       double y[TB], a[TB];
 #pragma unroll
       for (int q = 0; q < TB; ++q) {
         const int32_t r = first + dir * min(c0 + q, count - 1);
         y[q] = Y[r * MP + kk];
-        a[q] = inv[static_cast<int64_t>(r) * CP + k0 + kk];
+        a[q] = inv[static_cast<int64_t>(r) * CP + k];
       }
 #pragma unroll
       for (int q = 0; q < TB; ++q)
@@ -899,17 +917,31 @@ __device__ __forceinline__ void lattice_tridiag_parts(const DevProblem& P, const
     bp[b * MP + kk] = pr;
   }
   __syncthreads();
-  // backward carries, and the rows back to the face vector
+  // backward carries (the last block has none)
   for (int32_t e = tid; e < B * nm; e += nt) {
     const int32_t b = e / nm, kk = e - b * nm, r0 = b * m, n = min(m, R - r0);
-    if (n <= 0) continue;
-    const bool last = b == B - 1;
-    const double cin = last ? 0.0 : carry(bv, bp, b, kk, -1);
+    if (n <= 0 || b == B - 1) continue;
+    const double cin = carry(bv, bp, b, kk, -1);
     double pr = 1.0;
     sweep(kk, r0 + n - 1, n, -1, [&](int32_t r, double x, double iv) {
       pr = pr * (g * iv);
-      zs.wr(static_cast<int64_t>(r) * C + k0 + kk, last ? x : __builtin_fma(pr, cin, x));
+      Y[r * MP + kk] = __builtin_fma(pr, cin, x);
     });
+  }
+  __syncthreads();
+  // the pairs' folded rows into the global store's blocked layout
+  for (int32_t e = tid; e < R * nh; e += nt) {
+    const int32_t r = e / nh, j = e - r * nh, i = i0 + j;
+    const double x = Y[r * MP + j];
+    double a = x, b = 0.0;
+    if (2 * i + 1 != C) {
+      const double y = Y[r * MP + nh + j];
+      a = x + y;
+      b = x - y;
+    }
+    const int64_t at = (static_cast<int64_t>(i >> 2) * R + r) * 4 + (i & 3);
+    zs.wr(e0 + at, a);
+    zs.wr(o0 + at, b);
   }
 }
 
@@ -1003,14 +1035,19 @@ __device__ __forceinline__ void lattice_tridiag(const DevProblem& P, const Z& zs
 
 template <class Z>
 __device__ __forceinline__ void solve_lattice(const DevProblem& P, const Z& zs) {
+  // a split column's part-local tridiagonal solve also writes the second fold
+  bool folded = false;
+  if constexpr (std::is_same<Z, SplitStore>::value) folded = P.lat_mp != 0;
   lattice_fold(P, zs);
   zs.barrier();
   lattice_transform(P, zs);
   zs.barrier();
   lattice_tridiag(P, zs);
   zs.barrier();
-  lattice_fold(P, zs);
-  zs.barrier();
+  if (!folded) {
+    lattice_fold(P, zs);
+    zs.barrier();
+  }
   lattice_transform(P, zs);
   zs.barrier();
 }
@@ -2533,7 +2570,7 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       // when a part's modes and block ends fit
       const int32_t np = parts_of(d);
       if (np > 1) {
-        const int32_t mp = (C + np - 1) / np;
+        const int32_t mp = 2 * ((H + np - 1) / np);  // mode pairs (i, C-1-i) per part, two slots each
         const int32_t bp = std::max(1, std::min(R, JJA_LATTICE_TRI_THREADS / mp));
         const int64_t need = (static_cast<int64_t>(R) * mp + 4 * static_cast<int64_t>(bp) * mp) * 8;
         if (need <= kSplitLdsMax) {

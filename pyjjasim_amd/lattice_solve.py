@@ -145,13 +145,14 @@ class LatticeSolve:
         return max(1, min(self.R, TRI_THREADS * parts // self.C))
 
     def part_local(self, parts):
-        """(modes per part, blocks per mode) of the part-local tridiagonal
+        """(mode slots per part, blocks per mode) of the part-local tridiagonal
         solve a split column runs in each workgroup's LDS (all R rows of
-        ceil(C / parts) modes and 4 block-end arrays), or None when it does
-        not fit (the column-wide solve then runs)."""
+        ceil(H / parts) mode pairs (i, C-1-i), two slots each, and 4 block-end
+        arrays; holding both modes of a pair, the part also writes the second
+        fold), or None when it does not fit (the column-wide solve then runs)."""
         if parts <= 1:
             return None
-        mp = -(-self.C // parts)
+        mp = 2 * -(-self.H // parts)
         bp = max(1, min(self.R, TRI_THREADS // mp))
         need = 8 * (self.R * mp + 4 * bp * mp)
         return (mp, bp) if need <= SPLIT_LDS_MAX else None
