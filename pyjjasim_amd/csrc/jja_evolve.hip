@@ -586,38 +586,6 @@ constexpr int kFoldBatch = 4;
 // lat_t0 + R*EF + the same): a matrix-core operand, 16 rows x 4 i, is 512
 // contiguous bytes, and a wave of the fold reads 32-byte runs of 16 rows.
 // Either way element e of the loop lands at lat_t0 + e and lat_t0 + R*EF + e.
-// The folded rows of the row tiles (32 rows) p, p + parts, ... of part p of a
-// split column, in the global store's blocked layout: a tile's elements of one
-// block of four i are 128 consecutive doubles (32 rows x 4 i).
-__device__ __forceinline__ void lattice_fold_rows(const DevProblem& P, const SplitStore& zs) {
-  const int32_t R = P.lat_rows, C = P.lat_cols, H = (C + 1) >> 1, EF = P.lat_ef;
-  const int32_t nt = static_cast<int32_t>(blockDim.x), tid = static_cast<int32_t>(threadIdx.x);
-  const int32_t part = __builtin_amdgcn_readfirstlane(zs.vt / nt), parts = zs.vbs / nt;
-  const int64_t e0 = P.lat_t0, o0 = P.lat_t0 + static_cast<int64_t>(R) * EF;
-  const int32_t nrt = (R + 31) >> 5;
-  for (int32_t rt = part; rt < nrt; rt += parts) {
-    for (int32_t e = tid; e < 32 * EF; e += nt) {
-      const int32_t ib = e >> 7, rem = e & 127;
-      const int32_t r = 32 * rt + (rem >> 2), i = 4 * ib + (rem & 3);
-      if (r >= R) continue;
-      double a = 0.0, b = 0.0;
-      if (i < H) {
-        const int64_t row = static_cast<int64_t>(r) * C;
-        const double x = zs.rd(row + i), y = zs.rd(row + (C - 1 - i));
-        if (2 * i + 1 == C) {
-          a = x;
-        } else {
-          a = x + y;
-          b = x - y;
-        }
-      }
-      const int64_t at = (static_cast<int64_t>(ib) * R + r) * 4 + (rem & 3);
-      zs.wr(e0 + at, a);
-      zs.wr(o0 + at, b);
-    }
-  }
-}
-
 template <class Z>
 __device__ __forceinline__ void lattice_fold(const DevProblem& P, const Z& zs) {
   constexpr int KB = Z::kGlobal ? kFoldBatch : 1;
@@ -722,34 +690,20 @@ __device__ __forceinline__ void lattice_transform_lds(const DevProblem& P, const
 // The matrix cores round differently from a chain of fused multiply-adds: the
 // two stores agree to rounding.
 constexpr int kMfmaBatch = 4;
-// part_rows (a split column): part p takes the row tiles p, p + parts, ...
-// with all their mode tiles (its own folded rows, written by
-// lattice_fold_rows), so only a workgroup barrier separates fold and
-// transform; each tile's sums are the same whichever wave forms them.
 template <class Z>
-__device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, const Z& zs, bool part_rows = false) {
+__device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, const Z& zs) {
   constexpr int U = kMfmaBatch;
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, EF = P.lat_ef, CP = P.lat_cp;
   const int lane = zs.vt & (kWave - 1);
-  int32_t wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
-  int32_t NW = zs.vbs / kWave;
+  const int32_t wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
+  const int32_t NW = zs.vbs / kWave;
   const int32_t n_even = (C + 1) >> 1, n_odd = C >> 1;
   const int32_t nrt = (R + 31) >> 5, nte = (n_even + 31) >> 5, nto = (n_odd + 31) >> 5;
-  const int32_t nmt = nte + nto;
-  int32_t tasks = nrt * nmt, rt0 = 0, rstep = 1;
-  if (part_rows) {
-    const int32_t nt = static_cast<int32_t>(blockDim.x);
-    rt0 = __builtin_amdgcn_readfirstlane(zs.vt / nt);
-    rstep = zs.vbs / nt;
-    wave = __builtin_amdgcn_readfirstlane((zs.vt - rt0 * nt) / kWave);
-    NW = nt / kWave;
-    tasks = rt0 < nrt ? ((nrt - 1 - rt0) / rstep + 1) * nmt : 0;
-  }
+  const int32_t tasks = nrt * (nte + nto);
   const int lr = lane & 15, lk = lane >> 4;
   const GAS double* __restrict__ S = gbl(P.lat_sine);
   for (int32_t task = wave; task < tasks; task += NW) {
-    const int32_t rt = part_rows ? rt0 + rstep * (task / nmt) : task % nrt;
-    const int32_t mt = part_rows ? task % nmt : task / nrt;
+    const int32_t rt = task % nrt, mt = task / nrt;
     const int par = mt < nte ? 0 : 1;
     const int32_t m0 = 32 * (par ? mt - nte : mt);
     const int32_t r0 = 32 * rt;
@@ -1083,18 +1037,10 @@ template <class Z>
 __device__ __forceinline__ void solve_lattice(const DevProblem& P, const Z& zs) {
   // a split column's part-local tridiagonal solve also writes the second fold
   bool folded = false;
-  if constexpr (std::is_same<Z, SplitStore>::value) {
-    folded = P.lat_mp != 0;
-    // each part folds and transforms its own row tiles: a workgroup barrier
-    // between the two suffices
-    lattice_fold_rows(P, zs);
-    zs.wg_barrier();
-    lattice_transform_mfma(P, zs, true);
-  } else {
-    lattice_fold(P, zs);
-    zs.barrier();
-    lattice_transform(P, zs);
-  }
+  if constexpr (std::is_same<Z, SplitStore>::value) folded = P.lat_mp != 0;
+  lattice_fold(P, zs);
+  zs.barrier();
+  lattice_transform(P, zs);
   zs.barrier();
   lattice_tridiag(P, zs);
   zs.barrier();
