@@ -171,3 +171,21 @@ def test_gpu_split_lattice_adjoint_matches_factor_path():
             grads[solver] = [t.grad.cpu().numpy() for t in leaves]
     for a, b in zip(grads["lattice"], grads["cholesky"]):
         assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()
+
+
+def test_gpu_split_lattice_with_idle_parts_matches_oracle():
+    """A narrow lattice (5 face columns, 3 mode pairs including the middle mode)
+    split over 4 parts: the last part holds no mode pair and only meets the
+    column barriers; trajectories match the oracle to 1e-9."""
+    c = pa.SquareArray(30, 6)
+    Nt, kw = 200, _inputs(c, W=2)
+    ts = [99, Nt - 1]
+    plan = CircuitPlan(c, 0.05, block_size=256, face_store="global", parts=4)
+    assert plan.uses_lattice and plan.lattice.C == 5 and plan.lattice.part_local(4) == (2, 29)
+    print("[lattice] split narrow lattice, 4 parts", flush=True)
+    res = pa.TimeEvolutionProblem(c, time_step_count=Nt, store_time_steps=ts, solver="lattice", block_size=256,
+                                  face_store="global", parts=4, temperature=0.0, **kw).compute()
+    ref = to.evolve(c, dt=0.05, Nt=Nt, f=0.21, Is=kw["current_sources"], store_time_steps=ts)
+    th = res.get_theta()
+    assert np.abs(th - ref["theta"]).max() <= 1e-9 * np.abs(ref["theta"]).max()
+    np.testing.assert_array_equal(res.get_vortex_configuration(), to.vortex(c.get_cycle_matrix(), ref["theta"]))
