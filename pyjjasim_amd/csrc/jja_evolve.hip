@@ -2050,12 +2050,34 @@ int upload_if_changed(T* dev, const T* host, size_t n, std::vector<T>& shadow, b
   return JJA_OK;
 }
 
-// The dynamic LDS limit of a kernel, set once per value.
+// Raise a kernel's dynamic LDS limit to at least lds bytes. The limit belongs
+// to the kernel on a device, not to a plan, so the largest value any plan has
+// asked for is kept process-wide and never lowered (a plan that set a smaller
+// one would otherwise break a later launch of another plan).
+int raise_lds_attr(const void* kern, int lds) {
+  static std::mutex mu;
+  static std::vector<std::pair<std::pair<int, const void*>, int>> limits;
+  int dev = 0;
+  JJA_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto& a : limits)
+    if (a.first.first == dev && a.first.second == kern) {
+      if (a.second >= lds) return JJA_OK;
+      JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      a.second = lds;
+      return JJA_OK;
+    }
+  JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  limits.push_back({{dev, kern}, lds});
+  return JJA_OK;
+}
+
+// The dynamic LDS limit of a kernel with a dynamic LDS face store.
 int set_lds_limit(Plan* plan, const void* kern, int lds) {
   for (auto& a : plan->lds_attr)
     if (a.first == kern) {
       if (a.second >= lds) return JJA_OK;
-      JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      if (int rc = raise_lds_attr(kern, lds)) return rc;
       a.second = lds;
       return JJA_OK;
     }
@@ -2064,7 +2086,7 @@ int set_lds_limit(Plan* plan, const void* kern, int lds) {
   // the solve reads LDS at the plan's absolute byte offsets: the dynamic LDS
   // array must start at address 0, i.e. the kernel must have no static LDS
   if (attr.sharedSizeBytes != 0) return fail(JJA_EUNSUPPORTED, "kernel with a dynamic LDS face store has static LDS");
-  JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  if (int rc = raise_lds_attr(kern, lds)) return rc;
   plan->lds_attr.push_back({kern, lds});
   return JJA_OK;
 }
@@ -2403,8 +2425,9 @@ int launch_cooperative(Plan* plan, const void* kern, int BS, int32_t n, hipStrea
   int per_cu = -1;
   for (const auto& a : plan->per_cu)
     if (a.first == kern) per_cu = a.second;
+  if (lds > 0)
+    if (int rc = raise_lds_attr(kern, lds)) return rc;
   if (per_cu < 0) {
-    if (lds > 0) JJA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     JJA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BS, static_cast<size_t>(lds)));
     plan->per_cu.push_back({kern, per_cu});
   }

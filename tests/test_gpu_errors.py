@@ -157,3 +157,30 @@ def test_gpu_split_differentiable_evolution_reports_give_up():
             grads.append((th.detach().cpu().numpy(), leaf.grad.cpu().numpy()))
     np.testing.assert_array_equal(grads[0][0], grads[1][0])
     np.testing.assert_array_equal(grads[0][1], grads[1][1])
+
+
+def test_gpu_plans_with_different_lds_interleave():
+    """The dynamic LDS limit belongs to the kernel, not to a plan: launching a
+    plan that needs little LDS between two launches of one that needs 157 KB
+    must not lower the limit under the larger plan."""
+    def prob(n, Nt=30):
+        c = pa.SquareArray(n, n)
+        Is = c.current_base(0.0)[:, None, None] * np.array([0.5, 1.1])[None, :, None]
+        return pa.TimeEvolutionProblem(c, time_step=0.05, time_step_count=Nt, external_flux=0.2,
+                                       current_sources=Is, block_size=1024, store_time_steps=[Nt - 1])
+    print("[errors] interleaved plans with different LDS", flush=True)
+    big, small = prob(100), prob(6)
+    eb = pa.TimeEvolutionBatch([big], block_size=1024).make_engine()
+    es = pa.TimeEvolutionBatch([small], block_size=1024).make_engine()
+    try:
+        eb.advance(10)
+        es.advance(10)
+        eb.advance(20)
+        es.advance(20)
+        th_big = eb.results()[0][0]
+        th_small = es.results()[0][0]
+    finally:
+        eb.close()
+        es.close()
+    np.testing.assert_array_equal(th_big[:, :, -1], big.compute().get_theta()[:, :, -1])
+    np.testing.assert_array_equal(th_small[:, :, -1], small.compute().get_theta()[:, :, -1])
