@@ -690,65 +690,68 @@ __device__ __forceinline__ void lattice_transform_lds(const DevProblem& P, const
 // The matrix cores round differently from a chain of fused multiply-adds: the
 // two stores agree to rounding.
 constexpr int kMfmaBatch = 4;
+constexpr int kMfmaRowBlocks = 2;   // 16-row blocks per wave tile
+constexpr int kMfmaModeBlocks = 2;  // 16-mode blocks per wave tile
 template <class Z>
 __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, const Z& zs) {
-  constexpr int U = kMfmaBatch;
+  constexpr int U = kMfmaBatch, NR = kMfmaRowBlocks, NC = kMfmaModeBlocks;
+  constexpr int TR = 16 * NR, TC = 16 * NC;  // tile rows, tile modes (of one parity)
   const int32_t R = P.lat_rows, C = P.lat_cols, ES = P.lat_es, EF = P.lat_ef, CP = P.lat_cp;
   const int lane = zs.vt & (kWave - 1);
   const int32_t wave = __builtin_amdgcn_readfirstlane(zs.vt / kWave);
   const int32_t NW = zs.vbs / kWave;
   const int32_t n_even = (C + 1) >> 1, n_odd = C >> 1;
-  const int32_t nrt = (R + 31) >> 5, nte = (n_even + 31) >> 5, nto = (n_odd + 31) >> 5;
+  const int32_t nrt = (R + TR - 1) / TR, nte = (n_even + TC - 1) / TC, nto = (n_odd + TC - 1) / TC;
   const int32_t tasks = nrt * (nte + nto);
   const int lr = lane & 15, lk = lane >> 4;
   const GAS double* __restrict__ S = gbl(P.lat_sine);
   for (int32_t task = wave; task < tasks; task += NW) {
     const int32_t rt = task % nrt, mt = task / nrt;
     const int par = mt < nte ? 0 : 1;
-    const int32_t m0 = 32 * (par ? mt - nte : mt);
-    const int32_t r0 = 32 * rt;
+    const int32_t m0 = TC * (par ? mt - nte : mt);
+    const int32_t r0 = TR * rt;
     // operand addresses of this lane (clamped into the lattice)
-    const GAS double* fa[2];
-    const GAS double* sb[2];
+    const GAS double* fa[NR];
+    const GAS double* sb[NC];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NR; ++b) {
       const int32_t r = min(r0 + 16 * b + lr, R - 1);
       fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * EF * R + 4 * r + lk;
-      const int32_t j = min(2 * (m0 + 16 * b + lr) + par, C - 1);
-      sb[b] = S + j;
     }
-    vdouble4 acc[2][2];
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int c = 0; c < NC; ++c) sb[c] = S + min(2 * (m0 + 16 * c + lr) + par, C - 1);
+    vdouble4 acc[NR][NC];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) acc[b][c] = vdouble4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < NR; ++b)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[b][c] = vdouble4{0.0, 0.0, 0.0, 0.0};
     for (int32_t i0 = 0; i0 < EF; i0 += 4 * U) {
-      double av[U][2], bv[U][2];
+      double av[U][NR], bv[U][NC];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t i4 = i0 + 4 * u;                     // this step's first i (a multiple of 4)
         const int32_t blk = min(i4, EF - 4) >> 2;
         const int32_t ic = min(i4 + lk, ES - 1);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          av[u][b] = fa[b][static_cast<int64_t>(blk) * R * 4];
-          bv[u][b] = sb[b][static_cast<int64_t>(ic) * CP];
+        for (int b = 0; b < NR; ++b) {
+          const double v = fa[b][static_cast<int64_t>(blk) * R * 4];
+          av[u][b] = i4 < EF ? v : 0.0;
         }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) av[u][b] = i4 < EF ? av[u][b] : 0.0;
+        for (int c = 0; c < NC; ++c) bv[u][c] = sb[c][static_cast<int64_t>(ic) * CP];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NR; ++b)
 #pragma unroll
-          for (int c = 0; c < 2; ++c)
+          for (int c = 0; c < NC; ++c)
             acc[b][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][b], bv[u][c], acc[b][c], 0, 0, 0);
     }
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NR; ++b)
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < NC; ++c) {
         const int32_t j = 2 * (m0 + 16 * c + lr) + par;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
