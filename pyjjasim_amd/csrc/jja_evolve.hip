@@ -139,6 +139,7 @@ struct DevProblem {
   int32_t lat_pad_;
   int64_t lat_t0;              // store offset of the folded rows (even: 16-byte loads)
   const double* lat_sine;      // [ES][CP]
+  const double* lat_sine_p;    // [ES][2][CP/2]: the same table, even and odd modes apart (matrix-core B operands)
   const double* lat_inv;       // [R][CP] inverse Thomas pivots
   double lat_g;
 };
@@ -683,7 +684,8 @@ __device__ __forceinline__ void lattice_transform_lds(const DevProblem& P, const
 // v_mfma_f64_16x16x4f64 over i in steps of 4; lane l holds A[i][k] =
 // F[r0 + l%16][i0 + l/16] and B[k][j] = S[i0 + l/16][2(m0 + l%16) + p], and
 // result register q of lane l is D[l/16 + 4q][l%16]. E and O are kept in
-// blocks of four i by row, so each A load is 512 contiguous bytes.
+// blocks of four i by row, so each A load is 512 contiguous bytes; B comes from
+// the table with even and odd modes apart (lat_sine_p), four 128-byte rows.
 // kMfmaBatch steps of loads are issued before their products. Rows and modes
 // past the lattice load clamped (finite) operands and are not stored; the
 // folded rows are zero for i >= H, and table rows past ES are read clamped.
@@ -704,7 +706,8 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
   const int32_t nrt = (R + TR - 1) / TR, nte = (n_even + TC - 1) / TC, nto = (n_odd + TC - 1) / TC;
   const int32_t tasks = nrt * (nte + nto);
   const int lr = lane & 15, lk = lane >> 4;
-  const GAS double* __restrict__ S = gbl(P.lat_sine);
+  const GAS double* __restrict__ S = gbl(P.lat_sine_p);
+  const int32_t half = CP >> 1;
   for (int32_t task = wave; task < tasks; task += NW) {
     const int32_t rt = task % nrt, mt = task / nrt;
     const int par = mt < nte ? 0 : 1;
@@ -719,7 +722,7 @@ __device__ __forceinline__ void lattice_transform_mfma(const DevProblem& P, cons
       fa[b] = zs.p + P.lat_t0 + static_cast<int64_t>(par) * EF * R + 4 * r + lk;
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sb[c] = S + min(2 * (m0 + 16 * c + lr) + par, C - 1);
+    for (int c = 0; c < NC; ++c) sb[c] = S + par * half + min(m0 + 16 * c + lr, half - 1);
     vdouble4 acc[NR][NC];
 #pragma unroll
     for (int b = 0; b < NR; ++b)
@@ -2609,6 +2612,14 @@ int jja_plan_create(int32_t n_problems, const jja_problem_desc* descs, jja_plan*
       dp.lat_t0 = static_cast<int64_t>(d.n_faces) + (d.n_faces & 1);
       dp.lat_g = d.lattice_g;
       if ((rc = upload(plan, d.lattice_sine, static_cast<size_t>(dp.lat_es) * dp.lat_cp, &dp.lat_sine))) break;
+      {
+        // mode j of row i at i*CP + (j & 1)*(CP/2) + j/2 (CP is a multiple of 14, so even)
+        const size_t cp = dp.lat_cp, half = cp / 2, n = static_cast<size_t>(dp.lat_es) * cp;
+        std::vector<double> split(n);
+        for (size_t i = 0; i < static_cast<size_t>(dp.lat_es); ++i)
+          for (size_t j = 0; j < cp; ++j) split[i * cp + (j & 1) * half + j / 2] = d.lattice_sine[i * cp + j];
+        if ((rc = upload(plan, split.data(), n, &dp.lat_sine_p))) break;
+      }
       if ((rc = upload(plan, d.lattice_inv, static_cast<size_t>(R) * dp.lat_cp, &dp.lat_inv))) break;
     }
     const size_t Nj = d.n_junctions, Nf = d.n_faces;
