@@ -28,7 +28,8 @@ Extensions (keyword-only, defaults keep reference behaviour):
   parts            Workgroups per column for circuits with a global face store:
                    None (default) picks the largest power of two up to 16 that
                    keeps all columns' workgroups resident on the device; 1 keeps
-                   one workgroup per column. Results do not depend on it.
+                   one workgroup per column. With the factor, results do not
+                   depend on it; the lattice solve's rounding does (1e-12).
   local_stages     Lowest stages of the factor schedule solved wave-locally (each
                    wave solves whole independent subtrees, with no barrier between
                    those stages; for a split column, the stages above them are
@@ -39,15 +40,17 @@ Extensions (keyword-only, defaults keep reference behaviour):
                    depend on it.
   column_pairs     True: split columns are worked two at a time by one set of
                    workgroups (half the workgroup size, twice the parts), which
-                   stream the solve program once for both. Needs a split column
-                   and workgroups of 512 or 1024 threads. Default False: on
+                   stream the solve program once for both. Needs a split column,
+                   workgroups of 512 or 1024 threads and the factor (with
+                   solver="auto" a lattice circuit then uses the factor). Default False: on
                    SquareArray(512,512) it measured the same speed at W=16 and
                    1% faster at W=32. Results do not depend on it.
   solver           How the face system K is solved each step: "auto" (default)
                    takes the lattice solve for a rectangular lattice with one
-                   junction weight per direction (no mutual inductance) and one
-                   workgroup per column -- a sine transform of every lattice
-                   row, one tridiagonal solve per sine mode, the inverse
+                   junction weight per direction (no mutual inductance) -- a
+                   sine transform of every lattice row (on the matrix cores for
+                   a global face store), one tridiagonal solve per sine mode
+                   (part by part in LDS for a split column), the inverse
                    transform -- and the staged Cholesky factor otherwise;
                    "lattice" requires the former, "cholesky" forces the latter.
                    Both are exact solves; results agree to rounding.
